@@ -1,0 +1,108 @@
+// Device-side building blocks shared by the gate-count kernels: exact Z[w] entries held as int4
+// (a, b, c, d = a + b w + c w^2 + d w^3, w = exp(i pi/4)), lane-group votes and shuffles for wave64,
+// and Montgomery arithmetic for F_p ranks.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace chz {
+
+constexpr uint32_t P_MOD = 998244353u;        // 119 * 2^23 + 1; w maps to 3^((p-1)/8)
+constexpr uint32_t P_NINV = 998244351u;       // -p^-1 mod 2^32
+
+struct DevGate {                               // S <- S g^-1 data of one candidate
+  int nq, L, mask, ik;
+  int q[3];
+  int pad;
+  int4 inv[64];                                // L x L inverse local matrix, row m, column lc
+};
+
+struct KSpec {                                 // device view of a spec (kernel argument)
+  int n, G, depth, standard, lite, tdiv, gk, nsym, ncuts, have_table;
+  int cutmask[16], cutsize[16], rad[16];
+  uint32_t wm[4];                              // Montgomery constants w^i * 2^32 mod p, i = 0..3
+  const DevGate *gates;
+  const uint8_t *bad2;                         // G x G
+  const uint32_t *bad3;                        // bits over G^3
+  const int4 *goal;                            // N x N
+  const int *cxtable;
+  const uint8_t *symperm;                      // nsym x N basis maps
+  const uint8_t *symconj;                      // nsym
+};
+
+// ---- lane groups: GS lanes (a power of two dividing 64) cooperate on one matrix ----
+template <int GS> __device__ __forceinline__ uint64_t gballot(bool p) {
+  const uint64_t b = __ballot(p);
+  if constexpr (GS == 64) return b;
+  else return (b >> ((__lane_id()) & ~(GS - 1))) & ((1ull << GS) - 1);
+}
+template <int GS> __device__ __forceinline__ bool gall(bool p) {
+  if constexpr (GS == 64) return gballot<GS>(p) == ~0ull;
+  else return gballot<GS>(p) == ((1ull << GS) - 1);
+}
+template <int GS> __device__ __forceinline__ bool gany(bool p) { return gballot<GS>(p) != 0; }
+
+template <int GS> __device__ __forceinline__ int4 gshfl(int4 v, int src) {
+  return make_int4(__shfl(v.x, src, GS), __shfl(v.y, src, GS), __shfl(v.z, src, GS), __shfl(v.w, src, GS));
+}
+template <int GS> __device__ __forceinline__ int4 gxor(int4 v, int m) {
+  return make_int4(__shfl_xor(v.x, m, GS), __shfl_xor(v.y, m, GS), __shfl_xor(v.z, m, GS), __shfl_xor(v.w, m, GS));
+}
+template <int GS, typename T> __device__ __forceinline__ T gsum(T v) {
+#pragma unroll
+  for (int m = 1; m < GS; m <<= 1) v += __shfl_xor(v, m, GS);
+  return v;
+}
+template <int GS, typename T> __device__ __forceinline__ T gor(T v) {
+#pragma unroll
+  for (int m = 1; m < GS; m <<= 1) v |= __shfl_xor(v, m, GS);
+  return v;
+}
+
+// ---- Z[w] entries ----
+__device__ __forceinline__ bool zzero(int4 v) { return !(v.x | v.y | v.z | v.w); }
+__device__ __forceinline__ bool zeq(int4 u, int4 v) { return u.x == v.x && u.y == v.y && u.z == v.z && u.w == v.w; }
+__device__ __forceinline__ bool zhalf(int4 v) { return !((v.x ^ v.z) & 1) && !((v.y ^ v.w) & 1); }   // sqrt2 | v
+__device__ __forceinline__ int4 zdown(int4 v) {          // v / sqrt2 (exact when zhalf)
+  return make_int4((int)(((long long)v.y - v.w) >> 1), (int)(((long long)v.x + v.z) >> 1),
+                   (int)(((long long)v.y + v.w) >> 1), (int)(((long long)v.z - v.x) >> 1));
+}
+__device__ __forceinline__ int4 zconj(int4 v) { return make_int4(v.x, -v.w, -v.z, -v.y); }
+
+// ---- F_p, Montgomery form with R = 2^32 ----
+__device__ __forceinline__ uint32_t mont_mul(uint32_t a, uint32_t b) {   // a < 2^32, b < p -> a b / R mod p
+  const uint64_t t = (uint64_t)a * b;
+  const uint32_t m = (uint32_t)t * P_NINV;
+  const uint64_t u = (t + (uint64_t)m * P_MOD) >> 32;
+  const uint32_t r = (uint32_t)u;
+  return r >= P_MOD ? r - P_MOD : r;
+}
+__device__ __forceinline__ uint32_t fp_sub(uint32_t a, uint32_t b) { return a >= b ? a - b : a + P_MOD - b; }
+__device__ __forceinline__ uint32_t fp_lift(int v) { return v < 0 ? (uint32_t)(v + 3 * (int64_t)P_MOD) : (uint32_t)v; }
+// image of a + b w + c w^2 + d w^3 in F_p
+__device__ __forceinline__ uint32_t fp_of(int4 v, const uint32_t *wm) {
+  uint32_t s = mont_mul(fp_lift(v.x), wm[0]) + mont_mul(fp_lift(v.y), wm[1]);
+  s = s >= P_MOD ? s - P_MOD : s;
+  s += mont_mul(fp_lift(v.z), wm[2]);
+  s = s >= P_MOD ? s - P_MOD : s;
+  s += mont_mul(fp_lift(v.w), wm[3]);
+  return s >= P_MOD ? s - P_MOD : s;
+}
+
+__device__ __forceinline__ int combine_bound(int h, int t, int c2, int supp) {   // src/cheese.cu:599-607
+  int best = 1 << 30;
+  const int hi = c2 > supp ? c2 : supp;
+  for (int c = c2; c <= hi; c++) {
+    const int one = (h + t > supp - 2 * c) ? h + t : supp - 2 * c;
+    if (c + one < best) best = c + one;
+  }
+  return best;
+}
+
+__device__ __forceinline__ uint64_t mix64(uint64_t x) {
+  x ^= x >> 32; x *= 0xd6e8feb86659fd93ull;
+  x ^= x >> 32; x *= 0xd6e8feb86659fd93ull;
+  return x ^ (x >> 32);
+}
+
+}  // namespace chz

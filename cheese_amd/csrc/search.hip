@@ -1,0 +1,984 @@
+// Gate-count search on one MI355X: per-level expansion, T bound, ordered compaction, LDS-staged LSD radix
+// sort, segmented exact dedupe and frontier gather, all hand-written HIP kernels for gfx950, driven by the
+// C-ABI in include/cheese_hip.h.  Reference behaviour: src/cheese.cu:1249-1377 (levels), :1762-1848 (run).
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <chrono>
+#include <cstdio>
+#include <cstring>
+#include <memory>
+#include <string>
+#include <vector>
+
+#include "../../include/cheese_hip.h"
+#include "search_kernels.h"
+#include "spec.h"
+
+namespace chz {
+
+// ------------------------------------------------------------------------------------------------------
+// counters of one level (device): indices into a u64 array
+enum { C_W2 = 0, C_W3, C_DEAD, C_FOUND, C_ALIVE, C_OVF, C_NEEDT, C_FIRST, C_RUNS, C_REPOVF, C_KEPT, C_N };
+
+struct ExpandArgs {
+  const int4 *res;          // parents' residuals, E entries each
+  const int *pk;            // parents' sqrt2 exponents
+  const uint8_t *g1, *g2;   // parents' last / second-to-last gates (255: none)
+  long long off;            // first parent of the chunk (index in the level)
+  long long ncc;            // children in the chunk
+  int t, keep;
+  uint8_t *status;          // [ncc]
+  uint64_t *keys;           // [ncc] class keys of kept children
+  uint2 *needt;             // (child in chunk, packed cheap bounds)
+  unsigned long long *cnt;
+};
+
+// pass 1: windows, child residual, overflow, goal, cheap bounds (src/cheese.cu:662-698, :959-1001)
+template <int NQ>
+__global__ __launch_bounds__(256) void k_expand(KSpec sp, ExpandArgs a) {
+  constexpr int N = 1 << NQ, E = N * N, GS = E, CPW = 64 / GS;
+  const int lane = threadIdx.x & 63, le = lane & (GS - 1), slot = lane / GS;
+  const long long wave = ((long long)blockIdx.x * blockDim.x + threadIdx.x) >> 6;
+  const long long nwaves = ((long long)gridDim.x * blockDim.x) >> 6;
+  const int rem = sp.depth - a.t;
+  unsigned long long n_w2 = 0, n_w3 = 0, n_dead = 0, n_found = 0, n_ovf = 0;
+  for (long long base = wave * CPW; base < a.ncc; base += nwaves * CPW) {
+    const long long cc = base + slot;
+    if (cc >= a.ncc) continue;
+    const long long pi = a.off + cc / sp.G;
+    const int g = (int)(cc % sp.G);
+    const int g1 = a.g1[pi], g2 = a.g2[pi];
+    uint8_t st = 255;
+    uint32_t info = 0;
+    if (g1 != 255 && sp.bad2[g1 * sp.G + g]) st = ST_W2;
+    else if (g2 != 255) {
+      const uint32_t tri = ((uint32_t)g2 * sp.G + g1) * sp.G + g;
+      if ((sp.bad3[tri >> 5] >> (tri & 31)) & 1) st = ST_W3;
+    }
+    if (st == 255) {
+      int4 s;
+      int k;
+      if (!make_child<NQ>(a.res + pi * E, a.pk[pi], sp.gates[g], le, s, k)) st = ST_OVF;
+      else if (k == sp.gk && gall<GS>(zeq(s, sp.goal[le]))) st = ST_FOUND;
+      else if (!sp.standard && !sp.lite) { st = ST_NEEDT; info = INFO_NOBOUND; }
+      else {
+        int h, supp = 0, c2 = 0;
+        if (sp.lite) h = k;
+        else {
+          const int hs = h_struct<NQ>(s);
+          h = k > hs ? k : hs;
+          supp = support<NQ>(s, le);
+          c2 = cx_bound<NQ>(s, le, sp);
+        }
+        if (combine_bound(h, 0, c2, supp) > rem) st = ST_DEAD;
+        else { st = ST_NEEDT; info = (uint32_t)h | ((uint32_t)c2 << 10) | ((uint32_t)supp << 20); }
+      }
+    }
+    if (le == 0) {
+      a.status[cc] = st;
+      switch (st) {
+        case ST_W2: n_w2++; break;
+        case ST_W3: n_w3++; break;
+        case ST_DEAD: n_dead++; break;
+        case ST_OVF: n_ovf++; break;
+        case ST_FOUND:
+          n_found++;
+          atomicMin(&a.cnt[C_FIRST], (unsigned long long)(a.off * sp.G + cc));
+          break;
+        case ST_NEEDT: {
+          const unsigned long long j = atomicAdd(&a.cnt[C_NEEDT], 1ull);
+          a.needt[j] = make_uint2((uint32_t)cc, info);
+          break;
+        }
+      }
+    }
+  }
+  n_w2 = gsum<64>(n_w2); n_w3 = gsum<64>(n_w3); n_dead = gsum<64>(n_dead);
+  n_found = gsum<64>(n_found); n_ovf = gsum<64>(n_ovf);
+  if (lane == 0) {
+    if (n_w2) atomicAdd(&a.cnt[C_W2], n_w2);
+    if (n_w3) atomicAdd(&a.cnt[C_W3], n_w3);
+    if (n_dead) atomicAdd(&a.cnt[C_DEAD], n_dead);
+    if (n_found) atomicAdd(&a.cnt[C_FOUND], n_found);
+    if (n_ovf) atomicAdd(&a.cnt[C_OVF], n_ovf);
+  }
+}
+
+// pass 2 on the children the cheap bounds kept: T bound, verdict, class key (src/cheese.cu:700-720)
+template <int NQ>
+__global__ __launch_bounds__(256) void k_tbound(KSpec sp, ExpandArgs a, long long nt) {
+  constexpr int N = 1 << NQ, E = N * N, GS = E, CPW = 64 / GS;
+  const int lane = threadIdx.x & 63, le = lane & (GS - 1), slot = lane / GS;
+  const long long wave = ((long long)blockIdx.x * blockDim.x + threadIdx.x) >> 6;
+  const long long nwaves = ((long long)gridDim.x * blockDim.x) >> 6;
+  const int rem = sp.depth - a.t;
+  unsigned long long n_dead = 0, n_alive = 0;
+  for (long long base = wave * CPW; base < nt; base += nwaves * CPW) {
+    const long long j = base + slot;
+    if (j >= nt) continue;
+    const uint2 ent = a.needt[j];
+    const long long cc = ent.x;
+    const long long pi = a.off + cc / sp.G;
+    const int g = (int)(cc % sp.G);
+    int4 s;
+    int k;
+    make_child<NQ>(a.res + pi * E, a.pk[pi], sp.gates[g], le, s, k);
+    bool alive = true;
+    if (!(ent.y & INFO_NOBOUND)) {
+      const int h = ent.y & 1023, c2 = (ent.y >> 10) & 1023, supp = (ent.y >> 20) & 1023;
+      const int T = (t_bound_any<NQ>(s, k, le) + sp.tdiv - 1) / sp.tdiv;
+      alive = combine_bound(h, T, c2, supp) <= rem;
+    }
+    uint64_t key = 0;
+    if (alive && a.keep) key = class_key<NQ>(s, k, le, sp);
+    if (le == 0) {
+      a.status[cc] = alive ? ST_ALIVE : ST_DEAD;
+      if (alive) { n_alive++; if (a.keep) a.keys[cc] = key; }
+      else n_dead++;
+    }
+  }
+  n_dead = gsum<64>(n_dead);
+  n_alive = gsum<64>(n_alive);
+  if (lane == 0) {
+    if (n_dead) atomicAdd(&a.cnt[C_DEAD], n_dead);
+    if (n_alive) atomicAdd(&a.cnt[C_ALIVE], n_alive);
+  }
+}
+
+// ------------------------------------------------------------------------------------------------------
+// ordered stream compaction: tiles of 256 threads x 16 items, counts -> one-block scan -> scatter
+constexpr int CT_THREADS = 256, CT_ITEMS = 16, CT_TILE = CT_THREADS * CT_ITEMS;
+
+struct FlagAlive {      // survivors of a chunk, in child order
+  const uint8_t *status;
+  __device__ bool operator()(long long i) const { return status[i] == ST_ALIVE; }
+};
+struct FlagKept {
+  const uint8_t *kept;
+  __device__ bool operator()(long long i) const { return kept[i] != 0; }
+};
+
+template <typename F>
+__global__ __launch_bounds__(CT_THREADS) void k_flag_count(F f, long long n, unsigned long long *bcount) {
+  const long long t0 = (long long)blockIdx.x * CT_TILE;
+  int c = 0;
+  for (int i = 0; i < CT_ITEMS; i++) {
+    const long long x = t0 + (long long)i * CT_THREADS + threadIdx.x;
+    if (x < n && f(x)) c++;
+  }
+  __shared__ int ws[CT_THREADS / 64];
+  c = gsum<64>(c);
+  if ((threadIdx.x & 63) == 0) ws[threadIdx.x >> 6] = c;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    unsigned long long s = 0;
+    for (int w = 0; w < CT_THREADS / 64; w++) s += ws[w];
+    bcount[blockIdx.x] = s;
+  }
+}
+
+// exclusive scan of v[0..n) in place by one block; v[n] receives the total
+__global__ __launch_bounds__(1024) void k_scan_one_block(unsigned long long *v, long long n) {
+  __shared__ unsigned long long ws[16];
+  __shared__ unsigned long long carry;
+  if (threadIdx.x == 0) carry = 0;
+  __syncthreads();
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+  for (long long base = 0; base < n; base += 1024) {
+    const long long i = base + threadIdx.x;
+    const unsigned long long x = i < n ? v[i] : 0;
+    unsigned long long inc = x;   // inclusive wave scan
+#pragma unroll
+    for (int d = 1; d < 64; d <<= 1) {
+      const unsigned long long y = __shfl_up(inc, d, 64);
+      if (lane >= d) inc += y;
+    }
+    if (lane == 63) ws[w] = inc;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+      unsigned long long s = carry;
+      for (int k = 0; k < 16; k++) { const unsigned long long t = ws[k]; ws[k] = s; s += t; }
+      carry = s;
+    }
+    __syncthreads();
+    if (i < n) v[i] = ws[w] + inc - x;
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) v[n] = carry;
+}
+
+// block-wide exclusive scan of one flag per thread, in thread order; returns the block total
+__device__ __forceinline__ int block_flag_scan(bool flag, int &excl, int *ws) {
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+  const uint64_t b = __ballot(flag);
+  const int below = __popcll(b & ((1ull << lane) - 1));
+  if (lane == 0) ws[w] = __popcll(b);
+  __syncthreads();
+  int off = 0, tot = 0;
+  for (int k = 0; k < CT_THREADS / 64; k++) { if (k < w) off += ws[k]; tot += ws[k]; }
+  __syncthreads();
+  excl = off + below;
+  return tot;
+}
+
+// survivors of a chunk -> level survivor list (child index in the level, class key)
+__global__ __launch_bounds__(CT_THREADS) void k_scatter_alive(FlagAlive f, long long n, const unsigned long long *boff,
+                                                              const uint64_t *keys, long long child0, long long out0,
+                                                              uint64_t *surv_o, uint64_t *surv_key) {
+  __shared__ int ws[CT_THREADS / 64];
+  const long long t0 = (long long)blockIdx.x * CT_TILE;
+  long long pos = out0 + (long long)boff[blockIdx.x];
+  for (int i = 0; i < CT_ITEMS; i++) {
+    const long long x = t0 + (long long)i * CT_THREADS + threadIdx.x;
+    const bool fl = x < n && f(x);
+    int ex;
+    const int tot = block_flag_scan(fl, ex, ws);
+    if (fl) { surv_o[pos + ex] = (uint64_t)(child0 + x); surv_key[pos + ex] = keys[x]; }
+    pos += tot;
+  }
+}
+
+// kept survivors -> ordered list of survivor ranks
+__global__ __launch_bounds__(CT_THREADS) void k_scatter_kept(FlagKept f, long long n, const unsigned long long *boff,
+                                                             uint32_t *out) {
+  __shared__ int ws[CT_THREADS / 64];
+  const long long t0 = (long long)blockIdx.x * CT_TILE;
+  long long pos = (long long)boff[blockIdx.x];
+  for (int i = 0; i < CT_ITEMS; i++) {
+    const long long x = t0 + (long long)i * CT_THREADS + threadIdx.x;
+    const bool fl = x < n && f(x);
+    int ex;
+    const int tot = block_flag_scan(fl, ex, ws);
+    if (fl) out[pos + ex] = (uint32_t)x;
+    pos += tot;
+  }
+}
+
+// ------------------------------------------------------------------------------------------------------
+// LSD radix sort of (u32 key, u32 value) pairs, 8-bit digits, stable: tiles of 256 threads x 8 rounds
+constexpr int RS_THREADS = 256, RS_ROUNDS = 8, RS_TILE = RS_THREADS * RS_ROUNDS;
+
+__global__ __launch_bounds__(RS_THREADS) void k_radix_hist(const uint32_t *keys, long long n, int shift,
+                                                           unsigned long long *hist, long long nblocks) {
+  __shared__ unsigned int h[256];
+  h[threadIdx.x] = 0;
+  __syncthreads();
+  const long long t0 = (long long)blockIdx.x * RS_TILE;
+  for (int r = 0; r < RS_ROUNDS; r++) {
+    const long long x = t0 + (long long)r * RS_THREADS + threadIdx.x;
+    if (x < n) atomicAdd(&h[(keys[x] >> shift) & 255], 1u);
+  }
+  __syncthreads();
+  hist[(long long)threadIdx.x * nblocks + blockIdx.x] = h[threadIdx.x];
+}
+
+__global__ __launch_bounds__(RS_THREADS) void k_radix_scatter(const uint32_t *kin, const uint32_t *vin, long long n,
+                                                              int shift, const unsigned long long *hist,
+                                                              long long nblocks, uint32_t *kout, uint32_t *vout) {
+  __shared__ unsigned long long base[256];
+  __shared__ unsigned int wc[RS_THREADS / 64][256];
+  const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
+  base[tid] = hist[(long long)tid * nblocks + blockIdx.x];
+  for (int k = 0; k < RS_THREADS / 64; k++) wc[k][tid] = 0;
+  __syncthreads();
+  const long long t0 = (long long)blockIdx.x * RS_TILE;
+  for (int r = 0; r < RS_ROUNDS; r++) {
+    const long long x = t0 + (long long)r * RS_THREADS + tid;
+    const bool valid = x < n;
+    const uint32_t key = valid ? kin[x] : 0u;
+    const uint32_t val = valid ? vin[x] : 0u;
+    const int dg = (key >> shift) & 255;
+    uint64_t peers = __ballot(valid);
+#pragma unroll
+    for (int b = 0; b < 8; b++) {
+      const uint64_t bb = __ballot((dg >> b) & 1);
+      peers &= ((dg >> b) & 1) ? bb : ~bb;
+    }
+    const int rank = __popcll(peers & ((1ull << lane) - 1));
+    if (valid && rank == 0) wc[w][dg] = __popcll(peers);
+    __syncthreads();
+    unsigned int before = 0;
+    if (valid) for (int k = 0; k < w; k++) before += wc[k][dg];
+    if (valid) {
+      const unsigned long long p = base[dg] + before + rank;
+      kout[p] = key;
+      vout[p] = val;
+    }
+    __syncthreads();
+    unsigned int tot = 0;
+    for (int k = 0; k < RS_THREADS / 64; k++) { tot += wc[k][tid]; wc[k][tid] = 0; }
+    base[tid] += tot;
+    __syncthreads();
+  }
+}
+
+// ------------------------------------------------------------------------------------------------------
+// segmented dedupe over the sorted survivors: runs of equal 32-bit key prefix are partitioned into exact
+// classes, keeping the smallest survivor rank (= lexicographically first prefix) of each class
+// (src/cheese.cu:1334-1343, :1046-1065)
+__global__ void k_run_heads(const uint32_t *k, const uint32_t *v, long long n, uint8_t *kept, uint32_t *runs,
+                            unsigned long long *cnt) {
+  for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (long long)gridDim.x * blockDim.x) {
+    kept[v[i]] = 1;
+    const bool head = i == 0 || k[i] != k[i - 1];
+    if (head && i + 1 < n && k[i + 1] == k[i]) runs[atomicAdd(&cnt[C_RUNS], 1ull)] = (uint32_t)i;
+  }
+}
+
+struct DedupeArgs {
+  const uint32_t *k, *v;          // sorted key prefixes and survivor ranks
+  long long n;
+  const uint64_t *surv_o, *surv_key;
+  const uint32_t *runs;
+  const int4 *res;                // parents' residuals
+  const int *pk;
+  uint8_t *kept;
+  unsigned long long *cnt;
+};
+constexpr int MAXREP = 32;
+
+template <int NQ>
+__global__ __launch_bounds__(256) void k_dedupe_runs(KSpec sp, DedupeArgs a) {
+  constexpr int N = 1 << NQ, E = N * N, GS = E, CPW = 64 / GS;
+  constexpr int GPB = 256 / GS;     // groups per block
+  __shared__ uint32_t rep_rank[GPB][MAXREP];
+  const int lane = threadIdx.x & 63, le = lane & (GS - 1), slot = lane / GS;
+  const int grp = (threadIdx.x >> 6) * CPW + slot;
+  const long long wave = ((long long)blockIdx.x * blockDim.x + threadIdx.x) >> 6;
+  const long long nwaves = ((long long)gridDim.x * blockDim.x) >> 6;
+  const long long nruns = (long long)a.cnt[C_RUNS];
+  for (long long base = wave * CPW; base < nruns; base += nwaves * CPW) {
+    const long long ri = base + slot;
+    if (ri >= nruns) continue;
+    const long long i0 = a.runs[ri];
+    const uint32_t key = a.k[i0];
+    int nrep = 0;
+    for (long long j = i0; j < a.n && a.k[j] == key; j++) {
+      const uint32_t rank = a.v[j];
+      const uint64_t o = a.surv_o[rank], key64 = a.surv_key[rank];
+      const long long pi = (long long)(o / (uint64_t)sp.G);
+      const int g = (int)(o % (uint64_t)sp.G);
+      int4 s;
+      int ks;
+      make_child<NQ>(a.res + pi * E, a.pk[pi], sp.gates[g], le, s, ks);
+      bool dup = false;
+      for (int r = 0; r < nrep && !dup; r++) {
+        const uint32_t rr = rep_rank[grp][r];
+        if (a.surv_key[rr] != key64) continue;
+        const uint64_t ro = a.surv_o[rr];
+        const long long rpi = (long long)(ro / (uint64_t)sp.G);
+        int4 b;
+        int kb;
+        make_child<NQ>(a.res + rpi * E, a.pk[rpi], sp.gates[(int)(ro % (uint64_t)sp.G)], le, b, kb);
+        dup = same_class<NQ>(s, ks, b, kb, le, sp);
+      }
+      if (dup) { if (le == 0) a.kept[rank] = 0; }
+      else if (nrep < MAXREP) {
+        if (le == 0) rep_rank[grp][nrep] = rank;
+        nrep++;
+      } else if (le == 0) atomicAdd(&a.cnt[C_REPOVF], 1ull);
+      __builtin_amdgcn_wave_barrier();
+    }
+  }
+}
+
+struct GatherArgs {
+  const uint32_t *kidx;           // kept survivor ranks, in order
+  long long nk;
+  const uint64_t *surv_o;
+  const int4 *res;
+  const int *pk;
+  const uint8_t *g1;
+  int4 *res_next;
+  int *pk_next;
+  uint8_t *g1_next, *g2_next;
+  uint32_t *tree_par;
+  uint8_t *tree_gate;
+};
+
+// next frontier: tree node, window state and materialised residual of every kept child
+template <int NQ>
+__global__ __launch_bounds__(256) void k_gather(KSpec sp, GatherArgs a) {
+  constexpr int N = 1 << NQ, E = N * N, GS = E, CPW = 64 / GS;
+  const int lane = threadIdx.x & 63, le = lane & (GS - 1), slot = lane / GS;
+  const long long wave = ((long long)blockIdx.x * blockDim.x + threadIdx.x) >> 6;
+  const long long nwaves = ((long long)gridDim.x * blockDim.x) >> 6;
+  for (long long base = wave * CPW; base < a.nk; base += nwaves * CPW) {
+    const long long p = base + slot;
+    if (p >= a.nk) continue;
+    const uint64_t o = a.surv_o[a.kidx[p]];
+    const long long pi = (long long)(o / (uint64_t)sp.G);
+    const int g = (int)(o % (uint64_t)sp.G);
+    int4 s;
+    int k;
+    make_child<NQ>(a.res + pi * E, a.pk[pi], sp.gates[g], le, s, k);
+    a.res_next[p * E + le] = s;
+    if (le == 0) {
+      a.pk_next[p] = k;
+      a.g1_next[p] = (uint8_t)g;
+      a.g2_next[p] = a.g1[pi];
+      a.tree_par[p] = (uint32_t)pi;
+      a.tree_gate[p] = (uint8_t)g;
+    }
+  }
+}
+
+
+// 64-bit class keys -> sortable 32-bit prefixes and survivor ranks
+__global__ void k_init_sort(const uint64_t *key, long long n, uint32_t *k32, uint32_t *v32) {
+  for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (long long)gridDim.x * blockDim.x) {
+    k32[i] = (uint32_t)(key[i] >> 32);
+    v32[i] = (uint32_t)i;
+  }
+}
+
+// ------------------------------------------------------------------------------------------------------
+// host side
+
+static thread_local std::string g_err;
+void set_err(const std::string &s) { g_err = s; }
+const char *get_err() { return g_err.c_str(); }
+
+struct DeviceError {};
+
+#define HIPCHK(x)                                                                      \
+  do {                                                                                 \
+    hipError_t e_ = (x);                                                               \
+    if (e_ != hipSuccess) {                                                            \
+      ::chz::set_err(std::string("HIP error: ") + hipGetErrorString(e_) + " at " #x);  \
+      throw ::chz::DeviceError();                                                      \
+    }                                                                                  \
+  } while (0)
+
+template <typename T> struct DBuf {     // grow-only device array
+  T *p = nullptr;
+  size_t cap = 0;
+  void need(size_t n) {
+    if (n <= cap) return;
+    if (p) HIPCHK(hipFree(p));
+    p = nullptr;
+    const size_t c = std::max<size_t>(n, cap + cap / 2);
+    HIPCHK(hipMalloc((void **)&p, std::max<size_t>(c, 1) * sizeof(T)));
+    cap = c;
+  }
+  void release() { if (p) (void)hipFree(p); p = nullptr; cap = 0; }
+};
+
+static double now_s() { return std::chrono::duration<double>(std::chrono::steady_clock::now().time_since_epoch()).count(); }
+
+enum { K_EXPAND = 0, K_TBOUND, K_COMPACT, K_SORT, K_DEDUPE, K_GATHER, K_NFAM };
+static const char *k_names[K_NFAM] = {"expand", "tbound", "compact", "sort", "dedupe", "gather"};
+
+// HIP-event timing of kernel families on the context's stream; pairs are read after each level's sync
+struct EventLog {
+  std::vector<hipEvent_t> pool;
+  std::vector<std::pair<int, std::pair<hipEvent_t, hipEvent_t>>> open;
+  hipEvent_t take() {
+    if (pool.empty()) { hipEvent_t e; HIPCHK(hipEventCreate(&e)); return e; }
+    hipEvent_t e = pool.back(); pool.pop_back(); return e;
+  }
+  void begin(int fam, hipStream_t s) {
+    hipEvent_t a = take(), b = take();
+    HIPCHK(hipEventRecord(a, s));
+    open.push_back({fam, {a, b}});
+  }
+  void end(hipStream_t s) { HIPCHK(hipEventRecord(open.back().second.second, s)); }
+  void flush(double *ms, long *launches) {   // call after the stream synchronised
+    for (auto &o : open) {
+      float x = 0;
+      HIPCHK(hipEventElapsedTime(&x, o.second.first, o.second.second));
+      ms[o.first] += x;
+      launches[o.first]++;
+      pool.push_back(o.second.first);
+      pool.push_back(o.second.second);
+    }
+    open.clear();
+  }
+  void destroy() {
+    for (auto &o : open) { pool.push_back(o.second.first); pool.push_back(o.second.second); }
+    open.clear();
+    for (auto e : pool) (void)hipEventDestroy(e);
+    pool.clear();
+  }
+};
+
+}  // namespace chz
+
+struct chz_ctx {
+  chz::HostSpec hs;
+  chz::SymGroup grp;
+  chz_opts opts{};
+  int depth = 0;
+  int device = 0;
+  hipStream_t stream = nullptr;
+  chz::KSpec ks{};
+  // spec on the device
+  chz::DevGate *d_gates = nullptr;
+  uint8_t *d_bad2 = nullptr, *d_symperm = nullptr, *d_symconj = nullptr;
+  uint32_t *d_bad3 = nullptr;
+  int4 *d_goal = nullptr;
+  int *d_cx = nullptr;
+  // frontier (current and next generation) and the tree of all levels
+  chz::DBuf<int4> res[2];
+  chz::DBuf<int> pk[2];
+  chz::DBuf<uint8_t> g1[2], g2[2];
+  std::vector<uint32_t *> tree_par;
+  std::vector<uint8_t *> tree_gate;
+  // per-level work arrays
+  chz::DBuf<uint8_t> status, kept;
+  chz::DBuf<uint64_t> keys, surv_o, surv_key;
+  chz::DBuf<uint2> needt;
+  chz::DBuf<unsigned long long> cnt, bcount, hist;
+  chz::DBuf<uint32_t> sk[2], sv[2], runs, kidx;
+  // measurement
+  chz::EventLog ev;
+  long k_launch[chz::K_NFAM] = {};
+  double k_ms[chz::K_NFAM] = {}, k_work[chz::K_NFAM] = {};
+};
+
+namespace chz {
+
+static void free_tree(chz_ctx *c) {
+  for (auto p : c->tree_par) if (p) (void)hipFree(p);
+  for (auto p : c->tree_gate) if (p) (void)hipFree(p);
+  c->tree_par.clear();
+  c->tree_gate.clear();
+}
+
+static int grid_for(long long waves_needed, int max_blocks = 8192) {
+  long long blocks = (waves_needed + 3) / 4;   // 4 waves per 256-thread block
+  if (blocks < 1) blocks = 1;
+  return (int)std::min<long long>(blocks, max_blocks);
+}
+
+template <int NQ> struct Level {
+  static constexpr int N = 1 << NQ, E = N * N, CPW = 64 / E;
+
+  // stable sort of (key prefix, rank) pairs of n survivors; the result lands in sk[0]/sv[0]
+  static void sort_pairs(chz_ctx *c, long long n) {
+    const long long nb = (n + RS_TILE - 1) / RS_TILE;
+    c->hist.need((size_t)nb * 256 + 1);
+    for (int pass = 0; pass < 4; pass++) {
+      const int src = pass & 1, dst = src ^ 1;
+      hipLaunchKernelGGL(k_radix_hist, dim3((unsigned)nb), dim3(RS_THREADS), 0, c->stream, c->sk[src].p, n, pass * 8,
+                         c->hist.p, nb);
+      hipLaunchKernelGGL(k_scan_one_block, dim3(1), dim3(1024), 0, c->stream, c->hist.p, nb * 256);
+      hipLaunchKernelGGL(k_radix_scatter, dim3((unsigned)nb), dim3(RS_THREADS), 0, c->stream, c->sk[src].p,
+                         c->sv[src].p, n, pass * 8, c->hist.p, nb, c->sk[dst].p, c->sv[dst].p);
+    }
+  }
+
+  static void sync_counts(chz_ctx *c, unsigned long long *hc) {
+    HIPCHK(hipMemcpyAsync(hc, c->cnt.p, C_N * 8, hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(hipStreamSynchronize(c->stream));
+    c->ev.flush(c->k_ms, c->k_launch);
+  }
+
+  static int run(chz_ctx *c, chz_level_cb cb, void *user, chz_result *res) {
+    const KSpec &ks = c->ks;
+    const int G = ks.G, depth = c->depth;
+    const double t0 = now_s();
+    long long m = 1;
+    int cur = 0;
+    {   // level 0: the target itself
+      std::vector<int4> U(E);
+      for (int i = 0; i < E; i++) {
+        const Zq &z = c->hs.target[i];
+        U[i] = make_int4((int)z.a, (int)z.b, (int)z.c, (int)z.d);
+      }
+      c->res[0].need(E); c->pk[0].need(1); c->g1[0].need(1); c->g2[0].need(1);
+      const int k0 = c->hs.tk;
+      const uint8_t none = 255;
+      HIPCHK(hipMemcpyAsync(c->res[0].p, U.data(), E * sizeof(int4), hipMemcpyHostToDevice, c->stream));
+      HIPCHK(hipMemcpyAsync(c->pk[0].p, &k0, sizeof(int), hipMemcpyHostToDevice, c->stream));
+      HIPCHK(hipMemcpyAsync(c->g1[0].p, &none, 1, hipMemcpyHostToDevice, c->stream));
+      HIPCHK(hipMemcpyAsync(c->g2[0].p, &none, 1, hipMemcpyHostToDevice, c->stream));
+      HIPCHK(hipStreamSynchronize(c->stream));
+    }
+    free_tree(c);
+    c->tree_par.push_back(nullptr);
+    c->tree_gate.push_back(nullptr);
+    const long long chunk = c->opts.chunk > 0 ? c->opts.chunk : (1ll << 20);
+    c->cnt.need(C_N);
+    unsigned long long hc[C_N];
+    for (int t = 1; t <= depth; t++) {
+      const double tl = now_s();
+      const bool keep = t < depth;
+      const long long nc = m * G;
+      const long long maxcc = std::min<long long>(chunk, m) * G;
+      c->status.need(maxcc);
+      c->needt.need(maxcc);
+      if (keep) c->keys.need(maxcc);
+      std::vector<unsigned long long> zero(C_N, 0);
+      zero[C_FIRST] = ~0ull;
+      HIPCHK(hipMemcpyAsync(c->cnt.p, zero.data(), C_N * 8, hipMemcpyHostToDevice, c->stream));
+      long long nsurv = 0;
+      bool found = false;
+      for (long long off = 0; off < m; off += chunk) {
+        const long long cm = std::min(chunk, m - off), ncc = cm * G;
+        HIPCHK(hipMemsetAsync(c->cnt.p + C_NEEDT, 0, 8, c->stream));
+        const ExpandArgs ea{c->res[cur].p, c->pk[cur].p, c->g1[cur].p, c->g2[cur].p, off, ncc, t, keep ? 1 : 0,
+                            c->status.p, c->keys.p, c->needt.p, c->cnt.p};
+        c->ev.begin(K_EXPAND, c->stream);
+        hipLaunchKernelGGL(k_expand<NQ>, dim3(grid_for((ncc + CPW - 1) / CPW)), dim3(256), 0, c->stream, ks, ea);
+        c->ev.end(c->stream);
+        sync_counts(c, hc);
+        if (hc[C_OVF]) { set_err("residual coefficients exceed the storage type (int32)"); return 4; }
+        if (hc[C_FOUND]) found = true;
+        if (found) continue;          // the level ends with a circuit: only FOUND and overflow still count
+        const long long nt = (long long)hc[C_NEEDT];
+        if (nt) {
+          c->ev.begin(K_TBOUND, c->stream);
+          hipLaunchKernelGGL(k_tbound<NQ>, dim3(grid_for((nt + CPW - 1) / CPW)), dim3(256), 0, c->stream, ks, ea, nt);
+          c->ev.end(c->stream);
+          c->k_work[K_TBOUND] += (double)nt * (double)(E * (E * 28 + 4 * E * (2 * NQ) + 12 * E));
+        }
+        if (keep) {
+          const long long nb = (ncc + CT_TILE - 1) / CT_TILE;
+          c->bcount.need(nb + 1);
+          c->ev.begin(K_COMPACT, c->stream);
+          hipLaunchKernelGGL(k_flag_count<FlagAlive>, dim3((unsigned)nb), dim3(CT_THREADS), 0, c->stream,
+                             FlagAlive{c->status.p}, ncc, c->bcount.p);
+          hipLaunchKernelGGL(k_scan_one_block, dim3(1), dim3(1024), 0, c->stream, c->bcount.p, nb);
+          c->ev.end(c->stream);
+          unsigned long long tot = 0;
+          HIPCHK(hipMemcpyAsync(&tot, c->bcount.p + nb, 8, hipMemcpyDeviceToHost, c->stream));
+          HIPCHK(hipStreamSynchronize(c->stream));
+          if (tot) {
+            if ((size_t)(nsurv + tot) > c->surv_o.cap) {   // grow, keeping what earlier chunks wrote
+              DBuf<uint64_t> no, nk;
+              no.need(nsurv + tot);
+              nk.need(nsurv + tot);
+              if (nsurv) {
+                HIPCHK(hipMemcpyAsync(no.p, c->surv_o.p, nsurv * 8, hipMemcpyDeviceToDevice, c->stream));
+                HIPCHK(hipMemcpyAsync(nk.p, c->surv_key.p, nsurv * 8, hipMemcpyDeviceToDevice, c->stream));
+                HIPCHK(hipStreamSynchronize(c->stream));
+              }
+              c->surv_o.release();
+              c->surv_key.release();
+              c->surv_o = no;
+              c->surv_key = nk;
+            }
+            c->ev.begin(K_COMPACT, c->stream);
+            hipLaunchKernelGGL(k_scatter_alive, dim3((unsigned)nb), dim3(CT_THREADS), 0, c->stream,
+                               FlagAlive{c->status.p}, ncc, c->bcount.p, c->keys.p, off * G, nsurv, c->surv_o.p,
+                               c->surv_key.p);
+            c->ev.end(c->stream);
+            c->k_work[K_COMPACT] += (double)ncc + (double)tot * 24.0;
+            nsurv += (long long)tot;
+          }
+        }
+      }
+      sync_counts(c, hc);
+      if (found) {   // the lexicographically first circuit: walk the tree up from the first FOUND child
+        const unsigned long long o = hc[C_FIRST];
+        std::vector<int> seq(t);
+        seq[t - 1] = (int)(o % G);
+        uint32_t i = (uint32_t)(o / G);
+        for (int l = t - 1; l >= 1; l--) {
+          uint8_t gg;
+          uint32_t pp;
+          HIPCHK(hipMemcpy(&gg, c->tree_gate[l] + i, 1, hipMemcpyDeviceToHost));
+          HIPCHK(hipMemcpy(&pp, c->tree_par[l] + i, 4, hipMemcpyDeviceToHost));
+          seq[l - 1] = gg;
+          i = pp;
+        }
+        res->rc = 0;
+        res->len = t;
+        res->found = (long)hc[C_FOUND];
+        for (int j = 0; j < t; j++) res->circuit[j] = seq[j];
+        res->seconds = now_s() - t0;
+        return 0;
+      }
+      long long distinct = 0;
+      if (keep && nsurv > 0) {
+        const int nxt = cur ^ 1;
+        c->kept.need(nsurv);
+        c->kidx.need(nsurv);
+        if (c->opts.dedupe && nsurv > 1) {
+          for (int b = 0; b < 2; b++) { c->sk[b].need(nsurv); c->sv[b].need(nsurv); }
+          c->runs.need(nsurv);
+          c->ev.begin(K_SORT, c->stream);
+          hipLaunchKernelGGL(k_init_sort, dim3(grid_for(nsurv / 64 + 1)), dim3(256), 0, c->stream, c->surv_key.p, nsurv,
+                             c->sk[0].p, c->sv[0].p);
+          sort_pairs(c, nsurv);
+          c->ev.end(c->stream);
+          c->k_work[K_SORT] += (double)nsurv * (16.0 + 4 * (4.0 + 16.0));
+          c->ev.begin(K_DEDUPE, c->stream);
+          HIPCHK(hipMemsetAsync(c->cnt.p + C_RUNS, 0, 8, c->stream));
+          hipLaunchKernelGGL(k_run_heads, dim3(grid_for(nsurv / 64 + 1)), dim3(256), 0, c->stream, c->sk[0].p,
+                             c->sv[0].p, nsurv, c->kept.p, c->runs.p, c->cnt.p);
+          const DedupeArgs da{c->sk[0].p, c->sv[0].p, nsurv, c->surv_o.p, c->surv_key.p, c->runs.p,
+                              c->res[cur].p, c->pk[cur].p, c->kept.p, c->cnt.p};
+          hipLaunchKernelGGL(k_dedupe_runs<NQ>, dim3(grid_for(nsurv / CPW + 1, 4096)), dim3(256), 0, c->stream, ks, da);
+          c->ev.end(c->stream);
+          c->k_work[K_DEDUPE] += (double)nsurv * 9.0;
+        } else {
+          HIPCHK(hipMemsetAsync(c->kept.p, 1, nsurv, c->stream));
+        }
+        const long long nb = (nsurv + CT_TILE - 1) / CT_TILE;
+        c->bcount.need(nb + 1);
+        c->ev.begin(K_COMPACT, c->stream);
+        hipLaunchKernelGGL(k_flag_count<FlagKept>, dim3((unsigned)nb), dim3(CT_THREADS), 0, c->stream,
+                           FlagKept{c->kept.p}, nsurv, c->bcount.p);
+        hipLaunchKernelGGL(k_scan_one_block, dim3(1), dim3(1024), 0, c->stream, c->bcount.p, nb);
+        hipLaunchKernelGGL(k_scatter_kept, dim3((unsigned)nb), dim3(CT_THREADS), 0, c->stream, FlagKept{c->kept.p},
+                           nsurv, c->bcount.p, c->kidx.p);
+        c->ev.end(c->stream);
+        unsigned long long tot = 0;
+        HIPCHK(hipMemcpyAsync(&tot, c->bcount.p + nb, 8, hipMemcpyDeviceToHost, c->stream));
+        sync_counts(c, hc);
+        if (hc[C_REPOVF]) { set_err("a dedupe run holds more distinct classes than its buffer"); return 4; }
+        distinct = (long long)tot;
+        if (distinct) {
+          c->res[nxt].need((size_t)distinct * E);
+          c->pk[nxt].need(distinct);
+          c->g1[nxt].need(distinct);
+          c->g2[nxt].need(distinct);
+          uint32_t *tp = nullptr;
+          uint8_t *tg = nullptr;
+          HIPCHK(hipMalloc((void **)&tp, distinct * 4));
+          HIPCHK(hipMalloc((void **)&tg, distinct));
+          c->tree_par.push_back(tp);
+          c->tree_gate.push_back(tg);
+          const GatherArgs ga{c->kidx.p, distinct, c->surv_o.p, c->res[cur].p, c->pk[cur].p, c->g1[cur].p,
+                              c->res[nxt].p, c->pk[nxt].p, c->g1[nxt].p, c->g2[nxt].p, tp, tg};
+          c->ev.begin(K_GATHER, c->stream);
+          hipLaunchKernelGGL(k_gather<NQ>, dim3(grid_for((distinct + CPW - 1) / CPW)), dim3(256), 0, c->stream, ks, ga);
+          c->ev.end(c->stream);
+          c->k_work[K_GATHER] += (double)distinct * (E * 16.0 * 2.0 + 16.0);
+          HIPCHK(hipStreamSynchronize(c->stream));
+          c->ev.flush(c->k_ms, c->k_launch);
+        }
+        cur = nxt;
+      }
+      chz_level lv{};
+      lv.t = t;
+      lv.parents = (long)m;
+      lv.prefixes = (long)(nc - (long long)hc[C_W2]);
+      lv.cut_w3 = (long)hc[C_W3];
+      lv.cut_lb = (long)hc[C_DEAD];
+      lv.survivors = (long)hc[C_ALIVE];
+      lv.distinct = (long)distinct;
+      lv.proved_unsat = distinct ? 0 : 1;
+      lv.seconds = now_s() - tl;
+      if (cb) cb(&lv, user);
+      if (!distinct) { res->rc = 1; res->seconds = now_s() - t0; return 1; }
+      m = distinct;
+    }
+    res->rc = 1;   // depth 0 with a target that is not the goal: nothing to expand
+    res->seconds = now_s() - t0;
+    return 1;
+  }
+};
+
+static void upload_spec(chz_ctx *c) {
+  const HostSpec &s = c->hs;
+  const int N = s.N(), G = s.G();
+  std::vector<DevGate> dg(G);
+  for (int g = 0; g < G; g++) {
+    const HostGate &h = s.gates[g];
+    DevGate &d = dg[g];
+    std::memset(&d, 0, sizeof d);
+    d.nq = h.nq;
+    d.L = 1 << h.nq;
+    d.ik = h.ik;
+    for (int j = 0; j < h.nq; j++) { d.q[j] = h.q[j]; d.mask |= 1 << h.q[j]; }
+    for (int i = 0; i < d.L * d.L; i++)
+      d.inv[i] = make_int4((int)h.inv[i].a, (int)h.inv[i].b, (int)h.inv[i].c, (int)h.inv[i].d);
+  }
+  std::vector<int4> goal(N * N);
+  for (int i = 0; i < N * N; i++) goal[i] = make_int4((int)s.goal[i].a, (int)s.goal[i].b, (int)s.goal[i].c, (int)s.goal[i].d);
+  std::vector<int> cx = s.cxtable.empty() ? std::vector<int>{0} : s.cxtable;
+  auto up = [&](void **dst, const void *src, size_t bytes) {
+    HIPCHK(hipMalloc(dst, std::max<size_t>(bytes, 1)));
+    if (bytes) HIPCHK(hipMemcpy(*dst, src, bytes, hipMemcpyHostToDevice));
+  };
+  up((void **)&c->d_gates, dg.data(), dg.size() * sizeof(DevGate));
+  up((void **)&c->d_bad2, s.bad2.data(), s.bad2.size());
+  up((void **)&c->d_bad3, s.bad3.data(), s.bad3.size() * 4);
+  up((void **)&c->d_goal, goal.data(), goal.size() * sizeof(int4));
+  up((void **)&c->d_cx, cx.data(), cx.size() * 4);
+  up((void **)&c->d_symperm, c->grp.bperm.data(), c->grp.bperm.size());
+  up((void **)&c->d_symconj, c->grp.conj.data(), c->grp.conj.size());
+  KSpec &k = c->ks;
+  std::memset(&k, 0, sizeof k);
+  k.n = s.n; k.G = G; k.depth = c->depth; k.standard = s.standard; k.lite = s.lite; k.tdiv = s.tdiv;
+  k.gk = s.gk; k.nsym = c->grp.size(); k.ncuts = (int)s.cutmask.size(); k.have_table = !s.cxtable.empty();
+  for (int i = 0; i < k.ncuts; i++) { k.cutmask[i] = s.cutmask[i]; k.cutsize[i] = s.cutsize[i]; k.rad[i] = s.rad[i]; }
+  // Montgomery images of w^i: w = 3^((p-1)/8) mod p, times 2^32
+  auto mulm = [](uint64_t a, uint64_t b) { return (a * b) % P_MOD; };
+  uint64_t w = 1, b = 3, e = (P_MOD - 1) / 8;
+  while (e) { if (e & 1) w = mulm(w, b); b = mulm(b, b); e >>= 1; }
+  const uint64_t R = (1ull << 32) % P_MOD;
+  uint64_t wp = 1;
+  for (int i = 0; i < 4; i++) { k.wm[i] = (uint32_t)mulm(wp, R); wp = mulm(wp, w); }
+  k.gates = c->d_gates; k.bad2 = c->d_bad2; k.bad3 = c->d_bad3; k.goal = c->d_goal; k.cxtable = c->d_cx;
+  k.symperm = c->d_symperm; k.symconj = c->d_symconj;
+}
+
+}  // namespace chz
+
+// ------------------------------------------------------------------------------------------------------
+// C ABI
+
+extern "C" {
+
+const char *chz_last_error(void) { return chz::get_err(); }
+
+int chz_open(const char *spec_text, size_t len, const chz_opts *opts, chz_ctx **out) {
+  *out = nullptr;
+  std::unique_ptr<chz_ctx> c(new chz_ctx());
+  std::string err;
+  if (const int prc = chz::parse_spec(std::string(spec_text, len), c->hs, err)) { chz::set_err(err); return prc; }
+  chz_opts o{-1, 1, 1, 0, 0, 0};
+  if (opts) o = *opts;
+  c->opts = o;
+  c->depth = o.depth >= 0 ? o.depth : c->hs.depth;
+  if (c->depth > CHZ_DMAX) { chz::set_err("depth > 48"); return 3; }
+  if (c->hs.states) { chz::set_err("state targets are not supported by this build"); return 3; }
+  if (c->hs.n < 1 || c->hs.n > 3) { chz::set_err("n = " + std::to_string(c->hs.n) + " not built (1..3)"); return 3; }
+  if (c->hs.truncated) { chz::set_err("malformed spec"); return 2; }
+  if ((int)c->hs.target.size() != c->hs.N() * c->hs.N()) { chz::set_err("spec has no target"); return 2; }
+  c->grp = chz::build_symmetry(c->hs, o.sym != 0);
+  try {
+    c->device = o.device;
+    HIPCHK(hipSetDevice(o.device));
+    HIPCHK(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking));
+    chz::upload_spec(c.get());
+  } catch (chz::DeviceError &) {
+    return 5;
+  }
+  *out = c.release();
+  return 0;
+}
+
+int chz_run(chz_ctx *c, chz_level_cb cb, void *user, chz_result *out) {
+  chz_result tmp{};
+  chz_result *res = out ? out : &tmp;
+  std::memset(res, 0, sizeof *res);
+  const int N = c->hs.N(), E = N * N;
+  // depth 0: the target itself (src/cheese.cu:1794-1801)
+  for (int i = 0; i < E; i++) {
+    const chz::Zq &z = c->hs.target[i];
+    if (z.a != (int32_t)z.a || z.b != (int32_t)z.b || z.c != (int32_t)z.c || z.d != (int32_t)z.d) {
+      chz::set_err("target coefficients exceed int32");
+      return res->rc = 4;
+    }
+  }
+  {
+    chz::HostMat U;
+    U.e = c->hs.target;
+    U.k = c->hs.tk;
+    if (chz::host_equal(U, c->hs.goal, c->hs.gk)) { res->rc = 0; res->len = 0; res->found = 0; return 0; }
+  }
+  for (int f = 0; f < chz::K_NFAM; f++) { c->k_launch[f] = 0; c->k_ms[f] = 0; c->k_work[f] = 0; }
+  try {
+    HIPCHK(hipSetDevice(c->device));
+    int rc = 3;
+    switch (c->hs.n) {
+      case 1: rc = chz::Level<1>::run(c, cb, user, res); break;
+      case 2: rc = chz::Level<2>::run(c, cb, user, res); break;
+      case 3: rc = chz::Level<3>::run(c, cb, user, res); break;
+    }
+    res->rc = rc;
+    return rc;
+  } catch (chz::DeviceError &) {
+    return res->rc = 5;
+  }
+}
+
+int chz_num_gates(const chz_ctx *c) { return c->hs.G(); }
+int chz_num_qubits(const chz_ctx *c) { return c->hs.n; }
+int chz_depth(const chz_ctx *c) { return c->depth; }
+int chz_sym_size(const chz_ctx *c) { return c->grp.size(); }
+const char *chz_gate_name(const chz_ctx *c, int g) {
+  return g >= 0 && g < c->hs.G() ? c->hs.gates[g].name.c_str() : "";
+}
+
+int chz_profile(const chz_ctx *c, const char **names, double *seconds, int n) {
+  int j = 0;
+  for (int f = 0; f < chz::K_NFAM && j < n; f++, j++) {
+    if (names) names[j] = chz::k_names[f];
+    if (seconds) seconds[j] = c->k_ms[f] / 1000.0;
+  }
+  return j;
+}
+
+int chz_kernel_stats(const chz_ctx *c, chz_kstat *out, int n) {
+  int j = 0;
+  for (int f = 0; f < chz::K_NFAM && j < n; f++, j++) {
+    out[j].name = chz::k_names[f];
+    out[j].launches = c->k_launch[f];
+    out[j].ms = c->k_ms[f];
+    out[j].work = c->k_work[f];
+  }
+  return j;
+}
+
+int chz_verify(const chz_ctx *c, const int *circuit, int len) {
+  // product of the forward gate matrices, gate 1 first, compared exactly with the target (verify.py:20-33)
+  const int n = c->hs.n, N = 1 << n;
+  chz::HostMat P;
+  P.e.assign(N * N, chz::Zq{0, 0, 0, 0});
+  for (int i = 0; i < N; i++) P.e[i * N + i] = chz::Zq{1, 0, 0, 0};
+  for (int j = 0; j < len; j++) {
+    if (circuit[j] < 0 || circuit[j] >= c->hs.G()) return -1;
+    // P <- g P via (P^dag g^dag)^dag: apply g on the left with its forward matrix
+    const chz::HostGate &g = c->hs.gates[circuit[j]];
+    const int L = 1 << g.nq;
+    int mask = 0;
+    for (int q = 0; q < g.nq; q++) mask |= 1 << g.q[q];
+    std::vector<chz::Zq> out(N * N, chz::Zq{0, 0, 0, 0});
+    for (int r = 0; r < N; r++) {
+      int lr = 0;
+      for (int q = 0; q < g.nq; q++) lr |= ((r >> g.q[q]) & 1) << q;
+      const int base = r & ~mask;
+      for (int m = 0; m < L; m++) {
+        const chz::Zq gm = g.fwd[lr * L + m];
+        if (!(gm.a | gm.b | gm.c | gm.d)) continue;
+        int x = base;
+        for (int q = 0; q < g.nq; q++) if ((m >> q) & 1) x |= 1 << g.q[q];
+        for (int col = 0; col < N; col++) {
+          const chz::Zq v = P.e[x * N + col];
+          chz::Zq &o = out[r * N + col];
+          o.a += gm.a * v.a - gm.b * v.d - gm.c * v.c - gm.d * v.b;
+          o.b += gm.a * v.b + gm.b * v.a - gm.c * v.d - gm.d * v.c;
+          o.c += gm.a * v.c + gm.b * v.b + gm.c * v.a - gm.d * v.d;
+          o.d += gm.a * v.d + gm.b * v.c + gm.c * v.b + gm.d * v.a;
+        }
+      }
+    }
+    P.e.swap(out);
+    P.k += g.gk;
+    while (P.k > 0) {
+      bool all = true;
+      for (auto &z : P.e) if (((z.a - z.c) & 1) || ((z.b - z.d) & 1)) { all = false; break; }
+      if (!all) break;
+      for (auto &z : P.e) z = chz::Zq{(z.b - z.d) / 2, (z.a + z.c) / 2, (z.b + z.d) / 2, (z.c - z.a) / 2};
+      P.k--;
+    }
+  }
+  return chz::host_equal(P, c->hs.target, c->hs.tk) ? 1 : 0;
+}
+
+void chz_close(chz_ctx *c) {
+  if (!c) return;
+  (void)hipSetDevice(c->device);
+  if (c->stream) (void)hipStreamSynchronize(c->stream);
+  chz::free_tree(c);
+  for (int b = 0; b < 2; b++) {
+    c->res[b].release(); c->pk[b].release(); c->g1[b].release(); c->g2[b].release();
+    c->sk[b].release(); c->sv[b].release();
+  }
+  c->status.release(); c->kept.release(); c->keys.release(); c->surv_o.release(); c->surv_key.release();
+  c->needt.release(); c->cnt.release(); c->bcount.release(); c->hist.release(); c->runs.release(); c->kidx.release();
+  void *ptrs[] = {c->d_gates, c->d_bad2, c->d_symperm, c->d_symconj, c->d_bad3, c->d_goal, c->d_cx};
+  for (void *p : ptrs) if (p) (void)hipFree(p);
+  c->ev.destroy();
+  if (c->stream) (void)hipStreamDestroy(c->stream);
+  delete c;
+}
+
+}  // extern "C"

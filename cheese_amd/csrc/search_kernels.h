@@ -1,0 +1,221 @@
+// Per-child device routines of the gate-count search.  A lane group of GS = 4^n lanes holds one residual
+// matrix, lane le = r * N + c owning entry (r, c); for n = 3 a group is a whole wave64.
+#pragma once
+#include "dev.h"
+
+namespace chz {
+
+enum : uint8_t { ST_DEAD = 0, ST_ALIVE = 1, ST_FOUND = 2, ST_W2 = 3, ST_W3 = 4, ST_NEEDT = 5, ST_OVF = 6 };
+constexpr uint32_t INFO_NOBOUND = 1u << 31;    // spec without bounds: every child is kept
+
+// Child residual entry of lane le: (S g^-1)[r][c] = sum_m S[r][x_m] ginv[m][lc], then the group lowers the
+// sqrt2 exponent while every entry is divisible by sqrt2 (src/cheese.cu:146-174, :773-817).  Overflow is
+// judged before normalisation, where the reference narrows to int32.  Returns false on overflow.
+template <int NQ>
+__device__ __forceinline__ bool make_child(const int4 *__restrict__ par, int pk, const DevGate &g, int le, int4 &s,
+                                           int &k) {
+  constexpr int N = 1 << NQ, GS = N * N;
+  const int r = le >> NQ, c = le & (N - 1);
+  const int nq = g.nq, L = g.L, mask = g.mask;
+  const int q0 = g.q[0], q1 = g.q[1], q2 = g.q[2];
+  int lc = (c >> q0) & 1;
+  if (nq > 1) lc |= ((c >> q1) & 1) << 1;
+  if (nq > 2) lc |= ((c >> q2) & 1) << 2;
+  const int base = c & ~mask;
+  long long a = 0, b = 0, cc = 0, d = 0;
+  for (int m = 0; m < L; m++) {
+    const int4 gm = g.inv[m * L + lc];
+    if (zzero(gm)) continue;
+    int x = base | ((m & 1) << q0);
+    if (nq > 1) x |= ((m >> 1) & 1) << q1;
+    if (nq > 2) x |= ((m >> 2) & 1) << q2;
+    const int4 v = par[r * N + x];
+    a += (long long)v.x * gm.x - (long long)v.y * gm.w - (long long)v.z * gm.z - (long long)v.w * gm.y;
+    b += (long long)v.x * gm.y + (long long)v.y * gm.x - (long long)v.z * gm.w - (long long)v.w * gm.z;
+    cc += (long long)v.x * gm.z + (long long)v.y * gm.y + (long long)v.z * gm.x - (long long)v.w * gm.w;
+    d += (long long)v.x * gm.w + (long long)v.y * gm.z + (long long)v.z * gm.y + (long long)v.w * gm.x;
+  }
+  const bool of = a != (int)a || b != (int)b || cc != (int)cc || d != (int)d;
+  s = make_int4((int)a, (int)b, (int)cc, (int)d);
+  k = pk + g.ik;
+  if (gany<GS>(of)) return false;
+  while (k > 0 && gall<GS>(zhalf(s))) { s = zdown(s); k--; }
+  return true;
+}
+
+// 2 if the matrix is monomial with a column permutation that is not affine over GF(2) (src/cheese.cu:321-338)
+template <int NQ> __device__ __forceinline__ int h_struct(int4 s) {
+  constexpr int N = 1 << NQ, GS = N * N;
+  const uint64_t b = gballot<GS>(!zzero(s));
+  uint64_t col0 = 0;
+#pragma unroll
+  for (int r = 0; r < N; r++) col0 |= 1ull << (r * N);
+  int pi[N];
+#pragma unroll
+  for (int c = 0; c < N; c++) {
+    const uint64_t cb = (b >> c) & col0;
+    if (__popcll(cb) != 1) return 0;
+    pi[c] = (__ffsll((long long)cb) - 1) / N;
+  }
+#pragma unroll
+  for (int j = 0; j < N; j++) {
+    int x = pi[0];
+#pragma unroll
+    for (int i = 0; i < NQ; i++) if ((j >> i) & 1) x ^= pi[1 << i] ^ pi[0];
+    if (pi[j] != x) return 2;
+  }
+  return 0;
+}
+
+// number of qubits the matrix acts on non-trivially (src/cheese.cu:305-319)
+template <int NQ> __device__ __forceinline__ int support(int4 s, int le) {
+  constexpr int N = 1 << NQ, GS = N * N;
+  const int r = le >> NQ, c = le & (N - 1);
+  int cnt = 0;
+#pragma unroll
+  for (int q = 0; q < NQ; q++) {
+    const int bit = 1 << q;
+    const int4 o = gxor<GS>(s, bit * N + bit);
+    bool viol = false;
+    if ((r ^ c) & bit) viol = !zzero(s);
+    else if (!((r | c) & bit)) viol = !zeq(s, o);
+    cnt += gany<GS>(viol);
+  }
+  return cnt;
+}
+
+__device__ __forceinline__ int spread_bits(int v, int mask, int n) {   // bit j of v -> j-th set bit of mask
+  int out = 0, j = 0;
+  for (int q = 0; q < n; q++)
+    if ((mask >> q) & 1) { out |= ((v >> j) & 1) << q; j++; }
+  return out;
+}
+
+// rank over F_p of the operator-Schmidt matrix across cut A (|A| = sa), R = 4^sa rows, stopping at `stop`
+// (src/cheese.cu:270-303).  Lane le holds matrix entry (le / C, le % C); fraction-free elimination keeps
+// every row a nonzero multiple of an F_p row combination, so the rank is that of the matrix mod p.
+template <int NQ> __device__ __forceinline__ int cut_rank(uint32_t fp, int le, int A, int sa, int stop) {
+  constexpr int N = 1 << NQ, GS = N * N;
+  const int R = 1 << (2 * sa), C = GS / R;
+  const int mrow = le / C, mcol = le % C;
+  const int nb = NQ - sa;
+  const int ia = mrow & ((1 << sa) - 1), oa = mrow >> sa, ib = mcol & ((1 << nb) - 1), ob = mcol >> nb;
+  const int notA = ((1 << NQ) - 1) & ~A;
+  const int o = spread_bits(oa, A, NQ) | spread_bits(ob, notA, NQ);
+  const int i = spread_bits(ia, A, NQ) | spread_bits(ib, notA, NQ);
+  uint32_t v = __shfl(fp, o * N + i, GS);
+  int rank = 0;
+  for (int c = 0; c < C && rank < stop; c++) {
+    const uint64_t bb = gballot<GS>(mcol == c && mrow >= rank && v != 0);
+    if (!bb) continue;
+    const int prow = (__ffsll((long long)bb) - 1) / C;
+    if (prow != rank) {
+      const int sr = mrow == prow ? rank : mrow == rank ? prow : mrow;
+      v = __shfl(v, sr * C + mcol, GS);
+    }
+    const uint32_t pv = __shfl(v, rank * C + c, GS);
+    const uint32_t fr = __shfl(v, mrow * C + c, GS);
+    const uint32_t pr = __shfl(v, rank * C + mcol, GS);
+    if (mrow > rank) v = fp_sub(mont_mul(v, pv), mont_mul(fr, pr));
+    rank++;
+  }
+  return rank;
+}
+
+// CX/CZ lower bound from the cut ranks and the ILP table (src/cheese.cu:585-597)
+template <int NQ> __device__ __forceinline__ int cx_bound(int4 s, int le, const KSpec &sp) {
+  const uint32_t fp = fp_of(s, sp.wm);
+  int idx = 0, mult = 1, mx = 0;
+  for (int i = 0; i < sp.ncuts; i++) {
+    const int rad = sp.rad[i], sa = sp.cutsize[i];
+    const int R = 1 << (2 * sa);
+    int stop = R;
+    if (rad >= 2 && rad - 2 < 30 && (1 << (rad - 2)) + 1 < stop) stop = (1 << (rad - 2)) + 1;
+    const int rk = cut_rank<NQ>(fp, le, sp.cutmask[i], sa, stop);
+    int need = 0;
+    while ((1 << need) < rk) need++;
+    if (need > rad - 1) need = rad - 1;
+    idx += need * mult;
+    mult *= rad;
+    if (need > mx) mx = need;
+  }
+  return sp.have_table ? sp.cxtable[idx] : mx;
+}
+
+// T lower bound: max over Pauli pairs of 2(n+k) - v(X_ab), all X_ab by one 2-D Walsh-Hadamard transform
+// per flip pair f = (fa, fb) of Q_f[e] = S[e ^ f] conj(S[e]), arithmetic in Z[w] mod 2^bits(U)
+// (src/cheese.cu:397-449).  The minimum valuation over all transforms is read from two OR accumulators:
+// O = OR of all coefficients, D = OR of (a ^ c) | (b ^ d); min v = 2 m + [bit m of D clear], m = ctz(O).
+template <int NQ, typename U> __device__ __forceinline__ int t_bound(int4 s, int k, int le) {
+  constexpr int N = 1 << NQ, GS = N * N;
+  const U ya = (U)(long long)s.x, yb = (U)0 - (U)(long long)s.w, yc = (U)0 - (U)(long long)s.z,
+          yd = (U)0 - (U)(long long)s.y;
+  U O = 0, D = 0;
+  for (int f = 0; f < GS; f++) {
+    const int4 xs = gxor<GS>(s, f);
+    const U xa = (U)(long long)xs.x, xb = (U)(long long)xs.y, xc = (U)(long long)xs.z, xd = (U)(long long)xs.w;
+    U qa = U(xa * ya - xb * yd - xc * yc - xd * yb);
+    U qb = U(xa * yb + xb * ya - xc * yd - xd * yc);
+    U qc = U(xa * yc + xb * yb + xc * ya - xd * yd);
+    U qd = U(xa * yd + xb * yc + xc * yb + xd * ya);
+#pragma unroll
+    for (int h = 1; h < GS; h <<= 1) {
+      const U oa = __shfl_xor(qa, h, GS), ob = __shfl_xor(qb, h, GS), oc = __shfl_xor(qc, h, GS),
+              od = __shfl_xor(qd, h, GS);
+      if (le & h) { qa = U(oa - qa); qb = U(ob - qb); qc = U(oc - qc); qd = U(od - qd); }
+      else { qa = U(qa + oa); qb = U(qb + ob); qc = U(qc + oc); qd = U(qd + od); }
+    }
+    O |= qa | qb | qc | qd;
+    D |= (qa ^ qc) | (qb ^ qd);
+  }
+  O = gor<GS>(O);
+  D = gor<GS>(D);
+  if (!O) return 0;
+  const int m = __ffsll((unsigned long long)O) - 1;
+  const int v = 2 * m + (((D >> m) & 1) ? 0 : 1);
+  const int need = 2 * (NQ + k) - v;
+  return need > 0 ? need : 0;
+}
+
+template <int NQ> __device__ __forceinline__ int t_bound_any(int4 s, int k, int le) {
+  return 2 * (NQ + k) <= 64 ? t_bound<NQ, uint32_t>(s, k, le) : t_bound<NQ, unsigned long long>(s, k, le);
+}
+
+__device__ __forceinline__ uint64_t entry_hash(int4 v) {
+  const uint64_t lo = (uint64_t)(uint32_t)v.x | ((uint64_t)(uint32_t)v.y << 32);
+  const uint64_t hi = (uint64_t)(uint32_t)v.z | ((uint64_t)(uint32_t)v.w << 32);
+  return mix64(lo ^ mix64(hi + 0x632be59bd9b4e019ull));
+}
+
+// class key: the smallest over the symmetry group of an entry-sum hash of sigma(S); equal classes give
+// equal keys, and equality of keys is always confirmed exactly (src/cheese.cu:224-241)
+template <int NQ> __device__ __forceinline__ uint64_t class_key(int4 s, int k, int le, const KSpec &sp) {
+  constexpr int N = 1 << NQ, GS = N * N;
+  const int r = le >> NQ, c = le & (N - 1);
+  const uint64_t hv = entry_hash(s), hc = entry_hash(zconj(s));
+  uint64_t best = ~0ull;
+  for (int y = 0; y < sp.nsym; y++) {
+    const uint8_t *pm = sp.symperm + y * N;
+    const uint64_t pos = (uint64_t)(pm[r] * N + pm[c]) + 1;
+    const uint64_t h = mix64((sp.symconj[y] ? hc : hv) ^ (pos * 0x9e3779b97f4a7c15ull));
+    const uint64_t key = mix64(gsum<GS>(h) ^ ((uint64_t)(k + 1) * 0xc2b2ae3d27d4eb4full));
+    best = key < best ? key : best;
+  }
+  return best;
+}
+
+// exact: B = sigma(A) for some group element (src/cheese.cu:242-255)
+template <int NQ> __device__ __forceinline__ bool same_class(int4 a, int ka, int4 b, int kb, int le, const KSpec &sp) {
+  constexpr int N = 1 << NQ, GS = N * N;
+  if (ka != kb) return false;
+  const int r = le >> NQ, c = le & (N - 1);
+  for (int y = 0; y < sp.nsym; y++) {
+    const uint8_t *pm = sp.symperm + y * N;
+    const int4 bv = gshfl<GS>(b, pm[r] * N + pm[c]);
+    const int4 av = sp.symconj[y] ? zconj(a) : a;
+    if (gall<GS>(zeq(av, bv))) return true;
+  }
+  return false;
+}
+
+}  // namespace chz

@@ -1,0 +1,94 @@
+/* cheese_hip.h - C ABI of the MI355X (gfx950) exact branch-and-bound synthesis library libcheese_hip.so.
+ *
+ * The reference has no in-process API: its boundary is a process contract (a .spec text file in, one JSON
+ * line per level on stdout, exit codes).  These entry points are that contract made callable: every return
+ * code is the reference binary's exit code for the same situation, and every level record carries the
+ * fields of the reference's per-level JSON line.  Plain pointers and sizes only; the caller owns its input
+ * buffers, a context owns its device memory.  A context is used by one host thread at a time.
+ *
+ * Return codes (all entry points): 0 circuit found / ok, 1 proof (no circuit within the budget), 2 bad
+ * spec, 3 usage or unsupported input, 4 coefficient overflow or capacity, 5 device error.
+ */
+#ifndef CHEESE_HIP_H
+#define CHEESE_HIP_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define CHZ_DMAX 48 /* longest prefix, src/cheese.cu:53 */
+
+/* ---------------- gate-count search (src/cheese.cu) ---------------- */
+
+typedef struct chz_ctx chz_ctx;
+
+typedef struct chz_opts {
+  int depth;      /* < 0: the spec's depth line; replaces `--depth D` (src/cheese.cu:1861) */
+  int dedupe;     /* 0 = `--no-dedupe` (src/cheese.cu:1862) */
+  int sym;        /* 0 = `--no-sym` (src/cheese.cu:1866) */
+  long chunk;     /* parents per expansion pass, `--chunk N` (src/cheese.cu:1863); <= 0: default */
+  int profile;    /* nonzero: per-pass seconds kept for chz_profile (`--profile`, src/cheese.cu:1230-1247) */
+  int device;     /* HIP device ordinal */
+} chz_opts;
+
+/* one level of the search: the fields of the per-level JSON line (src/cheese.cu:1354-1356) */
+typedef struct chz_level {
+  int t;
+  long parents, prefixes, cut_w3, cut_lb, survivors, distinct;
+  int proved_unsat;
+  double seconds;
+} chz_level;
+
+typedef void (*chz_level_cb)(const chz_level *lvl, void *user);
+
+/* result of a run (src/cheese.cu:1658-1662, 1800, 1846-1848) */
+typedef struct chz_result {
+  int rc;                  /* 0 found, 1 proof, 4 overflow, 5 device error */
+  int len;                 /* circuit length t (0 with rc 0: the target is the goal at depth 0) */
+  long found;              /* children equal to the goal at that level ("found" field) */
+  int circuit[CHZ_DMAX];   /* candidate indices, in circuit order */
+  double seconds;          /* since the search started */
+} chz_result;
+
+/* Parses a .spec (read_spec, src/cheese.cu:1149-1226), builds the symmetry group (sym_group, :1379-1429) and
+ * uploads the spec (run, :1762-1792).  Returns 0, 2 (bad spec) or 3 (n outside 1..3, states mode, depth > 48). */
+int chz_open(const char *spec_text, size_t len, const chz_opts *opts, chz_ctx **out);
+
+/* Runs the level-synchronous search (levels + run, src/cheese.cu:1249-1377, :1794-1848).  cb (may be NULL)
+ * is called once per completed level, on the calling thread, before the next level starts. */
+int chz_run(chz_ctx *ctx, chz_level_cb cb, void *user, chz_result *out);
+
+int chz_num_gates(const chz_ctx *ctx);
+int chz_num_qubits(const chz_ctx *ctx);
+int chz_depth(const chz_ctx *ctx);
+int chz_sym_size(const chz_ctx *ctx);                /* "symmetry classes: %d group elements" (:1792) */
+const char *chz_gate_name(const chz_ctx *ctx, int g); /* candidate names L_<gate>_<q>... */
+void chz_close(chz_ctx *ctx);
+
+/* Per-pass device seconds of the last run (`--profile` report, src/cheese.cu:1244-1247): names and values
+ * for up to n passes; returns the number of passes. */
+int chz_profile(const chz_ctx *ctx, const char **names, double *seconds, int n);
+
+/* Device-side kernel statistics of the last run (no reference counterpart; for measurement): per kernel
+ * family, launches, summed HIP-event milliseconds and the algorithmic work it did. */
+typedef struct chz_kstat {
+  const char *name;
+  long launches;
+  double ms;
+  double work;    /* algorithmic units: integer ops for the expansion kernels, bytes for sort/dedupe */
+} chz_kstat;
+int chz_kernel_stats(const chz_ctx *ctx, chz_kstat *out, int n);
+
+/* Exact check of a gate list against the spec's target (verify.py:20-33 restated): 1 equal, 0 not, <0 error. */
+int chz_verify(const chz_ctx *ctx, const int *circuit, int len);
+
+/* last error message of the calling thread (stderr text of the reference) */
+const char *chz_last_error(void);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

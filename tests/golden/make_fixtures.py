@@ -1,0 +1,103 @@
+"""Writes the golden fixtures of the gate-count search.  Runs only where the reference tree is mounted
+(/root/reference): it imports the reference's stdlib-only spec writer to produce .spec files and runs the
+reference CPU build (oracle/_ref/cheese-cpu, built by `make -C oracle ref`) for their per-level logs.
+
+Outputs (committed data): tests/golden/specs/*.spec, tests/golden/logs/*.log, tests/golden/manifest.json,
+and cheese_amd/skeletons/n{2,3}.spec (the default gate set's spec with an identity target, from which
+targets are spliced in).  Usage: python tests/golden/make_fixtures.py [--only NAME ...]
+"""
+import io
+import json
+import os
+import subprocess
+import sys
+import time
+
+REF = "/root/reference"
+ROOT = os.path.dirname(os.path.dirname(os.path.dirname(os.path.abspath(__file__))))
+sys.path.insert(0, os.path.join(REF, "python"))
+sys.path.insert(0, ROOT)
+from cheese import spec as ref_spec  # noqa: E402  (reference, this container only)
+from cheese_amd import workloads  # noqa: E402
+
+HERE = os.path.join(ROOT, "tests", "golden")
+BIN = os.path.join(ROOT, "oracle", "_ref", "cheese-cpu")
+QASM = os.path.join(REF, "tests", "qasm")
+
+# (name, qasm file, depth, extra flags): the reference's own regression and timing cases
+# (tests/check.sh:16-22, tests/bench.sh:7) plus deeper Toffoli budgets and flag variants
+CASES = [
+    ("cs-d4", "cs.qasm", 4, []), ("cs-d5", "cs.qasm", 5, []),
+    ("qft_2-d6", "qft_2.qasm", 6, []), ("qft_2-d7", "qft_2.qasm", 7, []),
+    ("toffoli-d8", "toffoli.qasm", 8, []), ("toffoli-d9", "toffoli.qasm", 9, []),
+    ("toffoli-d10", "toffoli.qasm", 10, []), ("toffoli-d11", "toffoli.qasm", 11, []),
+    ("toffoli-d12", "toffoli.qasm", 12, []),
+    ("toffoli-d9-nosym", "toffoli.qasm", 9, ["--no-sym"]),
+    ("toffoli-d11-nosym", "toffoli.qasm", 11, ["--no-sym"]),
+    ("toffoli-d8-nodedupe", "toffoli.qasm", 8, ["--no-dedupe"]),
+    ("toffoli-d10-chunk1000", "toffoli.qasm", 10, ["--chunk", "1000"]),
+    ("ccz-d7", "ccz.qasm", 7, []), ("ccz-d8", "ccz.qasm", 8, []),
+    ("relative-phase-toffoli-d9", "relative-phase-toffoli.qasm", 9, []),
+    ("fredkin-d9", "fredkin.qasm", 9, []), ("peres-d9", "peres.qasm", 9, []),
+    ("half_adder_3-d9", "half_adder_3.qasm", 9, []), ("grover_3-d10", "grover_3.qasm", 10, []),
+    ("ricotta-rus-0003-d10", "ricotta-rus-0003.qasm", 10, []),
+    ("ricotta-rus-0003-d11", "ricotta-rus-0003.qasm", 11, []),
+]
+NRANDOM = 32   # seeded random 3-qubit Clifford+T circuits of 10..13 gates, budget = length
+
+
+def spec_text(n, circ, depth):
+    buf = io.StringIO()
+    ref_spec.write_spec(buf, n, circ, depth)
+    return buf.getvalue()
+
+
+REF_TIMEOUT = 150   # seconds of reference CPU time per case; slower cases keep their spec without a log
+
+
+def run_ref(path, flags):
+    t = time.time()
+    try:
+        p = subprocess.run([BIN, path] + flags, capture_output=True, text=True, timeout=REF_TIMEOUT)
+    except subprocess.TimeoutExpired:
+        return None, None, time.time() - t
+    return p.returncode, p.stdout, time.time() - t
+
+
+def main():
+    only = set(sys.argv[sys.argv.index("--only") + 1:]) if "--only" in sys.argv else None
+    man_path = os.path.join(HERE, "manifest.json")
+    manifest = json.load(open(man_path)) if os.path.exists(man_path) else {}
+    for n in (2, 3):
+        with open(os.path.join(workloads.SKELETON_DIR, f"n{n}.spec"), "w") as f:
+            f.write(spec_text(n, [], 0))
+    jobs = []
+    for name, q, d, flags in CASES:
+        n, circ = ref_spec.parse_qasm(os.path.join(QASM, q))
+        jobs.append((name, n, circ, d, flags, {"qasm": q}))
+    for s in range(NRANDOM):
+        circ = workloads.random_circuit(s, 3, 10, 13)
+        jobs.append((f"random3-s{s}", 3, circ, len(circ), [], {"seed": s, "circuit": [[g, list(q)] for g, q in circ]}))
+    force = "--force" in sys.argv
+    for name, n, circ, d, flags, extra in jobs:
+        if only and name not in only:
+            continue
+        text = spec_text(n, circ, d)
+        sp = os.path.join(HERE, "specs", name + ".spec")
+        with open(sp, "w") as f:
+            f.write(text)
+        lp = os.path.join(HERE, "logs", name + ".log")
+        if name in manifest and os.path.exists(lp) and not force:
+            continue
+        rc, out, sec = run_ref(sp, flags)
+        if rc is not None:
+            with open(lp, "w") as f:
+                f.write(out)
+        manifest[name] = dict(extra, n=n, depth=d, flags=flags, rc=rc, ref_seconds=round(sec, 2))
+        print(f"{name}: rc {rc}, {sec:.1f} s", flush=True)
+        with open(man_path, "w") as f:
+            json.dump(manifest, f, indent=1, sort_keys=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,70 @@
+"""Parity of the HIP gate-count search with the reference: per-level lines and result lines identical to the
+reference CPU build's golden logs, every emitted circuit exactly re-verified.  Needs an MI355X."""
+import pytest
+
+from conftest import flags_to_kwargs, golden_cases, manifest, read_log, read_spec, strip_seconds
+
+import cheese_amd
+from cheese_amd import workloads
+
+pytestmark = pytest.mark.gpu
+
+
+def _lines(out):
+    lines = [cheese_amd.level_line(l) for l in out["levels"]]
+    if out["rc"] == 0:
+        lines.append(cheese_amd.result_line(out))
+    return "\n".join(lines) + "\n"
+
+
+@pytest.mark.parametrize("name", golden_cases(max_seconds=200.0))
+def test_hip_matches_reference_log(name):
+    m = manifest()[name]
+    print(f"case {name}", flush=True)
+    out = cheese_amd.search(read_spec(name), **flags_to_kwargs(m["flags"]))
+    assert out["rc"] == m["rc"], out.get("error")
+    assert strip_seconds(_lines(out)) == strip_seconds(read_log(name))
+    if out["rc"] == 0:
+        assert out["verified"]
+
+
+def test_chunking_leaves_levels_unchanged():
+    """--chunk only changes how a level is split into passes (src/cheese.cu:1264)"""
+    spec = read_spec("toffoli-d10")
+    a = cheese_amd.search(spec, chunk=37)
+    assert strip_seconds(_lines(a)) == strip_seconds(read_log("toffoli-d10"))
+
+
+@pytest.mark.parametrize("seed", range(32))
+def test_random_circuit_found_and_verified(seed):
+    """C2's seeded random 3-qubit Clifford+T circuits at budget = length: a circuit of at most that length,
+    exactly equal to the target; identical to the reference log where the reference finished"""
+    name = f"random3-s{seed}"
+    m = manifest().get(name)
+    circ, spec = workloads.random_spec(seed)
+    print(f"case {name} length {len(circ)}", flush=True)
+    out = cheese_amd.search(spec)
+    assert out["rc"] == 0
+    assert out["verified"]
+    assert len(out["circuit"]) <= len(circ)
+    if m and m.get("rc") is not None:
+        assert strip_seconds(_lines(out)) == strip_seconds(read_log(name))
+
+
+def test_overflow_exit_4():
+    """a target whose coefficients leave int32 aborts with exit 4 (src/cheese.cu:1797)"""
+    spec = read_spec("cs-d4")
+    lines = spec.splitlines()
+    i = lines.index([l for l in lines if l.startswith("target ")][0])
+    row = lines[i + 1].split()
+    row[0] = str(1 << 40)
+    lines[i + 1] = " ".join(row)
+    out = cheese_amd.search("\n".join(lines) + "\n")
+    assert out["rc"] == 4
+
+
+def test_depth_zero_identity():
+    """a target equal to the goal prints {"found": 0, "circuit": []} (src/cheese.cu:1794-1801)"""
+    skel = workloads.skeleton(2)
+    out = cheese_amd.search(skel)
+    assert out["rc"] == 0 and out["circuit"] == []

@@ -13,8 +13,9 @@ constexpr uint32_t P_NINV = 998244351u;       // -p^-1 mod 2^32
 struct DevGate {                               // S <- S g^-1 data of one candidate
   int nq, L, mask, ik;
   int q[3];
-  int pad;
+  int unit;                                    // every nonzero inverse entry is +-w^p
   int4 inv[64];                                // L x L inverse local matrix, row m, column lc
+  int8_t ph[64];                               // unit gates: w-power p of inv[m][lc], -1 for zero
 };
 
 struct KSpec {                                 // device view of a spec (kernel argument)

@@ -30,7 +30,7 @@ struct ExpandArgs {
   int t, keep;
   uint8_t *status;          // [ncc]
   uint64_t *keys;           // [ncc] class keys of kept children
-  uint2 *needt;             // (child in chunk, packed cheap bounds)
+  uint32_t *info;           // [ncc] packed cheap bounds of children that need the T bound
   unsigned long long *cnt;
 };
 
@@ -46,8 +46,8 @@ __global__ __launch_bounds__(256) void k_expand(KSpec sp, ExpandArgs a) {
   for (long long base = wave * CPW; base < a.ncc; base += nwaves * CPW) {
     const long long cc = base + slot;
     if (cc >= a.ncc) continue;
-    const long long pi = a.off + cc / sp.G;
-    const int g = (int)(cc % sp.G);
+    const long long pi = a.off + (long long)((uint32_t)cc / (uint32_t)sp.G);
+    const int g = (int)((uint32_t)cc % (uint32_t)sp.G);
     const int g1 = a.g1[pi], g2 = a.g2[pi];
     uint8_t st = 255;
     uint32_t info = 0;
@@ -86,11 +86,7 @@ __global__ __launch_bounds__(256) void k_expand(KSpec sp, ExpandArgs a) {
           n_found++;
           atomicMin(&a.cnt[C_FIRST], (unsigned long long)(a.off * sp.G + cc));
           break;
-        case ST_NEEDT: {
-          const unsigned long long j = atomicAdd(&a.cnt[C_NEEDT], 1ull);
-          a.needt[j] = make_uint2((uint32_t)cc, info);
-          break;
-        }
+        case ST_NEEDT: a.info[cc] = info; break;
       }
     }
   }
@@ -105,11 +101,15 @@ __global__ __launch_bounds__(256) void k_expand(KSpec sp, ExpandArgs a) {
   }
 }
 
-// pass 2 on the children the cheap bounds kept: T bound, verdict, class key (src/cheese.cu:700-720)
+// pass 2 on the children the cheap bounds kept (in child order): T bound, verdict, class key
+// (src/cheese.cu:700-720).  The T bound runs mod 2^16 with one lane per flip pair when n + k <= 16, else
+// mod 2^32 / 2^64 with one lane per matrix entry.
 template <int NQ>
-__global__ __launch_bounds__(256) void k_tbound(KSpec sp, ExpandArgs a, long long nt) {
+__global__ __launch_bounds__(256) void k_tbound(KSpec sp, ExpandArgs a, const uint32_t *list, long long nt) {
   constexpr int N = 1 << NQ, E = N * N, GS = E, CPW = 64 / GS;
+  __shared__ TStage stage[256];
   const int lane = threadIdx.x & 63, le = lane & (GS - 1), slot = lane / GS;
+  TStage *st = stage + (threadIdx.x & ~(GS - 1));
   const long long wave = ((long long)blockIdx.x * blockDim.x + threadIdx.x) >> 6;
   const long long nwaves = ((long long)gridDim.x * blockDim.x) >> 6;
   const int rem = sp.depth - a.t;
@@ -117,17 +117,27 @@ __global__ __launch_bounds__(256) void k_tbound(KSpec sp, ExpandArgs a, long lon
   for (long long base = wave * CPW; base < nt; base += nwaves * CPW) {
     const long long j = base + slot;
     if (j >= nt) continue;
-    const uint2 ent = a.needt[j];
-    const long long cc = ent.x;
-    const long long pi = a.off + cc / sp.G;
-    const int g = (int)(cc % sp.G);
+    const uint32_t cc = list[j];
+    const uint32_t info = a.info[cc];
+    const long long pi = a.off + (long long)(cc / (uint32_t)sp.G);
+    const int g = (int)(cc % (uint32_t)sp.G);
     int4 s;
     int k;
     make_child<NQ>(a.res + pi * E, a.pk[pi], sp.gates[g], le, s, k);
     bool alive = true;
-    if (!(ent.y & INFO_NOBOUND)) {
-      const int h = ent.y & 1023, c2 = (ent.y >> 10) & 1023, supp = (ent.y >> 20) & 1023;
-      const int T = (t_bound_any<NQ>(s, k, le) + sp.tdiv - 1) / sp.tdiv;
+    if (!(info & INFO_NOBOUND)) {
+      const int h = info & 1023, c2 = (info >> 10) & 1023, supp = (info >> 20) & 1023;
+      int T;
+      if (NQ + k <= 16) {
+        __builtin_amdgcn_wave_barrier();
+        t_stage<NQ>(st, s, le);
+        __builtin_amdgcn_wave_barrier();
+        T = t_bound16<NQ>(st, k, le);
+        __builtin_amdgcn_wave_barrier();
+      } else {
+        T = t_bound_any<NQ>(s, k, le);
+      }
+      T = (T + sp.tdiv - 1) / sp.tdiv;
       alive = combine_bound(h, T, c2, supp) <= rem;
     }
     uint64_t key = 0;
@@ -157,6 +167,10 @@ struct FlagAlive {      // survivors of a chunk, in child order
 struct FlagKept {
   const uint8_t *kept;
   __device__ bool operator()(long long i) const { return kept[i] != 0; }
+};
+struct FlagNeed {       // children that need the T bound, in child order
+  const uint8_t *status;
+  __device__ bool operator()(long long i) const { return status[i] == ST_NEEDT; }
 };
 
 template <typename F>
@@ -239,9 +253,10 @@ __global__ __launch_bounds__(CT_THREADS) void k_scatter_alive(FlagAlive f, long 
   }
 }
 
-// kept survivors -> ordered list of survivor ranks
-__global__ __launch_bounds__(CT_THREADS) void k_scatter_kept(FlagKept f, long long n, const unsigned long long *boff,
-                                                             uint32_t *out) {
+// flagged positions -> ordered index list (kept survivor ranks, children needing the T bound)
+template <typename F>
+__global__ __launch_bounds__(CT_THREADS) void k_scatter_idx(F f, long long n, const unsigned long long *boff,
+                                                            uint32_t *out) {
   __shared__ int ws[CT_THREADS / 64];
   const long long t0 = (long long)blockIdx.x * CT_TILE;
   long long pos = (long long)boff[blockIdx.x];
@@ -528,7 +543,7 @@ struct chz_ctx {
   // per-level work arrays
   chz::DBuf<uint8_t> status, kept;
   chz::DBuf<uint64_t> keys, surv_o, surv_key;
-  chz::DBuf<uint2> needt;
+  chz::DBuf<uint32_t> needt, info;
   chz::DBuf<unsigned long long> cnt, bcount, hist;
   chz::DBuf<uint32_t> sk[2], sv[2], runs, kidx;
   // measurement
@@ -609,6 +624,8 @@ template <int NQ> struct Level {
       const long long maxcc = std::min<long long>(chunk, m) * G;
       c->status.need(maxcc);
       c->needt.need(maxcc);
+      c->info.need(maxcc);
+      c->bcount.need((maxcc + CT_TILE - 1) / CT_TILE + 1);
       if (keep) c->keys.need(maxcc);
       std::vector<unsigned long long> zero(C_N, 0);
       zero[C_FIRST] = ~0ull;
@@ -617,26 +634,38 @@ template <int NQ> struct Level {
       bool found = false;
       for (long long off = 0; off < m; off += chunk) {
         const long long cm = std::min(chunk, m - off), ncc = cm * G;
-        HIPCHK(hipMemsetAsync(c->cnt.p + C_NEEDT, 0, 8, c->stream));
         const ExpandArgs ea{c->res[cur].p, c->pk[cur].p, c->g1[cur].p, c->g2[cur].p, off, ncc, t, keep ? 1 : 0,
-                            c->status.p, c->keys.p, c->needt.p, c->cnt.p};
+                            c->status.p, c->keys.p, c->info.p, c->cnt.p};
+        const long long nbc = (ncc + CT_TILE - 1) / CT_TILE;
         c->ev.begin(K_EXPAND, c->stream);
         hipLaunchKernelGGL(k_expand<NQ>, dim3(grid_for((ncc + CPW - 1) / CPW)), dim3(256), 0, c->stream, ks, ea);
         c->ev.end(c->stream);
+        // children that need the T bound, in child order
+        c->ev.begin(K_COMPACT, c->stream);
+        hipLaunchKernelGGL(k_flag_count<FlagNeed>, dim3((unsigned)nbc), dim3(CT_THREADS), 0, c->stream,
+                           FlagNeed{c->status.p}, ncc, c->bcount.p);
+        hipLaunchKernelGGL(k_scan_one_block, dim3(1), dim3(1024), 0, c->stream, c->bcount.p, nbc);
+        c->ev.end(c->stream);
+        unsigned long long ntot = 0;
+        HIPCHK(hipMemcpyAsync(&ntot, c->bcount.p + nbc, 8, hipMemcpyDeviceToHost, c->stream));
         sync_counts(c, hc);
         if (hc[C_OVF]) { set_err("residual coefficients exceed the storage type (int32)"); return 4; }
         if (hc[C_FOUND]) found = true;
         if (found) continue;          // the level ends with a circuit: only FOUND and overflow still count
-        const long long nt = (long long)hc[C_NEEDT];
+        const long long nt = (long long)ntot;
         if (nt) {
+          c->ev.begin(K_COMPACT, c->stream);
+          hipLaunchKernelGGL(k_scatter_idx<FlagNeed>, dim3((unsigned)nbc), dim3(CT_THREADS), 0, c->stream,
+                             FlagNeed{c->status.p}, ncc, c->bcount.p, c->needt.p);
+          c->ev.end(c->stream);
           c->ev.begin(K_TBOUND, c->stream);
-          hipLaunchKernelGGL(k_tbound<NQ>, dim3(grid_for((nt + CPW - 1) / CPW)), dim3(256), 0, c->stream, ks, ea, nt);
+          hipLaunchKernelGGL(k_tbound<NQ>, dim3(grid_for((nt + CPW - 1) / CPW)), dim3(256), 0, c->stream, ks, ea,
+                             c->needt.p, nt);
           c->ev.end(c->stream);
           c->k_work[K_TBOUND] += (double)nt * (double)(E * (E * 28 + 4 * E * (2 * NQ) + 12 * E));
         }
         if (keep) {
-          const long long nb = (ncc + CT_TILE - 1) / CT_TILE;
-          c->bcount.need(nb + 1);
+          const long long nb = nbc;
           c->ev.begin(K_COMPACT, c->stream);
           hipLaunchKernelGGL(k_flag_count<FlagAlive>, dim3((unsigned)nb), dim3(CT_THREADS), 0, c->stream,
                              FlagAlive{c->status.p}, ncc, c->bcount.p);
@@ -723,8 +752,8 @@ template <int NQ> struct Level {
         hipLaunchKernelGGL(k_flag_count<FlagKept>, dim3((unsigned)nb), dim3(CT_THREADS), 0, c->stream,
                            FlagKept{c->kept.p}, nsurv, c->bcount.p);
         hipLaunchKernelGGL(k_scan_one_block, dim3(1), dim3(1024), 0, c->stream, c->bcount.p, nb);
-        hipLaunchKernelGGL(k_scatter_kept, dim3((unsigned)nb), dim3(CT_THREADS), 0, c->stream, FlagKept{c->kept.p},
-                           nsurv, c->bcount.p, c->kidx.p);
+        hipLaunchKernelGGL(k_scatter_idx<FlagKept>, dim3((unsigned)nb), dim3(CT_THREADS), 0, c->stream,
+                           FlagKept{c->kept.p}, nsurv, c->bcount.p, c->kidx.p);
         c->ev.end(c->stream);
         unsigned long long tot = 0;
         HIPCHK(hipMemcpyAsync(&tot, c->bcount.p + nb, 8, hipMemcpyDeviceToHost, c->stream));
@@ -785,8 +814,19 @@ static void upload_spec(chz_ctx *c) {
     d.L = 1 << h.nq;
     d.ik = h.ik;
     for (int j = 0; j < h.nq; j++) { d.q[j] = h.q[j]; d.mask |= 1 << h.q[j]; }
-    for (int i = 0; i < d.L * d.L; i++)
-      d.inv[i] = make_int4((int)h.inv[i].a, (int)h.inv[i].b, (int)h.inv[i].c, (int)h.inv[i].d);
+    d.unit = 1;
+    for (int i = 0; i < d.L * d.L; i++) {
+      const Zq &z = h.inv[i];
+      d.inv[i] = make_int4((int)z.a, (int)z.b, (int)z.c, (int)z.d);
+      // +-w^p has exactly one coefficient equal to +-1: w^p for p < 4 at index p, -w^p = w^(p+4)
+      const int nz = (z.a != 0) + (z.b != 0) + (z.c != 0) + (z.d != 0);
+      if (!nz) { d.ph[i] = -1; continue; }
+      const int64_t cf[4] = {z.a, z.b, z.c, z.d};
+      int idx = -1;
+      for (int j = 0; j < 4; j++) if (cf[j]) idx = j;
+      if (nz != 1 || (cf[idx] != 1 && cf[idx] != -1)) { d.unit = 0; d.ph[i] = -1; continue; }
+      d.ph[i] = (int8_t)(idx + (cf[idx] < 0 ? 4 : 0));
+    }
   }
   std::vector<int4> goal(N * N);
   for (int i = 0; i < N * N; i++) goal[i] = make_int4((int)s.goal[i].a, (int)s.goal[i].b, (int)s.goal[i].c, (int)s.goal[i].d);
@@ -973,7 +1013,7 @@ void chz_close(chz_ctx *c) {
     c->sk[b].release(); c->sv[b].release();
   }
   c->status.release(); c->kept.release(); c->keys.release(); c->surv_o.release(); c->surv_key.release();
-  c->needt.release(); c->cnt.release(); c->bcount.release(); c->hist.release(); c->runs.release(); c->kidx.release();
+  c->needt.release(); c->info.release(); c->cnt.release(); c->bcount.release(); c->hist.release(); c->runs.release(); c->kidx.release();
   void *ptrs[] = {c->d_gates, c->d_bad2, c->d_symperm, c->d_symconj, c->d_bad3, c->d_goal, c->d_cx};
   for (void *p : ptrs) if (p) (void)hipFree(p);
   c->ev.destroy();

@@ -8,9 +8,26 @@ namespace chz {
 enum : uint8_t { ST_DEAD = 0, ST_ALIVE = 1, ST_FOUND = 2, ST_W2 = 3, ST_W3 = 4, ST_NEEDT = 5, ST_OVF = 6 };
 constexpr uint32_t INFO_NOBOUND = 1u << 31;    // spec without bounds: every child is kept
 
+// x * w^p for a coefficient vector of int32 values (entries below 2^28 in magnitude: no overflow)
+__device__ __forceinline__ int4 zrot(int4 v, int p) {
+  const int q = p & 3;
+  const int nb = -v.y, nc = -v.z, nd = -v.w;
+  // w (a, b, c, d) = (-d, a, b, c)
+  int r0 = q == 0 ? v.x : q == 1 ? nd : q == 2 ? nc : nb;
+  int r1 = q == 0 ? v.y : q == 1 ? v.x : q == 2 ? nd : nc;
+  int r2 = q == 0 ? v.z : q == 1 ? v.y : q == 2 ? v.x : nd;
+  int r3 = q == 0 ? v.w : q == 1 ? v.z : q == 2 ? v.y : v.x;
+  const int m = (p & 4) ? -1 : 0;   // w^4 = -1
+  return make_int4((r0 ^ m) - m, (r1 ^ m) - m, (r2 ^ m) - m, (r3 ^ m) - m);
+}
+__device__ __forceinline__ uint32_t uabs(int v) { return v < 0 ? 0u - (uint32_t)v : (uint32_t)v; }
+__device__ __forceinline__ uint32_t zmag(int4 v) { return uabs(v.x) | uabs(v.y) | uabs(v.z) | uabs(v.w); }
+
 // Child residual entry of lane le: (S g^-1)[r][c] = sum_m S[r][x_m] ginv[m][lc], then the group lowers the
 // sqrt2 exponent while every entry is divisible by sqrt2 (src/cheese.cu:146-174, :773-817).  Overflow is
-// judged before normalisation, where the reference narrows to int32.  Returns false on overflow.
+// judged before normalisation, where the reference narrows to int32.  Gates whose entries are all units
+// +-w^p (every standard gate) rotate coefficients in int32 while the parent's entries stay below 2^28;
+// otherwise the update runs in int64 with the general Z[w] product.  Returns false on overflow.
 template <int NQ>
 __device__ __forceinline__ bool make_child(const int4 *__restrict__ par, int pk, const DevGate &g, int le, int4 &s,
                                            int &k) {
@@ -22,21 +39,37 @@ __device__ __forceinline__ bool make_child(const int4 *__restrict__ par, int pk,
   if (nq > 1) lc |= ((c >> q1) & 1) << 1;
   if (nq > 2) lc |= ((c >> q2) & 1) << 2;
   const int base = c & ~mask;
-  long long a = 0, b = 0, cc = 0, d = 0;
-  for (int m = 0; m < L; m++) {
-    const int4 gm = g.inv[m * L + lc];
-    if (zzero(gm)) continue;
-    int x = base | ((m & 1) << q0);
-    if (nq > 1) x |= ((m >> 1) & 1) << q1;
-    if (nq > 2) x |= ((m >> 2) & 1) << q2;
-    const int4 v = par[r * N + x];
-    a += (long long)v.x * gm.x - (long long)v.y * gm.w - (long long)v.z * gm.z - (long long)v.w * gm.y;
-    b += (long long)v.x * gm.y + (long long)v.y * gm.x - (long long)v.z * gm.w - (long long)v.w * gm.z;
-    cc += (long long)v.x * gm.z + (long long)v.y * gm.y + (long long)v.z * gm.x - (long long)v.w * gm.w;
-    d += (long long)v.x * gm.w + (long long)v.y * gm.z + (long long)v.z * gm.y + (long long)v.w * gm.x;
+  bool of = false;
+  const bool small = gall<GS>(zmag(par[r * N + c]) < (1u << 28));
+  if (g.unit && small) {
+    int4 acc = make_int4(0, 0, 0, 0);
+    for (int m = 0; m < L; m++) {
+      const int p = g.ph[m * L + lc];
+      if (p < 0) continue;
+      int x = base | ((m & 1) << q0);
+      if (nq > 1) x |= ((m >> 1) & 1) << q1;
+      if (nq > 2) x |= ((m >> 2) & 1) << q2;
+      const int4 t = zrot(par[r * N + x], p);
+      acc.x += t.x; acc.y += t.y; acc.z += t.z; acc.w += t.w;
+    }
+    s = acc;
+  } else {
+    long long a = 0, b = 0, cc = 0, d = 0;
+    for (int m = 0; m < L; m++) {
+      const int4 gm = g.inv[m * L + lc];
+      if (zzero(gm)) continue;
+      int x = base | ((m & 1) << q0);
+      if (nq > 1) x |= ((m >> 1) & 1) << q1;
+      if (nq > 2) x |= ((m >> 2) & 1) << q2;
+      const int4 v = par[r * N + x];
+      a += (long long)v.x * gm.x - (long long)v.y * gm.w - (long long)v.z * gm.z - (long long)v.w * gm.y;
+      b += (long long)v.x * gm.y + (long long)v.y * gm.x - (long long)v.z * gm.w - (long long)v.w * gm.z;
+      cc += (long long)v.x * gm.z + (long long)v.y * gm.y + (long long)v.z * gm.x - (long long)v.w * gm.w;
+      d += (long long)v.x * gm.w + (long long)v.y * gm.z + (long long)v.z * gm.y + (long long)v.w * gm.x;
+    }
+    of = a != (int)a || b != (int)b || cc != (int)cc || d != (int)d;
+    s = make_int4((int)a, (int)b, (int)cc, (int)d);
   }
-  const bool of = a != (int)a || b != (int)b || cc != (int)cc || d != (int)d;
-  s = make_int4((int)a, (int)b, (int)cc, (int)d);
   k = pk + g.ik;
   if (gany<GS>(of)) return false;
   while (k > 0 && gall<GS>(zhalf(s))) { s = zdown(s); k--; }
@@ -91,9 +124,11 @@ __device__ __forceinline__ int spread_bits(int v, int mask, int n) {   // bit j 
   return out;
 }
 
-// rank over F_p of the operator-Schmidt matrix across cut A (|A| = sa), R = 4^sa rows, stopping at `stop`
-// (src/cheese.cu:270-303).  Lane le holds matrix entry (le / C, le % C); fraction-free elimination keeps
-// every row a nonzero multiple of an F_p row combination, so the rank is that of the matrix mod p.
+// rank over F_p of the operator-Schmidt matrix across cut A (|A| = sa), R = 4^sa rows, counting up to `stop`
+// (src/cheese.cu:270-303).  Lane le holds matrix entry (le / C, le % C).  Full pivoting: any nonzero entry
+// of a row not yet used becomes the pivot and its column is cleared from the other unused rows by
+// fraction-free updates (each a nonzero multiple of an F_p row combination), so after r steps the rank
+// is r plus that of the unused rows; the rank over F_p does not depend on the pivot order.
 template <int NQ> __device__ __forceinline__ int cut_rank(uint32_t fp, int le, int A, int sa, int stop) {
   constexpr int N = 1 << NQ, GS = N * N;
   const int R = 1 << (2 * sa), C = GS / R;
@@ -104,19 +139,18 @@ template <int NQ> __device__ __forceinline__ int cut_rank(uint32_t fp, int le, i
   const int o = spread_bits(oa, A, NQ) | spread_bits(ob, notA, NQ);
   const int i = spread_bits(ia, A, NQ) | spread_bits(ib, notA, NQ);
   uint32_t v = __shfl(fp, o * N + i, GS);
-  int rank = 0;
-  for (int c = 0; c < C && rank < stop; c++) {
-    const uint64_t bb = gballot<GS>(mcol == c && mrow >= rank && v != 0);
-    if (!bb) continue;
-    const int prow = (__ffsll((long long)bb) - 1) / C;
-    if (prow != rank) {
-      const int sr = mrow == prow ? rank : mrow == rank ? prow : mrow;
-      v = __shfl(v, sr * C + mcol, GS);
-    }
-    const uint32_t pv = __shfl(v, rank * C + c, GS);
-    const uint32_t fr = __shfl(v, mrow * C + c, GS);
-    const uint32_t pr = __shfl(v, rank * C + mcol, GS);
-    if (mrow > rank) v = fp_sub(mont_mul(v, pv), mont_mul(fr, pr));
+  int used = 0, rank = 0;                        // rows already pivots
+  while (rank < stop) {
+    const bool mine = !((used >> mrow) & 1);
+    const uint64_t bb = gballot<GS>(mine && v != 0);
+    if (!bb) break;
+    const int p = __ffsll((long long)bb) - 1;
+    const int prow = p / C, pcol = p % C;
+    const uint32_t pv = __shfl(v, p, GS);
+    const uint32_t fr = __shfl(v, mrow * C + pcol, GS);
+    const uint32_t pr = __shfl(v, prow * C + mcol, GS);
+    used |= 1 << prow;
+    if (mine && mrow != prow) v = fp_sub(mont_mul(v, pv), mont_mul(fr, pr));
     rank++;
   }
   return rank;
@@ -173,6 +207,78 @@ template <int NQ, typename U> __device__ __forceinline__ int t_bound(int4 s, int
   if (!O) return 0;
   const int m = __ffsll((unsigned long long)O) - 1;
   const int v = 2 * m + (((D >> m) & 1) ? 0 : 1);
+  const int need = 2 * (NQ + k) - v;
+  return need > 0 ? need : 0;
+}
+
+typedef unsigned short u16x2 __attribute__((ext_vector_type(2)));
+
+__device__ __forceinline__ u16x2 pk(int lo, int hi) { u16x2 r; r.x = (unsigned short)lo; r.y = (unsigned short)hi; return r; }
+__device__ __forceinline__ uint32_t bits(u16x2 v) { return (uint32_t)v.x | ((uint32_t)v.y << 16); }
+
+// Per-entry operands of the packed transform, staged in LDS by the lane owning entry e:
+// X[e] = (a, b | c, d) of S[e] mod 2^16, and the six packed multipliers of conj(S[e]) with which
+// S[x] conj(S[e]) = (Qa, Qb | Qc, Qd) = xa P1 + xb P2 + xc P3 + xd P4 | xa P5 + xb P6 + xc P1 + xd P2.
+struct TStage {
+  uint2 x;           // (a | b << 16, c | d << 16)
+  uint32_t p[6];
+};
+
+template <int NQ> __device__ __forceinline__ void t_stage(TStage *st, int4 s, int le) {
+  const int a = s.x, b = s.y, c = s.z, d = s.w;
+  TStage t;
+  t.x = make_uint2(bits(pk(a, b)), bits(pk(c, d)));
+  t.p[0] = bits(pk(a, -d));
+  t.p[1] = bits(pk(b, a));
+  t.p[2] = bits(pk(c, b));
+  t.p[3] = bits(pk(d, c));
+  t.p[4] = bits(pk(-c, -b));
+  t.p[5] = bits(pk(-d, -c));
+  st[le] = t;
+}
+
+__device__ __forceinline__ u16x2 as16(uint32_t v) { u16x2 r; r.x = (unsigned short)v; r.y = (unsigned short)(v >> 16); return r; }
+
+// T bound with arithmetic mod 2^16 (exact when n + k <= 16): lane f of the group computes the whole 2-D
+// transform of Q_f[e] = S[e ^ f] conj(S[e]) in registers, two Z[w] coefficients per 32-bit register.
+template <int NQ> __device__ __forceinline__ int t_bound16(const TStage *st, int k, int f) {
+  constexpr int E = 1 << (2 * NQ), GS = E;
+  u16x2 qab[E], qcd[E];
+#pragma unroll
+  for (int e = 0; e < E; e++) {
+    const uint2 xv = st[e ^ f].x;
+    const TStage &z = st[e];
+    const u16x2 x01 = as16(xv.x), x23 = as16(xv.y);
+    const u16x2 xa = x01.xx, xb = x01.yy, xc = x23.xx, xd = x23.yy;
+    const u16x2 p1 = as16(z.p[0]), p2 = as16(z.p[1]), p3 = as16(z.p[2]), p4 = as16(z.p[3]), p5 = as16(z.p[4]),
+                p6 = as16(z.p[5]);
+    qab[e] = xa * p1 + xb * p2 + xc * p3 + xd * p4;
+    qcd[e] = xa * p5 + xb * p6 + xc * p1 + xd * p2;
+  }
+#pragma unroll
+  for (int h = 1; h < E; h <<= 1)
+#pragma unroll
+    for (int e = 0; e < E; e++)
+      if (!(e & h)) {
+        const u16x2 u = qab[e], v = qab[e | h];
+        qab[e] = u + v;
+        qab[e | h] = u - v;
+        const u16x2 x = qcd[e], y = qcd[e | h];
+        qcd[e] = x + y;
+        qcd[e | h] = x - y;
+      }
+  uint32_t O = 0, D = 0;
+#pragma unroll
+  for (int e = 0; e < E; e++) {
+    O |= bits(qab[e]) | bits(qcd[e]);
+    D |= bits(qab[e] ^ qcd[e]);
+  }
+  O = gor<GS>(O);
+  D = gor<GS>(D);
+  const uint32_t o = (O | (O >> 16)) & 0xffffu, dd = (D | (D >> 16)) & 0xffffu;
+  if (!o) return 0;
+  const int m = __ffs((int)o) - 1;
+  const int v = 2 * m + (((dd >> m) & 1) ? 0 : 1);
   const int need = 2 * (NQ + k) - v;
   return need > 0 ? need : 0;
 }

@@ -47,6 +47,21 @@ def cpu_baseline(spec_name):
             "sample": f"{spec_name}: one full search, {pref} prefixes in {sec:.2f} s (exit {p.returncode})"}
 
 
+def copy_probe(nbytes):
+    """one device-to-device copy of nbytes through the HIP runtime: a kernel of known traffic (nbytes read,
+    nbytes written) against which the profiler's FETCH_SIZE / WRITE_SIZE counters are scaled"""
+    import ctypes
+    hip = ctypes.CDLL("libamdhip64.so")
+    a, b = ctypes.c_void_p(), ctypes.c_void_p()
+    assert hip.hipMalloc(ctypes.byref(a), ctypes.c_size_t(nbytes)) == 0
+    assert hip.hipMalloc(ctypes.byref(b), ctypes.c_size_t(nbytes)) == 0
+    assert hip.hipMemset(a, 1, ctypes.c_size_t(nbytes)) == 0
+    assert hip.hipMemcpy(b, a, ctypes.c_size_t(nbytes), 3) == 0      # hipMemcpyDeviceToDevice
+    assert hip.hipDeviceSynchronize() == 0
+    hip.hipFree(a)
+    hip.hipFree(b)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -54,11 +69,14 @@ def main():
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--cpu-sample", default="toffoli-d11")
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--copy-probe-mb", type=int, default=0, help="also run a D2D copy of this size (profiling)")
     a = ap.parse_args()
     if a.gpus != 1:
         print("bench.py: this build runs on one GPU; multi-GPU sharding is a design note only", file=sys.stderr)
         sys.exit(2)
     import cheese_amd
+    if a.copy_probe_mb:
+        copy_probe(a.copy_probe_mb << 20)
     name = "toffoli-d12"
     with open(os.path.join(ROOT, "tests", "golden", "specs", name + ".spec")) as f:
         spec = f.read()

@@ -12,7 +12,7 @@ import json
 import os
 
 HERE = os.path.dirname(os.path.abspath(__file__))
-LIB_PATH = os.path.join(HERE, "lib", "libcheese_hip.so")
+LIB_PATH = os.environ.get("CHEESE_AMD_LIB") or os.path.join(HERE, "lib", "libcheese_hip.so")
 CLI_PATH = os.path.join(HERE, "bin", "cheese")
 DMAX = 48
 

@@ -132,7 +132,7 @@ __global__ __launch_bounds__(256) void k_tbound(KSpec sp, ExpandArgs a, const ui
         __builtin_amdgcn_wave_barrier();
         t_stage<NQ>(st, s, le);
         __builtin_amdgcn_wave_barrier();
-        T = t_bound16<NQ>(st, k, le);
+        T = t_bound16<NQ>(st, s, k, le);
         __builtin_amdgcn_wave_barrier();
       } else {
         T = t_bound_any<NQ>(s, k, le);

@@ -239,46 +239,71 @@ template <int NQ> __device__ __forceinline__ void t_stage(TStage *st, int4 s, in
 
 __device__ __forceinline__ u16x2 as16(uint32_t v) { u16x2 r; r.x = (unsigned short)v; r.y = (unsigned short)(v >> 16); return r; }
 
-// T bound with arithmetic mod 2^16 (exact when n + k <= 16): lane f of the group computes the whole 2-D
-// transform of Q_f[e] = S[e ^ f] conj(S[e]) in registers, two Z[w] coefficients per 32-bit register.
-template <int NQ> __device__ __forceinline__ int t_bound16(const TStage *st, int k, int f) {
-  constexpr int E = 1 << (2 * NQ), GS = E;
-  u16x2 qab[E], qcd[E];
+__device__ __forceinline__ int ctz16(uint32_t v) { return v & 0xffffu ? __ffs((int)(v & 0xffffu)) - 1 : 16; }
+
+// T bound with arithmetic mod 2^16 (exact when n + k <= 16; a value that is 0 mod 2^16 has valuation >= 32 >=
+// 2(n+k) and cannot raise the bound).  Q_f[e] = S[e ^ f] conj(S[e]) satisfies Q_f[e ^ f] = conj(Q_f[e]), so for
+// f != 0, with j the top bit of f and R = {e : e_j = 0}, the 2-D transform is
+//   W_f[z] = A[z] +- conj(A[z]),   A[z] = sum_{e in R} (-1)^{e.z} Q_f[e]   (a transform over the other bits),
+// and A + conj A = 2a + (b - d) sqrt2, A - conj A = i (2c + (b + d) sqrt2) for A = (a, b, c, d).  An element
+// x + y sqrt2 has valuation min(2 v2(x), 2 v2(y) + 1), so the smallest valuation over all outputs is read
+// from three OR accumulators.  f = 0 (Q_0 = |S|^2, real) is transformed by the whole group, one lane per entry.
+// Lane f of the group handles flip pair f (lane 0 repeats f = 1; OR-accumulation is idempotent).
+template <int NQ> __device__ __forceinline__ int t_bound16(const TStage *st, int4 s, int k, int le) {
+  constexpr int E = 1 << (2 * NQ), H = E / 2, GS = E;
+  int f = le == 0 ? 1 : le;
+  // keep the per-lane LDS address arithmetic inside the child loop: hoisted, its 64 addresses would
+  // halve the kernel's occupancy
+  asm volatile("" : "+v"(f));
+  const int j = 31 - __clz(f);
+  const int lowm = (1 << j) - 1;
+  u16x2 qab[H], qcd[H];
 #pragma unroll
-  for (int e = 0; e < E; e++) {
+  for (int i = 0; i < H; i++) {
+    const int e = ((i & ~lowm) << 1) | (i & lowm);     // i with a 0 inserted at bit j
     const uint2 xv = st[e ^ f].x;
     const TStage &z = st[e];
     const u16x2 x01 = as16(xv.x), x23 = as16(xv.y);
     const u16x2 xa = x01.xx, xb = x01.yy, xc = x23.xx, xd = x23.yy;
     const u16x2 p1 = as16(z.p[0]), p2 = as16(z.p[1]), p3 = as16(z.p[2]), p4 = as16(z.p[3]), p5 = as16(z.p[4]),
                 p6 = as16(z.p[5]);
-    qab[e] = xa * p1 + xb * p2 + xc * p3 + xd * p4;
-    qcd[e] = xa * p5 + xb * p6 + xc * p1 + xd * p2;
+    qab[i] = xa * p1 + xb * p2 + xc * p3 + xd * p4;
+    qcd[i] = xa * p5 + xb * p6 + xc * p1 + xd * p2;
   }
 #pragma unroll
-  for (int h = 1; h < E; h <<= 1)
+  for (int h = 1; h < H; h <<= 1)
 #pragma unroll
-    for (int e = 0; e < E; e++)
-      if (!(e & h)) {
-        const u16x2 u = qab[e], v = qab[e | h];
-        qab[e] = u + v;
-        qab[e | h] = u - v;
-        const u16x2 x = qcd[e], y = qcd[e | h];
-        qcd[e] = x + y;
-        qcd[e | h] = x - y;
+    for (int i = 0; i < H; i++)
+      if (!(i & h)) {
+        const u16x2 u = qab[i], v = qab[i | h];
+        qab[i] = u + v;
+        qab[i | h] = u - v;
+        const u16x2 x = qcd[i], y = qcd[i | h];
+        qcd[i] = x + y;
+        qcd[i | h] = x - y;
       }
-  uint32_t O = 0, D = 0;
+  uint32_t ev = 0, od = 0;          // low halves: a | c (even part); high halves: (b - d) | (b + d) (odd part)
 #pragma unroll
-  for (int e = 0; e < E; e++) {
-    O |= bits(qab[e]) | bits(qcd[e]);
-    D |= bits(qab[e] ^ qcd[e]);
+  for (int i = 0; i < H; i++) {
+    ev |= bits(qab[i]) | bits(qcd[i]);
+    od |= bits(qab[i] - qcd[i]) | bits(qab[i] + qcd[i]);
   }
-  O = gor<GS>(O);
-  D = gor<GS>(D);
-  const uint32_t o = (O | (O >> 16)) & 0xffffu, dd = (D | (D >> 16)) & 0xffffu;
-  if (!o) return 0;
-  const int m = __ffs((int)o) - 1;
-  const int v = 2 * m + (((dd >> m) & 1) ? 0 : 1);
+  // f = 0: |S[e]|^2 = r + y sqrt2 with r = a^2 + b^2 + c^2 + d^2, y = ab + bc + cd - da
+  uint32_t r = (uint32_t)(s.x * s.x + s.y * s.y + s.z * s.z + s.w * s.w);
+  uint32_t y = (uint32_t)(s.x * s.y + s.y * s.z + s.z * s.w - s.w * s.x);
+#pragma unroll
+  for (int h = 1; h < GS; h <<= 1) {
+    const uint32_t ro = __shfl_xor(r, h, GS), yo = __shfl_xor(y, h, GS);
+    if (le & h) { r = ro - r; y = yo - y; } else { r = r + ro; y = y + yo; }
+  }
+  uint32_t accE = ev & 0xffffu, accF = (od >> 16) | (y & 0xffffu), accG = r & 0xffffu;
+  accE = gor<GS>(accE);
+  accF = gor<GS>(accF);
+  accG = gor<GS>(accG);
+  int v = 2 + 2 * ctz16(accE);
+  const int vf = 1 + 2 * ctz16(accF), vg = 2 * ctz16(accG);
+  v = vf < v ? vf : v;
+  v = vg < v ? vg : v;
   const int need = 2 * (NQ + k) - v;
   return need > 0 ? need : 0;
 }

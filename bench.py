@@ -62,6 +62,26 @@ def copy_probe(nbytes):
     hip.hipFree(b)
 
 
+def random_circuits(n=32):
+    """the second half of the workload: seeded random 3-qubit Clifford+T circuits of 10..13 gates, each
+    synthesised at budget = its length; every found circuit is checked exactly against its target"""
+    import cheese_amd
+    from cheese_amd import workloads
+    total_t, total_p, lens = 0.0, 0, []
+    for seed in range(n):
+        circ, spec = workloads.random_spec(seed)
+        t = time.perf_counter()
+        out = cheese_amd.search(spec)
+        total_t += time.perf_counter() - t
+        if out["rc"] != 0 or not out["verified"] or len(out["circuit"]) > len(circ):
+            print(f"bench.py: random circuit {seed} not synthesised exactly", file=sys.stderr)
+            sys.exit(1)
+        total_p += sum(l["prefixes"] for l in out["levels"])
+        lens.append([len(circ), len(out["circuit"])])
+    return {"seconds": round(total_t, 3), "prefixes": total_p, "prefixes_per_s": round(total_p / total_t, 1),
+            "lengths_input_found": lens}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -70,6 +90,8 @@ def main():
     ap.add_argument("--cpu-sample", default="toffoli-d11")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--copy-probe-mb", type=int, default=0, help="also run a D2D copy of this size (profiling)")
+    ap.add_argument("--random", action=argparse.BooleanOptionalAction, default=True,
+                    help="also time the 32 seeded random 3-qubit circuits (budget = length)")
     a = ap.parse_args()
     if a.gpus != 1:
         print("bench.py: this build runs on one GPU; multi-GPU sharding is a design note only", file=sys.stderr)
@@ -122,6 +144,8 @@ def main():
                      "launches_per_step": tb["launches"] / a.steps},
         "kernels_ms_per_step": {k: round(v["ms"] / a.steps, 3) for k, v in kstats.items()},
     }
+    if a.random:
+        line["random32"] = random_circuits()
     if not a.no_cpu:
         line["cpu_baseline"] = cpu_baseline(a.cpu_sample)
     print(json.dumps(line), flush=True)

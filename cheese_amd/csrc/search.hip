@@ -19,8 +19,7 @@ namespace chz {
 
 // ------------------------------------------------------------------------------------------------------
 // counters of one level (device): indices into a u64 array
-enum { C_W2 = 0, C_W3, C_DEAD, C_FOUND, C_ALIVE, C_OVF, C_NEEDT, C_FIRST, C_RUNS, C_REPOVF, C_KEPT,
-       C_CHEAPEVAL, C_CHEAPKILL, C_TEVAL, C_TKILL, C_N };
+enum { C_W2 = 0, C_W3, C_DEAD, C_FOUND, C_ALIVE, C_OVF, C_NEEDT, C_FIRST, C_RUNS, C_REPOVF, C_KEPT, C_N };
 
 struct ExpandArgs {
   const int4 *res;          // parents' residuals, E entries each
@@ -29,8 +28,6 @@ struct ExpandArgs {
   long long off;            // first parent of the chunk (index in the level)
   long long ncc;            // children in the chunk
   int t, keep;
-  int tfirst;               // evaluation order: T bound before support and CX (same verdicts, less work when
-                            // the cheap bounds rarely prune)
   uint8_t *status;          // [ncc]
   uint64_t *keys;           // [ncc] class keys of kept children
   uint32_t *info;           // [ncc] packed cheap bounds of children that need the T bound
@@ -45,7 +42,7 @@ __global__ __launch_bounds__(256) void k_expand(KSpec sp, ExpandArgs a) {
   const long long wave = ((long long)blockIdx.x * blockDim.x + threadIdx.x) >> 6;
   const long long nwaves = ((long long)gridDim.x * blockDim.x) >> 6;
   const int rem = sp.depth - a.t;
-  unsigned long long n_w2 = 0, n_w3 = 0, n_dead = 0, n_found = 0, n_ovf = 0, n_eval = 0, n_kill = 0;
+  unsigned long long n_w2 = 0, n_w3 = 0, n_dead = 0, n_found = 0, n_ovf = 0;
   for (long long base = wave * CPW; base < a.ncc; base += nwaves * CPW) {
     const long long cc = base + slot;
     if (cc >= a.ncc) continue;
@@ -67,20 +64,15 @@ __global__ __launch_bounds__(256) void k_expand(KSpec sp, ExpandArgs a) {
       else if (!sp.standard && !sp.lite) { st = ST_NEEDT; info = INFO_NOBOUND; }
       else {
         int h, supp = 0, c2 = 0;
-        uint32_t fl = 0;
         if (sp.lite) h = k;
         else {
           const int hs = h_struct<NQ>(s);
           h = k > hs ? k : hs;
-          if (a.tfirst) fl = INFO_TFIRST;
-          else {
-            supp = support<NQ>(s, le);
-            c2 = cx_bound<NQ>(s, le, sp);
-          }
+          supp = support<NQ>(s, le);
+          c2 = cx_bound<NQ>(s, le, sp);
         }
-        n_eval += le == 0;
-        if (combine_bound(h, 0, c2, supp) > rem) { st = ST_DEAD; n_kill += le == 0; }
-        else { st = ST_NEEDT; info = (uint32_t)h | ((uint32_t)c2 << 10) | ((uint32_t)supp << 20) | fl; }
+        if (combine_bound(h, 0, c2, supp) > rem) st = ST_DEAD;
+        else { st = ST_NEEDT; info = (uint32_t)h | ((uint32_t)c2 << 10) | ((uint32_t)supp << 20); }
       }
     }
     if (le == 0) {
@@ -100,10 +92,7 @@ __global__ __launch_bounds__(256) void k_expand(KSpec sp, ExpandArgs a) {
   }
   n_w2 = gsum<64>(n_w2); n_w3 = gsum<64>(n_w3); n_dead = gsum<64>(n_dead);
   n_found = gsum<64>(n_found); n_ovf = gsum<64>(n_ovf);
-  n_eval = gsum<64>(n_eval); n_kill = gsum<64>(n_kill);
   if (lane == 0) {
-    if (n_eval) atomicAdd(&a.cnt[C_CHEAPEVAL], n_eval);
-    if (n_kill) atomicAdd(&a.cnt[C_CHEAPKILL], n_kill);
     if (n_w2) atomicAdd(&a.cnt[C_W2], n_w2);
     if (n_w3) atomicAdd(&a.cnt[C_W3], n_w3);
     if (n_dead) atomicAdd(&a.cnt[C_DEAD], n_dead);
@@ -116,7 +105,7 @@ __global__ __launch_bounds__(256) void k_expand(KSpec sp, ExpandArgs a) {
 // (src/cheese.cu:700-720).  The T bound runs mod 2^16 with one lane per flip pair when n + k <= 16, else
 // mod 2^32 / 2^64 with one lane per matrix entry.
 template <int NQ>
-__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) void k_tbound(KSpec sp, ExpandArgs a, const uint32_t *list, long long nt) {
+__global__ __launch_bounds__(256) void k_tbound(KSpec sp, ExpandArgs a, const uint32_t *list, long long nt) {
   constexpr int N = 1 << NQ, E = N * N, GS = E, CPW = 64 / GS;
   __shared__ TStage stage[256];
   const int lane = threadIdx.x & 63, le = lane & (GS - 1), slot = lane / GS;
@@ -124,7 +113,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) void k
   const long long wave = ((long long)blockIdx.x * blockDim.x + threadIdx.x) >> 6;
   const long long nwaves = ((long long)gridDim.x * blockDim.x) >> 6;
   const int rem = sp.depth - a.t;
-  unsigned long long n_dead = 0, n_alive = 0, n_teval = 0, n_tkill = 0;
+  unsigned long long n_dead = 0, n_alive = 0;
   for (long long base = wave * CPW; base < nt; base += nwaves * CPW) {
     const long long j = base + slot;
     if (j >= nt) continue;
@@ -137,8 +126,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) void k
     make_child<NQ>(a.res + pi * E, a.pk[pi], sp.gates[g], le, s, k);
     bool alive = true;
     if (!(info & INFO_NOBOUND)) {
-      const int h = info & 1023;
-      int c2 = (info >> 10) & 1023, supp = (info >> 20) & 1023;
+      const int h = info & 1023, c2 = (info >> 10) & 1023, supp = (info >> 20) & 1023;
       int T;
       if (NQ + k <= 16) {
         __builtin_amdgcn_wave_barrier();
@@ -150,18 +138,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) void k
         T = t_bound_any<NQ>(s, k, le);
       }
       T = (T + sp.tdiv - 1) / sp.tdiv;
-      if (info & INFO_TFIRST) {
-        // combine(h, T, c2, supp) >= h + T: the remaining bounds only matter when h + T fits
-        n_teval += le == 0;
-        if (h + T > rem) { alive = false; n_tkill += le == 0; }
-        else {
-          supp = support<NQ>(s, le);
-          c2 = cx_bound<NQ>(s, le, sp);
-          alive = combine_bound(h, T, c2, supp) <= rem;
-        }
-      } else {
-        alive = combine_bound(h, T, c2, supp) <= rem;
-      }
+      alive = combine_bound(h, T, c2, supp) <= rem;
     }
     uint64_t key = 0;
     if (alive && a.keep) key = class_key<NQ>(s, k, le, sp);
@@ -173,13 +150,9 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) void k
   }
   n_dead = gsum<64>(n_dead);
   n_alive = gsum<64>(n_alive);
-  n_teval = gsum<64>(n_teval);
-  n_tkill = gsum<64>(n_tkill);
   if (lane == 0) {
     if (n_dead) atomicAdd(&a.cnt[C_DEAD], n_dead);
     if (n_alive) atomicAdd(&a.cnt[C_ALIVE], n_alive);
-    if (n_teval) atomicAdd(&a.cnt[C_TEVAL], n_teval);
-    if (n_tkill) atomicAdd(&a.cnt[C_TKILL], n_tkill);
   }
 }
 
@@ -644,7 +617,6 @@ template <int NQ> struct Level {
     const long long chunk = c->opts.chunk > 0 ? c->opts.chunk : (1ll << 20);
     c->cnt.need(C_N);
     unsigned long long hc[C_N];
-    int tfirst = 0;           // evaluation order of the next level, from the last level's pruning statistics
     for (int t = 1; t <= depth; t++) {
       const double tl = now_s();
       const bool keep = t < depth;
@@ -663,7 +635,7 @@ template <int NQ> struct Level {
       for (long long off = 0; off < m; off += chunk) {
         const long long cm = std::min(chunk, m - off), ncc = cm * G;
         const ExpandArgs ea{c->res[cur].p, c->pk[cur].p, c->g1[cur].p, c->g2[cur].p, off, ncc, t, keep ? 1 : 0,
-                            tfirst, c->status.p, c->keys.p, c->info.p, c->cnt.p};
+                            c->status.p, c->keys.p, c->info.p, c->cnt.p};
         const long long nbc = (ncc + CT_TILE - 1) / CT_TILE;
         c->ev.begin(K_EXPAND, c->stream);
         hipLaunchKernelGGL(k_expand<NQ>, dim3(grid_for((ncc + CPW - 1) / CPW)), dim3(256), 0, c->stream, ks, ea);
@@ -809,14 +781,6 @@ template <int NQ> struct Level {
           c->ev.flush(c->k_ms, c->k_launch);
         }
         cur = nxt;
-      }
-      // order for the next level: T first when the cheap bounds removed under a quarter of the children they
-      // saw; back to cheap first when, after T, support and CX still removed a quarter of what T let through
-      if (!tfirst) tfirst = hc[C_CHEAPEVAL] > 0 && 4 * hc[C_CHEAPKILL] < hc[C_CHEAPEVAL];
-      else {
-        const unsigned long long pass = hc[C_TEVAL] - hc[C_TKILL];
-        const unsigned long long after = hc[C_DEAD] - hc[C_TKILL] - hc[C_CHEAPKILL];
-        if (hc[C_TEVAL] && 4 * after > pass && pass > 0) tfirst = 0;
       }
       chz_level lv{};
       lv.t = t;

@@ -18,7 +18,8 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 METRIC = "prefixes/s & time-to-proof, Toffoli d=12; served solves/s + p50/p99 vs plain"
-VALU_PEAK_TOPS = 256 * 4 * 16 * 2.4e9 / 1e12   # 256 CUs x 4 SIMD16 x 2.4 GHz, one 32-bit op per lane-clock
+# 256 CUs x 4 SIMD16 x 2.4 GHz x 2 (packed 16-bit halves): peak 16-bit integer VALU operations per second
+VALU16_PEAK_TOPS = 256 * 4 * 16 * 2 * 2.4e9 / 1e12
 
 
 def _strip(lines):
@@ -121,7 +122,7 @@ def main():
             for f in s:
                 s[f] += v[f]
     total = sum(times)
-    tb = kstats.get("tbound", {"ms": 0.0, "work": 0.0, "launches": 0})
+    tb = kstats.get("expand", {"ms": 0.0, "work": 0.0, "launches": 0})
     achieved = tb["work"] / (tb["ms"] / 1e3) / 1e12 if tb["ms"] else 0.0
     line = {
         "metric": METRIC,
@@ -138,10 +139,11 @@ def main():
         "data": "reference target tests/qasm/toffoli.qasm, exact; log checked against the reference each step",
         "config": {"workload": "Toffoli gate-count proof, budget 12 (3 qubits, 27 candidates, 12 symmetries)",
                    "prefixes_per_step": prefixes, "parallelism": "1 GPU"},
-        "roofline": {"bound": "valu", "achieved": round(achieved, 3), "peak": VALU_PEAK_TOPS, "unit": "Top/s",
-                     "frac": round(achieved / VALU_PEAK_TOPS, 4), "traffic": None,
-                     "kernel": "k_tbound", "kernel_ms_per_step": round(tb["ms"] / a.steps, 3),
-                     "launches_per_step": tb["launches"] / a.steps},
+        "roofline": {"bound": "valu", "achieved": round(achieved, 3), "peak": VALU16_PEAK_TOPS, "unit": "Top/s",
+                     "frac": round(achieved / VALU16_PEAK_TOPS, 4), "traffic": None,
+                     "kernel": "k_level", "kernel_ms_per_step": round(tb["ms"] / a.steps, 3),
+                     "launches_per_step": tb["launches"] / a.steps,
+                     "work": "16-bit integer ops of the T-bound transforms only (cheap bounds not counted)"},
         "kernels_ms_per_step": {k: round(v["ms"] / a.steps, 3) for k, v in kstats.items()},
     }
     if a.random:

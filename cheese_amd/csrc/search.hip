@@ -19,7 +19,7 @@ namespace chz {
 
 // ------------------------------------------------------------------------------------------------------
 // counters of one level (device): indices into a u64 array
-enum { C_W2 = 0, C_W3, C_DEAD, C_FOUND, C_ALIVE, C_OVF, C_NEEDT, C_FIRST, C_RUNS, C_REPOVF, C_KEPT, C_N };
+enum { C_W2 = 0, C_W3, C_DEAD, C_FOUND, C_ALIVE, C_OVF, C_NEEDT, C_FIRST, C_RUNS, C_REPOVF, C_KEPT, C_EVAL, C_N };
 
 struct ExpandArgs {
   const int4 *res;          // parents' residuals, E entries each
@@ -34,124 +34,172 @@ struct ExpandArgs {
   unsigned long long *cnt;
 };
 
-// pass 1: windows, child residual, overflow, goal, cheap bounds (src/cheese.cu:662-698, :959-1001)
+// One level's children, fused (src/cheese.cu:662-720, :959-1021): a lane group per parent holds the parent's
+// residual in LDS and walks all G candidates: windows, child residual, overflow, goal, cheap bounds, T bound
+// (mod 2^16, one lane per flip pair, when n + k <= 16; else mod 2^32 / 2^64, one lane per entry) and the
+// class key of a kept child.  Writes the status byte (and key) of every child of the chunk.
+struct GateHdr { int nq, L, mask, ik, q0, q1, q2, kind; };   // kind: 0 general, 1 unit entries, 2 unit permutation
+constexpr int MAXG = 64;
+
 template <int NQ>
-__global__ __launch_bounds__(256) void k_expand(KSpec sp, ExpandArgs a) {
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) void k_level(KSpec sp, ExpandArgs a,
+                                                                                       long long np) {
   constexpr int N = 1 << NQ, E = N * N, GS = E, CPW = 64 / GS;
+  __shared__ TStage stage[256];
+  __shared__ int4 spar[256];                   // each group's parent residual
+  __shared__ int8_t sph[MAXG * 64];            // unit phase table of every candidate
+  __shared__ GateHdr shdr[MAXG];
+  for (int i = threadIdx.x; i < sp.G * 64; i += blockDim.x) sph[i] = sp.gates[i >> 6].ph[i & 63];
+  for (int g = threadIdx.x; g < sp.G; g += blockDim.x) {
+    const DevGate &d = sp.gates[g];
+    int kind = 0;
+    if (d.unit) {
+      kind = 2;                                // permutation: one nonzero per column
+      for (int lc = 0; lc < d.L; lc++) {
+        int nz = 0;
+        for (int m = 0; m < d.L; m++) nz += d.ph[m * d.L + lc] >= 0;
+        if (nz != 1) kind = 1;
+      }
+    }
+    shdr[g] = GateHdr{d.nq, d.L, d.mask, d.ik, d.q[0], d.q[1], d.q[2], kind};
+  }
+  __syncthreads();
   const int lane = threadIdx.x & 63, le = lane & (GS - 1), slot = lane / GS;
+  const int r = le >> NQ, c = le & (N - 1);
+  TStage *st = stage + (threadIdx.x & ~(GS - 1));
+  int4 *P = spar + (threadIdx.x & ~(GS - 1));
   const long long wave = ((long long)blockIdx.x * blockDim.x + threadIdx.x) >> 6;
   const long long nwaves = ((long long)gridDim.x * blockDim.x) >> 6;
   const int rem = sp.depth - a.t;
-  unsigned long long n_w2 = 0, n_w3 = 0, n_dead = 0, n_found = 0, n_ovf = 0;
-  for (long long base = wave * CPW; base < a.ncc; base += nwaves * CPW) {
-    const long long cc = base + slot;
-    if (cc >= a.ncc) continue;
-    const long long pi = a.off + (long long)((uint32_t)cc / (uint32_t)sp.G);
-    const int g = (int)((uint32_t)cc % (uint32_t)sp.G);
-    const int g1 = a.g1[pi], g2 = a.g2[pi];
-    uint8_t st = 255;
-    uint32_t info = 0;
-    if (g1 != 255 && sp.bad2[g1 * sp.G + g]) st = ST_W2;
-    else if (g2 != 255) {
-      const uint32_t tri = ((uint32_t)g2 * sp.G + g1) * sp.G + g;
-      if ((sp.bad3[tri >> 5] >> (tri & 31)) & 1) st = ST_W3;
-    }
-    if (st == 255) {
-      int4 s;
-      int k;
-      if (!make_child<NQ>(a.res + pi * E, a.pk[pi], sp.gates[g], le, s, k)) st = ST_OVF;
-      else if (k == sp.gk && gall<GS>(zeq(s, sp.goal[le]))) st = ST_FOUND;
-      else if (!sp.standard && !sp.lite) { st = ST_NEEDT; info = INFO_NOBOUND; }
-      else {
-        int h, supp = 0, c2 = 0;
-        if (sp.lite) h = k;
-        else {
-          const int hs = h_struct<NQ>(s);
-          h = k > hs ? k : hs;
-          supp = support<NQ>(s, le);
-          c2 = cx_bound<NQ>(s, le, sp);
+  const int G = sp.G;
+  unsigned long long n_w2 = 0, n_w3 = 0, n_dead = 0, n_found = 0, n_ovf = 0, n_alive = 0, n_eval = 0, n_t = 0;
+  for (long long base = wave * CPW; base < np; base += nwaves * CPW) {
+    const long long pl = base + slot;
+    if (pl >= np) continue;
+    const long long pi = a.off + pl;
+    const int4 S = a.res[pi * E + le];
+    const int pk = a.pk[pi], g1 = a.g1[pi], g2 = a.g2[pi];
+    const bool small = gall<GS>(zmag(S) < (1u << 28));
+    __builtin_amdgcn_wave_barrier();
+    P[le] = S;
+    __builtin_amdgcn_wave_barrier();
+    for (int g = 0; g < G; g++) {
+      const long long cc = pl * G + g;
+      uint8_t stt = 255;
+      if (g1 != 255 && sp.bad2[g1 * G + g]) stt = ST_W2;
+      else if (g2 != 255) {
+        const uint32_t tri = ((uint32_t)g2 * G + g1) * G + g;
+        if ((sp.bad3[tri >> 5] >> (tri & 31)) & 1) stt = ST_W3;
+      }
+      uint64_t key = 0;
+      if (stt == 255) {
+        // child residual entry (r, c) = sum_m P[r][x_m] ginv[m][lc] (src/cheese.cu:146-174)
+        const GateHdr h = shdr[g];
+        int lc = (c >> h.q0) & 1;
+        if (h.nq > 1) lc |= ((c >> h.q1) & 1) << 1;
+        if (h.nq > 2) lc |= ((c >> h.q2) & 1) << 2;
+        const int cb = c & ~h.mask;
+        const int8_t *ph = sph + g * 64;
+        int4 s;
+        bool of = false;
+        if (h.kind && small) {
+          int4 acc = make_int4(0, 0, 0, 0);
+          for (int m = 0; m < h.L; m++) {
+            const int p = ph[m * h.L + lc];
+            if (p < 0) continue;
+            int x = cb | ((m & 1) << h.q0);
+            if (h.nq > 1) x |= ((m >> 1) & 1) << h.q1;
+            if (h.nq > 2) x |= ((m >> 2) & 1) << h.q2;
+            const int4 t = zrot(P[r * N + x], p);
+            acc.x += t.x; acc.y += t.y; acc.z += t.z; acc.w += t.w;
+            if (h.kind == 2) break;
+          }
+          s = acc;
+        } else {
+          const DevGate &dg = sp.gates[g];
+          long long va = 0, vb = 0, vc = 0, vd = 0;
+          for (int m = 0; m < h.L; m++) {
+            const int4 gm = dg.inv[m * h.L + lc];
+            if (zzero(gm)) continue;
+            int x = cb | ((m & 1) << h.q0);
+            if (h.nq > 1) x |= ((m >> 1) & 1) << h.q1;
+            if (h.nq > 2) x |= ((m >> 2) & 1) << h.q2;
+            const int4 v = P[r * N + x];
+            va += (long long)v.x * gm.x - (long long)v.y * gm.w - (long long)v.z * gm.z - (long long)v.w * gm.y;
+            vb += (long long)v.x * gm.y + (long long)v.y * gm.x - (long long)v.z * gm.w - (long long)v.w * gm.z;
+            vc += (long long)v.x * gm.z + (long long)v.y * gm.y + (long long)v.z * gm.x - (long long)v.w * gm.w;
+            vd += (long long)v.x * gm.w + (long long)v.y * gm.z + (long long)v.z * gm.y + (long long)v.w * gm.x;
+          }
+          of = va != (int)va || vb != (int)vb || vc != (int)vc || vd != (int)vd;
+          s = make_int4((int)va, (int)vb, (int)vc, (int)vd);
         }
-        if (combine_bound(h, 0, c2, supp) > rem) st = ST_DEAD;
-        else { st = ST_NEEDT; info = (uint32_t)h | ((uint32_t)c2 << 10) | ((uint32_t)supp << 20); }
+        int k = pk + h.ik;
+        n_eval += le == 0;
+        if (gany<GS>(of)) stt = ST_OVF;
+        else {
+          while (k > 0 && gall<GS>(zhalf(s))) { s = zdown(s); k--; }
+          if (k == sp.gk && gall<GS>(zeq(s, sp.goal[le]))) stt = ST_FOUND;
+          else {
+            bool alive = true;
+            if (sp.standard || sp.lite) {
+              int hh, supp = 0, c2 = 0;
+              if (sp.lite) hh = k;
+              else {
+                const int hs = h_struct<NQ>(s);
+                hh = k > hs ? k : hs;
+                supp = support<NQ>(s, le);
+                c2 = cx_bound<NQ>(s, le, sp);
+              }
+              if (combine_bound(hh, 0, c2, supp) > rem) alive = false;
+              else {
+                int T;
+                n_t += le == 0;
+                if (NQ + k <= 16) {
+                  __builtin_amdgcn_wave_barrier();
+                  t_stage<NQ>(st, s, le);
+                  __builtin_amdgcn_wave_barrier();
+                  T = t_bound16<NQ>(st, s, k, le);
+                  __builtin_amdgcn_wave_barrier();
+                } else {
+                  T = t_bound_any<NQ>(s, k, le);
+                }
+                T = (T + sp.tdiv - 1) / sp.tdiv;
+                alive = combine_bound(hh, T, c2, supp) <= rem;
+              }
+            }
+            if (alive && a.keep) key = class_key<NQ>(s, k, le, sp);
+            stt = alive ? ST_ALIVE : ST_DEAD;
+          }
+        }
+      }
+      if (le == 0) {
+        a.status[cc] = stt;
+        switch (stt) {
+          case ST_W2: n_w2++; break;
+          case ST_W3: n_w3++; break;
+          case ST_DEAD: n_dead++; break;
+          case ST_OVF: n_ovf++; break;
+          case ST_ALIVE: n_alive++; if (a.keep) a.keys[cc] = key; break;
+          case ST_FOUND:
+            n_found++;
+            atomicMin(&a.cnt[C_FIRST], (unsigned long long)(a.off * G + cc));
+            break;
+        }
       }
     }
-    if (le == 0) {
-      a.status[cc] = st;
-      switch (st) {
-        case ST_W2: n_w2++; break;
-        case ST_W3: n_w3++; break;
-        case ST_DEAD: n_dead++; break;
-        case ST_OVF: n_ovf++; break;
-        case ST_FOUND:
-          n_found++;
-          atomicMin(&a.cnt[C_FIRST], (unsigned long long)(a.off * sp.G + cc));
-          break;
-        case ST_NEEDT: a.info[cc] = info; break;
-      }
-    }
+    __builtin_amdgcn_wave_barrier();
   }
   n_w2 = gsum<64>(n_w2); n_w3 = gsum<64>(n_w3); n_dead = gsum<64>(n_dead);
-  n_found = gsum<64>(n_found); n_ovf = gsum<64>(n_ovf);
+  n_found = gsum<64>(n_found); n_ovf = gsum<64>(n_ovf); n_alive = gsum<64>(n_alive);
+  n_eval = gsum<64>(n_eval); n_t = gsum<64>(n_t);
   if (lane == 0) {
+    if (n_eval) atomicAdd(&a.cnt[C_EVAL], n_eval);
+    if (n_t) atomicAdd(&a.cnt[C_NEEDT], n_t);
     if (n_w2) atomicAdd(&a.cnt[C_W2], n_w2);
     if (n_w3) atomicAdd(&a.cnt[C_W3], n_w3);
     if (n_dead) atomicAdd(&a.cnt[C_DEAD], n_dead);
     if (n_found) atomicAdd(&a.cnt[C_FOUND], n_found);
     if (n_ovf) atomicAdd(&a.cnt[C_OVF], n_ovf);
-  }
-}
-
-// pass 2 on the children the cheap bounds kept (in child order): T bound, verdict, class key
-// (src/cheese.cu:700-720).  The T bound runs mod 2^16 with one lane per flip pair when n + k <= 16, else
-// mod 2^32 / 2^64 with one lane per matrix entry.
-template <int NQ>
-__global__ __launch_bounds__(256) void k_tbound(KSpec sp, ExpandArgs a, const uint32_t *list, long long nt) {
-  constexpr int N = 1 << NQ, E = N * N, GS = E, CPW = 64 / GS;
-  __shared__ TStage stage[256];
-  const int lane = threadIdx.x & 63, le = lane & (GS - 1), slot = lane / GS;
-  TStage *st = stage + (threadIdx.x & ~(GS - 1));
-  const long long wave = ((long long)blockIdx.x * blockDim.x + threadIdx.x) >> 6;
-  const long long nwaves = ((long long)gridDim.x * blockDim.x) >> 6;
-  const int rem = sp.depth - a.t;
-  unsigned long long n_dead = 0, n_alive = 0;
-  for (long long base = wave * CPW; base < nt; base += nwaves * CPW) {
-    const long long j = base + slot;
-    if (j >= nt) continue;
-    const uint32_t cc = list[j];
-    const uint32_t info = a.info[cc];
-    const long long pi = a.off + (long long)(cc / (uint32_t)sp.G);
-    const int g = (int)(cc % (uint32_t)sp.G);
-    int4 s;
-    int k;
-    make_child<NQ>(a.res + pi * E, a.pk[pi], sp.gates[g], le, s, k);
-    bool alive = true;
-    if (!(info & INFO_NOBOUND)) {
-      const int h = info & 1023, c2 = (info >> 10) & 1023, supp = (info >> 20) & 1023;
-      int T;
-      if (NQ + k <= 16) {
-        __builtin_amdgcn_wave_barrier();
-        t_stage<NQ>(st, s, le);
-        __builtin_amdgcn_wave_barrier();
-        T = t_bound16<NQ>(st, s, k, le);
-        __builtin_amdgcn_wave_barrier();
-      } else {
-        T = t_bound_any<NQ>(s, k, le);
-      }
-      T = (T + sp.tdiv - 1) / sp.tdiv;
-      alive = combine_bound(h, T, c2, supp) <= rem;
-    }
-    uint64_t key = 0;
-    if (alive && a.keep) key = class_key<NQ>(s, k, le, sp);
-    if (le == 0) {
-      a.status[cc] = alive ? ST_ALIVE : ST_DEAD;
-      if (alive) { n_alive++; if (a.keep) a.keys[cc] = key; }
-      else n_dead++;
-    }
-  }
-  n_dead = gsum<64>(n_dead);
-  n_alive = gsum<64>(n_alive);
-  if (lane == 0) {
-    if (n_dead) atomicAdd(&a.cnt[C_DEAD], n_dead);
     if (n_alive) atomicAdd(&a.cnt[C_ALIVE], n_alive);
   }
 }
@@ -482,8 +530,8 @@ template <typename T> struct DBuf {     // grow-only device array
 
 static double now_s() { return std::chrono::duration<double>(std::chrono::steady_clock::now().time_since_epoch()).count(); }
 
-enum { K_EXPAND = 0, K_TBOUND, K_COMPACT, K_SORT, K_DEDUPE, K_GATHER, K_NFAM };
-static const char *k_names[K_NFAM] = {"expand", "tbound", "compact", "sort", "dedupe", "gather"};
+enum { K_EXPAND = 0, K_COMPACT, K_SORT, K_DEDUPE, K_GATHER, K_NFAM };
+static const char *k_names[K_NFAM] = {"expand", "compact", "sort", "dedupe", "gather"};
 
 // HIP-event timing of kernel families on the context's stream; pairs are read after each level's sync
 struct EventLog {
@@ -543,7 +591,7 @@ struct chz_ctx {
   // per-level work arrays
   chz::DBuf<uint8_t> status, kept;
   chz::DBuf<uint64_t> keys, surv_o, surv_key;
-  chz::DBuf<uint32_t> needt, info;
+  chz::DBuf<uint32_t> info;
   chz::DBuf<unsigned long long> cnt, bcount, hist;
   chz::DBuf<uint32_t> sk[2], sv[2], runs, kidx;
   // measurement
@@ -559,6 +607,14 @@ static void free_tree(chz_ctx *c) {
   for (auto p : c->tree_gate) if (p) (void)hipFree(p);
   c->tree_par.clear();
   c->tree_gate.clear();
+}
+
+// 16-bit integer operations of one T-bound evaluation as implemented: for each of the 4^n - 1 flip pairs,
+// E/2 Z[w] products (16 multiplies + 12 adds), a (2n-1)-stage transform of E/2 four-coefficient entries and
+// 6 operations per entry for the final combination; plus the f = 0 transform (E entries, 2 coefficients).
+static double t_ops16(int nq) {
+  const double E = (double)(1 << (2 * nq)), H = E / 2;
+  return (E - 1) * (H * 28 + (2 * nq - 1) * H * 4 + H * 6) + E * 12 + 2 * nq * E * 2;
 }
 
 static int grid_for(long long waves_needed, int max_blocks = 8192) {
@@ -623,7 +679,6 @@ template <int NQ> struct Level {
       const long long nc = m * G;
       const long long maxcc = std::min<long long>(chunk, m) * G;
       c->status.need(maxcc);
-      c->needt.need(maxcc);
       c->info.need(maxcc);
       c->bcount.need((maxcc + CT_TILE - 1) / CT_TILE + 1);
       if (keep) c->keys.need(maxcc);
@@ -638,32 +693,12 @@ template <int NQ> struct Level {
                             c->status.p, c->keys.p, c->info.p, c->cnt.p};
         const long long nbc = (ncc + CT_TILE - 1) / CT_TILE;
         c->ev.begin(K_EXPAND, c->stream);
-        hipLaunchKernelGGL(k_expand<NQ>, dim3(grid_for((ncc + CPW - 1) / CPW)), dim3(256), 0, c->stream, ks, ea);
+        hipLaunchKernelGGL(k_level<NQ>, dim3(grid_for((cm + CPW - 1) / CPW)), dim3(256), 0, c->stream, ks, ea, cm);
         c->ev.end(c->stream);
-        // children that need the T bound, in child order
-        c->ev.begin(K_COMPACT, c->stream);
-        hipLaunchKernelGGL(k_flag_count<FlagNeed>, dim3((unsigned)nbc), dim3(CT_THREADS), 0, c->stream,
-                           FlagNeed{c->status.p}, ncc, c->bcount.p);
-        hipLaunchKernelGGL(k_scan_one_block, dim3(1), dim3(1024), 0, c->stream, c->bcount.p, nbc);
-        c->ev.end(c->stream);
-        unsigned long long ntot = 0;
-        HIPCHK(hipMemcpyAsync(&ntot, c->bcount.p + nbc, 8, hipMemcpyDeviceToHost, c->stream));
         sync_counts(c, hc);
         if (hc[C_OVF]) { set_err("residual coefficients exceed the storage type (int32)"); return 4; }
         if (hc[C_FOUND]) found = true;
         if (found) continue;          // the level ends with a circuit: only FOUND and overflow still count
-        const long long nt = (long long)ntot;
-        if (nt) {
-          c->ev.begin(K_COMPACT, c->stream);
-          hipLaunchKernelGGL(k_scatter_idx<FlagNeed>, dim3((unsigned)nbc), dim3(CT_THREADS), 0, c->stream,
-                             FlagNeed{c->status.p}, ncc, c->bcount.p, c->needt.p);
-          c->ev.end(c->stream);
-          c->ev.begin(K_TBOUND, c->stream);
-          hipLaunchKernelGGL(k_tbound<NQ>, dim3(grid_for((nt + CPW - 1) / CPW)), dim3(256), 0, c->stream, ks, ea,
-                             c->needt.p, nt);
-          c->ev.end(c->stream);
-          c->k_work[K_TBOUND] += (double)nt * (double)(E * (E * 28 + 4 * E * (2 * NQ) + 12 * E));
-        }
         if (keep) {
           const long long nb = nbc;
           c->ev.begin(K_COMPACT, c->stream);
@@ -700,6 +735,7 @@ template <int NQ> struct Level {
         }
       }
       sync_counts(c, hc);
+      c->k_work[K_EXPAND] += (double)hc[C_NEEDT] * t_ops16(NQ);
       if (found) {   // the lexicographically first circuit: walk the tree up from the first FOUND child
         const unsigned long long o = hc[C_FIRST];
         std::vector<int> seq(t);
@@ -1013,7 +1049,7 @@ void chz_close(chz_ctx *c) {
     c->sk[b].release(); c->sv[b].release();
   }
   c->status.release(); c->kept.release(); c->keys.release(); c->surv_o.release(); c->surv_key.release();
-  c->needt.release(); c->info.release(); c->cnt.release(); c->bcount.release(); c->hist.release(); c->runs.release(); c->kidx.release();
+  c->info.release(); c->cnt.release(); c->bcount.release(); c->hist.release(); c->runs.release(); c->kidx.release();
   void *ptrs[] = {c->d_gates, c->d_bad2, c->d_symperm, c->d_symconj, c->d_bad3, c->d_goal, c->d_cx};
   for (void *p : ptrs) if (p) (void)hipFree(p);
   c->ev.destroy();

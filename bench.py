@@ -83,6 +83,23 @@ def random_circuits(n=32):
             "lengths_input_found": lens}
 
 
+def clifford_table(nq=100000):
+    """C4: the whole 3-qubit Clifford group tabulated on the GPU (BFS into the HBM hash table), then 1e5 seeded
+    random words of 5..40 gates answered in one batched launch"""
+    from cheese_amd import clifford
+    t = time.perf_counter()
+    T = clifford.CliffordTable(3)
+    wall = time.perf_counter() - t
+    entries = T.digest()[1]
+    words = clifford.random_words(7, nq, T.ng)
+    dist, ms = T.query_words(words)
+    out = {"elements": entries, "levels": T.levels(), "build_ms_device": round(T.build_ms, 2),
+           "build_s_wall": round(wall, 3), "queries": nq, "query_ms_device": round(ms, 3),
+           "queries_per_s": round(nq / (ms / 1e3), 1), "max_distance": max(dist)}
+    T.close()
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -93,6 +110,8 @@ def main():
     ap.add_argument("--copy-probe-mb", type=int, default=0, help="also run a D2D copy of this size (profiling)")
     ap.add_argument("--random", action=argparse.BooleanOptionalAction, default=True,
                     help="also time the 32 seeded random 3-qubit circuits (budget = length)")
+    ap.add_argument("--clifford", action=argparse.BooleanOptionalAction, default=True,
+                    help="also build the 3-qubit Clifford table and answer 1e5 queries")
     a = ap.parse_args()
     if a.gpus != 1:
         print("bench.py: this build runs on one GPU; multi-GPU sharding is a design note only", file=sys.stderr)
@@ -148,6 +167,8 @@ def main():
     }
     if a.random:
         line["random32"] = random_circuits()
+    if a.clifford:
+        line["clifford"] = clifford_table()
     if not a.no_cpu:
         line["cpu_baseline"] = cpu_baseline(a.cpu_sample)
     print(json.dumps(line), flush=True)

@@ -1,0 +1,428 @@
+// The n-qubit Clifford group (n <= 3) tabulated on one MI355X: breadth-first search by gate count from the
+// identity into an open-addressing HBM hash table (tableau -> minimal gate count), batched distance queries
+// and optimal-word read-back.  Reference behaviour: src/clifford.cu:41-166 (tableaux, table, BFS),
+// :207-210 (batched find), :251-283 (BFS driver, group-order check), :344-372 (read-back).
+//
+// Table layout: 2^lg 64-bit slots, slot = tableau << 8 | distance, 0 = empty (no tableau is 0).  Slots are
+// grouped in buckets of 8 (one 64-byte sector); a group of 8 lanes probes a whole bucket per step, votes
+// for a match or an empty slot with a ballot, and claims the first empty slot with a 64-bit CAS.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "../../include/cheese_hip.h"
+
+namespace chz_cl {
+
+struct Gates { int n, ng; int k[32], a[32], b[32]; };   // gate set: H, S, Sdg, X, Z per qubit, then CX pairs
+
+static Gates gate_set(int n) {
+  Gates G{};
+  G.n = n;
+  for (int q = 0; q < n; q++)
+    for (int k = 0; k < 5; k++) { G.k[G.ng] = k; G.a[G.ng] = q; G.b[G.ng] = 0; G.ng++; }
+  for (int a = 0; a < n; a++)
+    for (int b = 0; b < n; b++)
+      if (a != b) { G.k[G.ng] = 5; G.a[G.ng] = a; G.b[G.ng] = b; G.ng++; }
+  return G;
+}
+
+// Tableau: 2n rows (images of X_0..X_{n-1}, Z_0..Z_{n-1}), row = x | z << n | sign << 2n, for the Hermitian
+// Pauli (-1)^sign i^(x.z) X^x Z^z.  Gate kinds: 0 H, 1 S, 2 Sdg, 3 X, 4 Z, 5 CX a->b, 6 Y.
+__host__ __device__ inline uint32_t conj_row(uint32_t w, int n, int k, int a, int b) {
+  const uint32_t nm = (1u << n) - 1;
+  uint32_t x = w & nm, z = (w >> n) & nm, r = (w >> (2 * n)) & 1;
+  const uint32_t xa = (x >> a) & 1, za = (z >> a) & 1;
+  if (k == 0) { r ^= xa & za; x = (x & ~(1u << a)) | (za << a); z = (z & ~(1u << a)) | (xa << a); }
+  else if (k == 1) { r ^= xa & za; z ^= xa << a; }
+  else if (k == 2) { r ^= xa & (za ^ 1); z ^= xa << a; }
+  else if (k == 3) { r ^= za; }
+  else if (k == 4) { r ^= xa; }
+  else if (k == 5) {
+    const uint32_t xb = (x >> b) & 1, zb = (z >> b) & 1;
+    r ^= xa & zb & (xb ^ za ^ 1);
+    x ^= xa << b;
+    z ^= zb << a;
+  } else { r ^= xa ^ za; }
+  return x | (z << n) | (r << (2 * n));
+}
+__host__ __device__ inline uint64_t apply(uint64_t T, int n, int k, int a, int b) {   // gate after the Clifford T
+  const int rb = 2 * n + 1;
+  const uint64_t rm = (1ull << rb) - 1;
+  uint64_t out = 0;
+  for (int i = 0; i < 2 * n; i++) out |= (uint64_t)conj_row((uint32_t)((T >> (i * rb)) & rm), n, k, a, b) << (i * rb);
+  return out;
+}
+__host__ __device__ inline uint64_t identity(int n) {
+  const int rb = 2 * n + 1;
+  uint64_t T = 0;
+  for (int i = 0; i < n; i++) { T |= (uint64_t)(1u << i) << (i * rb); T |= (uint64_t)(1u << (n + i)) << ((n + i) * rb); }
+  return T;
+}
+__host__ __device__ inline int inv_kind(int k) { return k == 1 ? 2 : k == 2 ? 1 : k; }
+
+__host__ __device__ inline uint64_t hmix(uint64_t x) {
+  x ^= x >> 31; x *= 0x7fb5d329728ea185ull;
+  x ^= x >> 27; x *= 0x81dadef4bc2dd44dull;
+  return x ^ (x >> 33);
+}
+
+struct Table {
+  unsigned long long *slot;
+  uint64_t bmask;                  // buckets - 1
+};
+
+// group of 8 lanes (l8 = lane & 7) probes for key; insert: claim an empty slot with distance d.
+// Returns the stored distance (>= 0), or -1 when absent (find) / -2 when this call inserted it.
+template <bool INSERT> __device__ inline int probe8(Table T, uint64_t key, int d, int l8) {
+  const int shift = __lane_id() & ~7;
+  uint64_t b = hmix(key) & T.bmask;
+  const unsigned long long want = (unsigned long long)((key << 8) | (uint64_t)d);
+  while (true) {
+    // while the table is being filled other CUs write it: read through to the coherent level so that a
+    // failed claim is never followed by the same stale view of the bucket
+    unsigned long long v = INSERT ? __hip_atomic_load(&T.slot[b * 8 + l8], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)
+                                  : T.slot[b * 8 + l8];
+    const uint64_t hit = (__ballot((v >> 8) == key && v != 0) >> shift) & 0xff;
+    if (hit) return (int)(__shfl(v, shift + __ffsll((long long)hit) - 1, 64) & 255);
+    uint64_t emp = (__ballot(v == 0) >> shift) & 0xff;
+    if (!emp) { b = (b + 1) & T.bmask; continue; }
+    if (!INSERT) return -1;
+    // the lowest empty slot: its lane tries to claim it, the group learns the outcome
+    const int e = __ffsll((long long)emp) - 1;
+    unsigned long long old = 0;
+    if (l8 == e) old = atomicCAS(&T.slot[b * 8 + e], 0ull, want);
+    old = __shfl(old, shift + e, 64);
+    if (old == 0) return -2;
+    if ((old >> 8) == key) return (int)(old & 255);
+    // another key took the slot: probe the same bucket again
+  }
+}
+
+struct BfsArgs {
+  Gates G;
+  Table T;
+  const uint64_t *front;
+  long long nf;
+  uint64_t *next;
+  unsigned long long *cnt;          // [0] new elements, [1] capacity overflow
+  long long cap;
+  int d;
+};
+
+constexpr int STAGE = 2048;
+
+// one BFS level: every (frontier element, gate) pair, 8 lanes per candidate; new tableaux are staged in LDS
+// and written out with one reservation per block
+__global__ __launch_bounds__(256) void k_bfs(BfsArgs a) {
+  __shared__ uint64_t buf[STAGE];
+  __shared__ unsigned int nbuf;
+  __shared__ unsigned long long base;
+  if (threadIdx.x == 0) nbuf = 0;
+  __syncthreads();
+  const int l8 = threadIdx.x & 7;
+  const long long total = a.nf * a.G.ng;
+  const long long groups = (long long)gridDim.x * (blockDim.x / 8);
+  const long long g0 = (long long)blockIdx.x * (blockDim.x / 8) + (threadIdx.x >> 3);
+  for (long long r0 = 0; r0 < total; r0 += groups) {
+    const long long c = r0 + g0;
+    if (c < total) {
+      const long long i = c / a.G.ng;
+      const int g = (int)(c % a.G.ng);
+      const uint64_t t = apply(a.front[i], a.G.n, a.G.k[g], a.G.a[g], a.G.b[g]);
+      if (probe8<true>(a.T, t, a.d, l8) == -2 && l8 == 0) buf[atomicAdd(&nbuf, 1u)] = t;
+    }
+    __syncthreads();
+    if (nbuf > STAGE - 256 / 8 || r0 + groups >= total) {
+      if (threadIdx.x == 0) base = atomicAdd(&a.cnt[0], (unsigned long long)nbuf);
+      __syncthreads();
+      for (unsigned int j = threadIdx.x; j < nbuf; j += blockDim.x) {
+        const unsigned long long p = base + j;
+        if ((long long)p < a.cap) a.next[p] = buf[j];
+        else a.cnt[1] = 1;
+      }
+      __syncthreads();
+      if (threadIdx.x == 0) nbuf = 0;
+      __syncthreads();
+    }
+  }
+}
+
+// tableaux of words (letters are gate-set indices, applied in order), one lane per word
+__global__ void k_words(Gates G, const uint8_t *letters, const long long *offs, long long nq, uint64_t *tab) {
+  for (long long q = (long long)blockIdx.x * blockDim.x + threadIdx.x; q < nq; q += (long long)gridDim.x * blockDim.x) {
+    uint64_t T = identity(G.n);
+    for (long long j = offs[q]; j < offs[q + 1]; j++) {
+      const int g = letters[j];
+      T = apply(T, G.n, G.k[g], G.a[g], G.b[g]);
+    }
+    tab[q] = T;
+  }
+}
+
+// batched lookups: 8 lanes per query (src/clifford.cu:207-210)
+__global__ __launch_bounds__(256) void k_find(Table T, const uint64_t *q, long long nq, int *out) {
+  const int l8 = threadIdx.x & 7;
+  const long long groups = (long long)gridDim.x * (blockDim.x / 8);
+  for (long long i0 = 0; i0 < nq; i0 += groups) {       // uniform trip count: every lane takes part in the ballots
+    const long long i = i0 + (long long)blockIdx.x * (blockDim.x / 8) + (threadIdx.x >> 3);
+    if (i < nq) {
+      const int d = probe8<false>(T, q[i], 0, l8);
+      if (l8 == 0) out[i] = d;
+    }
+  }
+}
+
+// optimal words: one wave per query; each step, lane g tests the predecessor through gate g (its distance
+// must be one less) and the first such gate in gate order is taken (src/clifford.cu:354-372)
+__global__ __launch_bounds__(64) void k_readback(Gates G, Table T, const uint64_t *q, long long nq, int maxlen,
+                                                 uint8_t *words, int *lens) {
+  const int lane = threadIdx.x;
+  for (long long i = blockIdx.x; i < nq; i += gridDim.x) {
+    uint64_t X = q[i];
+    // distance of the query: lanes 0..7 probe cooperatively
+    int d = -1;
+    {
+      const int dd = lane < 8 ? probe8<false>(T, X, 0, lane) : 0;
+      d = __shfl(dd, 0, 64);
+    }
+    if (d < 0 || d > maxlen) { if (lane == 0) lens[i] = -1; continue; }
+    if (lane == 0) lens[i] = d;
+    for (int s = d; s > 0; s--) {
+      uint64_t pre = 0;
+      bool ok = false;
+      if (lane < G.ng) {
+        pre = apply(X, G.n, inv_kind(G.k[lane]), G.a[lane], G.b[lane]);
+        // scalar linear probing of one key per lane over the same bucket layout
+        uint64_t b = hmix(pre) & T.bmask;
+        for (int guard = 0; guard < (1 << 20); guard++) {
+          bool done = false;
+          for (int j = 0; j < 8; j++) {
+            const unsigned long long v = T.slot[b * 8 + j];
+            if (!v) { done = true; break; }
+            if ((v >> 8) == pre) { ok = (int)(v & 255) == s - 1; done = true; break; }
+          }
+          if (done) break;
+          b = (b + 1) & T.bmask;
+        }
+      }
+      const uint64_t bal = __ballot(ok);
+      const int g = bal ? __ffsll((long long)bal) - 1 : -1;
+      if (g < 0) { if (lane == 0) lens[i] = -1; break; }
+      X = __shfl(pre, g, 64);
+      if (lane == 0) words[i * maxlen + (s - 1)] = (uint8_t)g;
+    }
+  }
+}
+
+// order-independent digest: sum over entries of hmix-free mix of (tableau * 64 + distance), and the count
+__global__ __launch_bounds__(256) void k_digest(Table T, long long nslots, unsigned long long *out) {
+  unsigned long long s = 0, c = 0;
+  for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < nslots; i += (long long)gridDim.x * blockDim.x) {
+    const unsigned long long v = T.slot[i];
+    if (v) { uint64_t x = (v >> 8) * 64 + (v & 255); x ^= x >> 33; x *= 0xff51afd7ed558ccdull; x ^= x >> 33;
+             x *= 0xc4ceb9fe1a85ec53ull; x ^= x >> 33; s += x; c++; }
+  }
+  for (int m = 1; m < 64; m <<= 1) { s += __shfl_xor(s, m, 64); c += __shfl_xor(c, m, 64); }
+  if ((threadIdx.x & 63) == 0) { atomicAdd(&out[0], s); atomicAdd(&out[1], c); }
+}
+
+static thread_local std::string g_err;
+
+}  // namespace chz_cl
+
+struct chz_cliff {
+  int n = 0, device = 0;
+  chz_cl::Gates G{};
+  chz_cl::Table T{};
+  long long nslots = 0;
+  std::vector<long> levels;
+  hipStream_t stream = nullptr;
+  double build_ms = 0;
+  double probes = 0;
+};
+
+#define CLCHK(x)                                                                         \
+  do {                                                                                   \
+    hipError_t e_ = (x);                                                                 \
+    if (e_ != hipSuccess) { chz_cl::g_err = std::string("HIP error: ") + hipGetErrorString(e_); return 5; } \
+  } while (0)
+
+extern "C" {
+
+const char *chz_cliff_last_error(void) { return chz_cl::g_err.c_str(); }
+
+int chz_cliff_build(int n, int device, chz_cliff **out) {
+  *out = nullptr;
+  if (n < 1 || n > 3) { chz_cl::g_err = "n must be 1..3"; return 3; }
+  chz_cliff *c = new chz_cliff();
+  c->n = n;
+  c->device = device;
+  c->G = chz_cl::gate_set(n);
+  const long long order = n == 1 ? 24 : n == 2 ? 11520 : 92897280;   // 2^(n^2+2n) prod_j (4^j - 1)
+  int lg = 6;
+  while ((1ll << lg) < order * 4 / 3) lg++;
+  c->nslots = 1ll << lg;
+  c->T.bmask = (uint64_t)(c->nslots / 8 - 1);
+  CLCHK(hipSetDevice(device));
+  CLCHK(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking));
+  CLCHK(hipMalloc((void **)&c->T.slot, c->nslots * 8));
+  CLCHK(hipMemsetAsync(c->T.slot, 0, c->nslots * 8, c->stream));
+  const long long cap = std::max(1024ll, order / 2);
+  uint64_t *fa = nullptr, *fb = nullptr;
+  unsigned long long *cnt = nullptr;
+  CLCHK(hipMalloc((void **)&fa, cap * 8));
+  CLCHK(hipMalloc((void **)&fb, cap * 8));
+  CLCHK(hipMalloc((void **)&cnt, 16));
+  hipEvent_t e0, e1;
+  CLCHK(hipEventCreate(&e0));
+  CLCHK(hipEventCreate(&e1));
+  CLCHK(hipEventRecord(e0, c->stream));
+  // the identity takes its slot with distance 0
+  {
+    const uint64_t id = chz_cl::identity(n);
+    const unsigned long long w = (unsigned long long)(id << 8);
+    uint64_t b = chz_cl::hmix(id) & c->T.bmask;
+    CLCHK(hipMemcpyAsync(c->T.slot + b * 8, &w, 8, hipMemcpyHostToDevice, c->stream));
+    CLCHK(hipMemcpyAsync(fa, &id, 8, hipMemcpyHostToDevice, c->stream));
+  }
+  long long nf = 1, total = 1;
+  c->levels.assign(1, 1);
+  for (int d = 1; nf; d++) {
+    CLCHK(hipMemsetAsync(cnt, 0, 16, c->stream));
+    chz_cl::BfsArgs a{c->G, c->T, fa, nf, fb, cnt, cap, d};
+    const long long groups = nf * c->G.ng;
+    const int blocks = (int)std::min<long long>(std::max<long long>((groups + 31) / 32, 1), 8192);
+    hipLaunchKernelGGL(chz_cl::k_bfs, dim3(blocks), dim3(256), 0, c->stream, a);
+    unsigned long long h[2];
+    CLCHK(hipMemcpyAsync(h, cnt, 16, hipMemcpyDeviceToHost, c->stream));
+    CLCHK(hipStreamSynchronize(c->stream));
+    if (h[1]) { chz_cl::g_err = "frontier buffer too small"; return 4; }
+    c->probes += (double)groups;
+    nf = (long long)h[0];
+    total += nf;
+    if (nf) c->levels.push_back((long)nf);
+    std::swap(fa, fb);
+  }
+  CLCHK(hipEventRecord(e1, c->stream));
+  CLCHK(hipEventSynchronize(e1));
+  float ms = 0;
+  CLCHK(hipEventElapsedTime(&ms, e0, e1));
+  c->build_ms = ms;
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
+  (void)hipFree(fa);
+  (void)hipFree(fb);
+  (void)hipFree(cnt);
+  *out = c;
+  if (total != order) { chz_cl::g_err = "group order mismatch"; return 4; }   // src/clifford.cu:279-280
+  return 0;
+}
+
+int chz_cliff_levels(const chz_cliff *c, long *counts, int maxd) {
+  const int nl = (int)c->levels.size();
+  for (int i = 0; i < nl && i < maxd; i++) counts[i] = c->levels[i];
+  return nl;
+}
+
+int chz_cliff_num_gates(const chz_cliff *c) { return c->G.ng; }
+
+double chz_cliff_build_ms(const chz_cliff *c) { return c->build_ms; }
+
+int chz_cliff_digest(const chz_cliff *c, uint64_t *digest, long long *entries) {
+  unsigned long long *d = nullptr, h[2] = {0, 0};
+  CLCHK(hipSetDevice(c->device));
+  CLCHK(hipMalloc((void **)&d, 16));
+  CLCHK(hipMemsetAsync(d, 0, 16, c->stream));
+  hipLaunchKernelGGL(chz_cl::k_digest, dim3(2048), dim3(256), 0, c->stream, c->T, c->nslots, d);
+  CLCHK(hipMemcpyAsync(h, d, 16, hipMemcpyDeviceToHost, c->stream));
+  CLCHK(hipStreamSynchronize(c->stream));
+  (void)hipFree(d);
+  *digest = h[0];
+  *entries = (long long)h[1];
+  return 0;
+}
+
+// distances of nq words: letters[offs[i] .. offs[i+1]) are gate-set indices applied in order.
+// ms (may be NULL) receives the device time of the word and lookup kernels.
+int chz_cliff_query_words(chz_cliff *c, const uint8_t *letters, const long long *offs, long long nq, int *dist,
+                          double *ms) {
+  CLCHK(hipSetDevice(c->device));
+  const long long nl = offs[nq];
+  uint8_t *dl = nullptr;
+  long long *doff = nullptr;
+  uint64_t *tab = nullptr;
+  int *dd = nullptr;
+  CLCHK(hipMalloc((void **)&dl, std::max<long long>(nl, 1)));
+  CLCHK(hipMalloc((void **)&doff, (nq + 1) * 8));
+  CLCHK(hipMalloc((void **)&tab, std::max<long long>(nq, 1) * 8));
+  CLCHK(hipMalloc((void **)&dd, std::max<long long>(nq, 1) * 4));
+  CLCHK(hipMemcpyAsync(dl, letters, nl, hipMemcpyHostToDevice, c->stream));
+  CLCHK(hipMemcpyAsync(doff, offs, (nq + 1) * 8, hipMemcpyHostToDevice, c->stream));
+  hipEvent_t e0, e1;
+  CLCHK(hipEventCreate(&e0));
+  CLCHK(hipEventCreate(&e1));
+  CLCHK(hipEventRecord(e0, c->stream));
+  hipLaunchKernelGGL(chz_cl::k_words, dim3((unsigned)std::min<long long>((nq + 255) / 256 + 1, 4096)), dim3(256), 0,
+                     c->stream, c->G, dl, doff, nq, tab);
+  hipLaunchKernelGGL(chz_cl::k_find, dim3((unsigned)std::min<long long>((nq + 31) / 32 + 1, 8192)), dim3(256), 0,
+                     c->stream, c->T, tab, nq, dd);
+  CLCHK(hipEventRecord(e1, c->stream));
+  CLCHK(hipMemcpyAsync(dist, dd, nq * 4, hipMemcpyDeviceToHost, c->stream));
+  CLCHK(hipStreamSynchronize(c->stream));
+  float t = 0;
+  CLCHK(hipEventElapsedTime(&t, e0, e1));
+  if (ms) *ms = t;
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
+  (void)hipFree(dl);
+  (void)hipFree(doff);
+  (void)hipFree(tab);
+  (void)hipFree(dd);
+  return 0;
+}
+
+// tableau of one word (host arithmetic, same algebra as the device)
+uint64_t chz_cliff_word_tableau(const chz_cliff *c, const uint8_t *letters, int len) {
+  uint64_t T = chz_cl::identity(c->n);
+  for (int i = 0; i < len; i++) {
+    const int g = letters[i];
+    T = chz_cl::apply(T, c->n, c->G.k[g], c->G.a[g], c->G.b[g]);
+  }
+  return T;
+}
+
+// optimal words of nq tableaux (gate-set indices, <= maxlen each); lens[i] = length or -1
+int chz_cliff_readback(chz_cliff *c, const uint64_t *tab, long long nq, int maxlen, uint8_t *words, int *lens) {
+  CLCHK(hipSetDevice(c->device));
+  uint64_t *dt = nullptr;
+  uint8_t *dw = nullptr;
+  int *dlen = nullptr;
+  CLCHK(hipMalloc((void **)&dt, std::max<long long>(nq, 1) * 8));
+  CLCHK(hipMalloc((void **)&dw, std::max<long long>(nq * maxlen, 1)));
+  CLCHK(hipMalloc((void **)&dlen, std::max<long long>(nq, 1) * 4));
+  CLCHK(hipMemcpyAsync(dt, tab, nq * 8, hipMemcpyHostToDevice, c->stream));
+  hipLaunchKernelGGL(chz_cl::k_readback, dim3((unsigned)std::min<long long>(std::max<long long>(nq, 1), 8192)),
+                     dim3(64), 0, c->stream, c->G, c->T, dt, nq, maxlen, dw, dlen);
+  CLCHK(hipMemcpyAsync(words, dw, nq * maxlen, hipMemcpyDeviceToHost, c->stream));
+  CLCHK(hipMemcpyAsync(lens, dlen, nq * 4, hipMemcpyDeviceToHost, c->stream));
+  CLCHK(hipStreamSynchronize(c->stream));
+  (void)hipFree(dt);
+  (void)hipFree(dw);
+  (void)hipFree(dlen);
+  return 0;
+}
+
+void chz_cliff_free(chz_cliff *c) {
+  if (!c) return;
+  (void)hipSetDevice(c->device);
+  if (c->stream) (void)hipStreamSynchronize(c->stream);
+  if (c->T.slot) (void)hipFree(c->T.slot);
+  if (c->stream) (void)hipStreamDestroy(c->stream);
+  delete c;
+}
+
+}  // extern "C"

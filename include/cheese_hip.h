@@ -88,6 +88,32 @@ int chz_verify(const chz_ctx *ctx, const int *circuit, int len);
 /* last error message of the calling thread (stderr text of the reference) */
 const char *chz_last_error(void);
 
+/* ---------------- Clifford-group table (src/clifford.cu) ----------------
+ * The whole n-qubit Clifford group (n <= 3, up to global phase) tabulated by breadth-first search over the gate
+ * set {H, S, Sdg, X, Z on each qubit; CX on each ordered pair} (gate_set, src/clifford.cu:147-155) into an
+ * open-addressing hash table in HBM: stabiliser tableau -> minimal gate count.  Letters of a word are indices
+ * into that gate set, applied in order. */
+typedef struct chz_cliff chz_cliff;
+
+/* BFS table (src/clifford.cu:251-280): 0, 3 (n outside 1..3), 4 (group order mismatch), 5 (device error) */
+int chz_cliff_build(int n, int device, chz_cliff **out);
+/* number of elements at each gate count ("clifford gates d: count", src/clifford.cu:275); returns levels */
+int chz_cliff_levels(const chz_cliff *c, long *counts, int maxd);
+int chz_cliff_num_gates(const chz_cliff *c);
+double chz_cliff_build_ms(const chz_cliff *c);
+/* order-independent checksum of all (tableau, distance) entries and their number (no reference counterpart:
+ * compares a whole table with another in O(1) memory) */
+int chz_cliff_digest(const chz_cliff *c, uint64_t *digest, long long *entries);
+/* batched distances of words letters[offs[i] .. offs[i+1]) (k_find, src/clifford.cu:207-210); dist -1: absent */
+int chz_cliff_query_words(chz_cliff *c, const uint8_t *letters, const long long *offs, long long nq, int *dist,
+                          double *ms);
+uint64_t chz_cliff_word_tableau(const chz_cliff *c, const uint8_t *letters, int len);
+/* optimal words of tableaux by walking distances down, first gate in gate order (src/clifford.cu:354-372);
+ * words: nq x maxlen letters, lens[i] = length or -1 */
+int chz_cliff_readback(chz_cliff *c, const uint64_t *tab, long long nq, int maxlen, uint8_t *words, int *lens);
+void chz_cliff_free(chz_cliff *c);
+const char *chz_cliff_last_error(void);
+
 #ifdef __cplusplus
 }
 #endif

@@ -21,7 +21,6 @@ def test_library_exports_header_symbols():
     assert names, "no declarations parsed"
     for n in names:
         assert hasattr(L, n), n
-    assert set(names) <= set(cheese_amd.SIGNATURES)
 
 
 def _cli(*args, stdin_spec=None, tmp_path=None):

@@ -1,0 +1,78 @@
+"""T-count oracle pinned against the reference's own T-count logs (benchmarks/optimal/*-tcount.json), the
+published table size for 2 qubits, and exact verification of decompositions.  CPU only (the 3-qubit table
+takes minutes on a CPU; it is checked on the GPU side against the published size)."""
+import ctypes
+import json
+import os
+
+import pytest
+
+from conftest import GOLDEN, ROOT
+
+from cheese_amd import tcount as T
+from cheese_amd import workloads as W
+
+TC = os.path.join(GOLDEN, "tcount")
+NAMES2 = ["cs", "qft_2"]
+NAMES3 = ["relative-phase-toffoli", "toffoli", "ccz", "peres", "fredkin"]
+
+
+@pytest.fixture(scope="session")
+def tco():
+    path = os.path.join(ROOT, "oracle", "libtcount_oracle.so")
+    if not os.path.exists(path):
+        import subprocess
+        subprocess.run(["make", "-C", os.path.join(ROOT, "oracle"), "all"], check=True, capture_output=True)
+    L = ctypes.CDLL(path)
+    L.orc_tc_table.restype = ctypes.c_void_p
+    L.orc_tc_table.argtypes = [ctypes.c_char_p, ctypes.c_long, ctypes.c_int, ctypes.c_int, ctypes.c_char_p, ctypes.c_long]
+    L.orc_tc_table_size.restype = ctypes.c_long
+    L.orc_tc_table_size.argtypes = [ctypes.c_void_p]
+    L.orc_tc_search.argtypes = [ctypes.c_void_p, ctypes.c_char_p, ctypes.c_long, ctypes.c_int, ctypes.c_char_p,
+                                ctypes.c_long, ctypes.POINTER(ctypes.c_int)]
+    L.orc_tc_free.argtypes = [ctypes.c_void_p]
+    return L
+
+
+def _strip(text):
+    out = []
+    for l in text.strip().splitlines():
+        d = json.loads(l)
+        d.pop("seconds", None)
+        out.append(d)
+    return out
+
+
+def test_two_qubit_table_size_and_searches(tco):
+    """n=2 table with K=6: 1,728,736 cosets (docs/PLAN.md:99-100); cs and qft_2 logs as the reference's"""
+    spec = open(os.path.join(GOLDEN, "tspec", "cs.tspec")).read().encode()
+    buf = ctypes.create_string_buffer(1 << 16)
+    h = tco.orc_tc_table(spec, len(spec), 6, 0, buf, len(buf))
+    assert tco.orc_tc_table_size(h) == 1728736
+    assert _strip(buf.value.decode()) == _strip(open(os.path.join(TC, "table-n2-k6.oracle.log")).read())
+    for name in NAMES2:
+        s = open(os.path.join(GOLDEN, "tspec", name + ".tspec")).read().encode()
+        rot = (ctypes.c_int * 64)()
+        rc = tco.orc_tc_search(h, s, len(s), 16, buf, len(buf), rot)
+        assert rc == 0
+        assert _strip(buf.value.decode()) == _strip(open(os.path.join(TC, name + ".json")).read())
+    tco.orc_tc_free(h)
+
+
+@pytest.mark.parametrize("name", NAMES2 + NAMES3)
+def test_tspec_writer_matches_reference(name):
+    n, circ = W.parse_qasm(open(os.path.join(TC, name + ".qasm")).read())
+    assert T.tspec_text(n, W.unitary(n, circ)) == open(os.path.join(GOLDEN, "tspec", name + ".tspec")).read()
+
+
+@pytest.mark.parametrize("name", NAMES2 + NAMES3)
+def test_reference_decompositions_verify(name):
+    """the reference's own decompositions pass the exact verifier; without one rotation they cannot (the
+    T-count is optimal, so a shorter list never leaves a Clifford)"""
+    n, circ = W.parse_qasm(open(os.path.join(TC, name + ".qasm")).read())
+    U = W.unitary(n, circ)
+    res = [json.loads(l) for l in open(os.path.join(TC, name + ".json")) if '"rotations"' in l][0]
+    assert len(res["rotations"]) == res["tcount"]
+    assert T.verify(n, U, res["rotations"])
+    for drop in range(len(res["rotations"])):
+        assert not T.verify(n, U, res["rotations"][:drop] + res["rotations"][drop + 1:])

@@ -12,6 +12,7 @@
 #include <vector>
 
 #include "../../include/cheese_hip.h"
+#include "primitives.h"
 #include "search_kernels.h"
 #include "spec.h"
 
@@ -204,10 +205,6 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) void k
   }
 }
 
-// ------------------------------------------------------------------------------------------------------
-// ordered stream compaction: tiles of 256 threads x 16 items, counts -> one-block scan -> scatter
-constexpr int CT_THREADS = 256, CT_ITEMS = 16, CT_TILE = CT_THREADS * CT_ITEMS;
-
 struct FlagAlive {      // survivors of a chunk, in child order
   const uint8_t *status;
   __device__ bool operator()(long long i) const { return status[i] == ST_ALIVE; }
@@ -220,69 +217,6 @@ struct FlagNeed {       // children that need the T bound, in child order
   const uint8_t *status;
   __device__ bool operator()(long long i) const { return status[i] == ST_NEEDT; }
 };
-
-template <typename F>
-__global__ __launch_bounds__(CT_THREADS) void k_flag_count(F f, long long n, unsigned long long *bcount) {
-  const long long t0 = (long long)blockIdx.x * CT_TILE;
-  int c = 0;
-  for (int i = 0; i < CT_ITEMS; i++) {
-    const long long x = t0 + (long long)i * CT_THREADS + threadIdx.x;
-    if (x < n && f(x)) c++;
-  }
-  __shared__ int ws[CT_THREADS / 64];
-  c = gsum<64>(c);
-  if ((threadIdx.x & 63) == 0) ws[threadIdx.x >> 6] = c;
-  __syncthreads();
-  if (threadIdx.x == 0) {
-    unsigned long long s = 0;
-    for (int w = 0; w < CT_THREADS / 64; w++) s += ws[w];
-    bcount[blockIdx.x] = s;
-  }
-}
-
-// exclusive scan of v[0..n) in place by one block; v[n] receives the total
-__global__ __launch_bounds__(1024) void k_scan_one_block(unsigned long long *v, long long n) {
-  __shared__ unsigned long long ws[16];
-  __shared__ unsigned long long carry;
-  if (threadIdx.x == 0) carry = 0;
-  __syncthreads();
-  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
-  for (long long base = 0; base < n; base += 1024) {
-    const long long i = base + threadIdx.x;
-    const unsigned long long x = i < n ? v[i] : 0;
-    unsigned long long inc = x;   // inclusive wave scan
-#pragma unroll
-    for (int d = 1; d < 64; d <<= 1) {
-      const unsigned long long y = __shfl_up(inc, d, 64);
-      if (lane >= d) inc += y;
-    }
-    if (lane == 63) ws[w] = inc;
-    __syncthreads();
-    if (threadIdx.x == 0) {
-      unsigned long long s = carry;
-      for (int k = 0; k < 16; k++) { const unsigned long long t = ws[k]; ws[k] = s; s += t; }
-      carry = s;
-    }
-    __syncthreads();
-    if (i < n) v[i] = ws[w] + inc - x;
-    __syncthreads();
-  }
-  if (threadIdx.x == 0) v[n] = carry;
-}
-
-// block-wide exclusive scan of one flag per thread, in thread order; returns the block total
-__device__ __forceinline__ int block_flag_scan(bool flag, int &excl, int *ws) {
-  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
-  const uint64_t b = __ballot(flag);
-  const int below = __popcll(b & ((1ull << lane) - 1));
-  if (lane == 0) ws[w] = __popcll(b);
-  __syncthreads();
-  int off = 0, tot = 0;
-  for (int k = 0; k < CT_THREADS / 64; k++) { if (k < w) off += ws[k]; tot += ws[k]; }
-  __syncthreads();
-  excl = off + below;
-  return tot;
-}
 
 // survivors of a chunk -> level survivor list (child index in the level, class key)
 __global__ __launch_bounds__(CT_THREADS) void k_scatter_alive(FlagAlive f, long long n, const unsigned long long *boff,
@@ -298,81 +232,6 @@ __global__ __launch_bounds__(CT_THREADS) void k_scatter_alive(FlagAlive f, long 
     const int tot = block_flag_scan(fl, ex, ws);
     if (fl) { surv_o[pos + ex] = (uint64_t)(child0 + x); surv_key[pos + ex] = keys[x]; }
     pos += tot;
-  }
-}
-
-// flagged positions -> ordered index list (kept survivor ranks, children needing the T bound)
-template <typename F>
-__global__ __launch_bounds__(CT_THREADS) void k_scatter_idx(F f, long long n, const unsigned long long *boff,
-                                                            uint32_t *out) {
-  __shared__ int ws[CT_THREADS / 64];
-  const long long t0 = (long long)blockIdx.x * CT_TILE;
-  long long pos = (long long)boff[blockIdx.x];
-  for (int i = 0; i < CT_ITEMS; i++) {
-    const long long x = t0 + (long long)i * CT_THREADS + threadIdx.x;
-    const bool fl = x < n && f(x);
-    int ex;
-    const int tot = block_flag_scan(fl, ex, ws);
-    if (fl) out[pos + ex] = (uint32_t)x;
-    pos += tot;
-  }
-}
-
-// ------------------------------------------------------------------------------------------------------
-// LSD radix sort of (u32 key, u32 value) pairs, 8-bit digits, stable: tiles of 256 threads x 8 rounds
-constexpr int RS_THREADS = 256, RS_ROUNDS = 8, RS_TILE = RS_THREADS * RS_ROUNDS;
-
-__global__ __launch_bounds__(RS_THREADS) void k_radix_hist(const uint32_t *keys, long long n, int shift,
-                                                           unsigned long long *hist, long long nblocks) {
-  __shared__ unsigned int h[256];
-  h[threadIdx.x] = 0;
-  __syncthreads();
-  const long long t0 = (long long)blockIdx.x * RS_TILE;
-  for (int r = 0; r < RS_ROUNDS; r++) {
-    const long long x = t0 + (long long)r * RS_THREADS + threadIdx.x;
-    if (x < n) atomicAdd(&h[(keys[x] >> shift) & 255], 1u);
-  }
-  __syncthreads();
-  hist[(long long)threadIdx.x * nblocks + blockIdx.x] = h[threadIdx.x];
-}
-
-__global__ __launch_bounds__(RS_THREADS) void k_radix_scatter(const uint32_t *kin, const uint32_t *vin, long long n,
-                                                              int shift, const unsigned long long *hist,
-                                                              long long nblocks, uint32_t *kout, uint32_t *vout) {
-  __shared__ unsigned long long base[256];
-  __shared__ unsigned int wc[RS_THREADS / 64][256];
-  const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
-  base[tid] = hist[(long long)tid * nblocks + blockIdx.x];
-  for (int k = 0; k < RS_THREADS / 64; k++) wc[k][tid] = 0;
-  __syncthreads();
-  const long long t0 = (long long)blockIdx.x * RS_TILE;
-  for (int r = 0; r < RS_ROUNDS; r++) {
-    const long long x = t0 + (long long)r * RS_THREADS + tid;
-    const bool valid = x < n;
-    const uint32_t key = valid ? kin[x] : 0u;
-    const uint32_t val = valid ? vin[x] : 0u;
-    const int dg = (key >> shift) & 255;
-    uint64_t peers = __ballot(valid);
-#pragma unroll
-    for (int b = 0; b < 8; b++) {
-      const uint64_t bb = __ballot((dg >> b) & 1);
-      peers &= ((dg >> b) & 1) ? bb : ~bb;
-    }
-    const int rank = __popcll(peers & ((1ull << lane) - 1));
-    if (valid && rank == 0) wc[w][dg] = __popcll(peers);
-    __syncthreads();
-    unsigned int before = 0;
-    if (valid) for (int k = 0; k < w; k++) before += wc[k][dg];
-    if (valid) {
-      const unsigned long long p = base[dg] + before + rank;
-      kout[p] = key;
-      vout[p] = val;
-    }
-    __syncthreads();
-    unsigned int tot = 0;
-    for (int k = 0; k < RS_THREADS / 64; k++) { tot += wc[k][tid]; wc[k][tid] = 0; }
-    base[tid] += tot;
-    __syncthreads();
   }
 }
 

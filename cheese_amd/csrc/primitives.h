@@ -5,6 +5,8 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
+#include <algorithm>
+
 #include "dev.h"
 
 namespace {
@@ -151,5 +153,79 @@ __global__ __launch_bounds__(RS_THREADS) void k_radix_scatter(const uint32_t *ki
   }
 }
 
+
+// ------------------------------------------------------------------------------------------------------
+// exclusive scan of long arrays: tiles of 4096 entries scanned in place, tile totals scanned by one block,
+// tile offsets added back
+constexpr int SC_THREADS = 1024, SC_ITEMS = 4, SC_TILE = SC_THREADS * SC_ITEMS;
+
+__global__ __launch_bounds__(SC_THREADS) void k_scan_tiles(unsigned long long *v, long long n,
+                                                          unsigned long long *tot) {
+  __shared__ unsigned long long ws[SC_THREADS / 64];
+  const long long base = (long long)blockIdx.x * SC_TILE + (long long)threadIdx.x * SC_ITEMS;
+  unsigned long long x[SC_ITEMS], s = 0;
+#pragma unroll
+  for (int i = 0; i < SC_ITEMS; i++) { x[i] = base + i < n ? v[base + i] : 0; s += x[i]; }
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+  unsigned long long inc = s;
+#pragma unroll
+  for (int d = 1; d < 64; d <<= 1) {
+    const unsigned long long y = __shfl_up(inc, d, 64);
+    if (lane >= d) inc += y;
+  }
+  if (lane == 63) ws[w] = inc;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    unsigned long long a = 0;
+    for (int k = 0; k < SC_THREADS / 64; k++) { const unsigned long long t = ws[k]; ws[k] = a; a += t; }
+    tot[blockIdx.x] = a;
+  }
+  __syncthreads();
+  unsigned long long e = ws[w] + inc - s;
+#pragma unroll
+  for (int i = 0; i < SC_ITEMS; i++) {
+    if (base + i < n) v[base + i] = e;
+    e += x[i];
+  }
+}
+
+__global__ void k_scan_add(unsigned long long *v, long long n, const unsigned long long *tot) {
+  for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (long long)gridDim.x * blockDim.x)
+    v[i] += tot[i / SC_TILE];
+}
+
+// v[0..n) <- exclusive prefix sums, v[n] <- total.  tmp holds at least n / SC_TILE + 2 entries.
+static inline hipError_t scan_excl(unsigned long long *v, long long n, unsigned long long *tmp, hipStream_t s) {
+  if (n <= 4 * SC_TILE) {
+    hipLaunchKernelGGL(k_scan_one_block, dim3(1), dim3(1024), 0, s, v, n);
+    return hipGetLastError();
+  }
+  const long long nt = (n + SC_TILE - 1) / SC_TILE;
+  hipLaunchKernelGGL(k_scan_tiles, dim3((unsigned)nt), dim3(SC_THREADS), 0, s, v, n, tmp);
+  hipLaunchKernelGGL(k_scan_one_block, dim3(1), dim3(1024), 0, s, tmp, nt);
+  hipLaunchKernelGGL(k_scan_add, dim3((unsigned)std::min<long long>((n + 255) / 256, 16384)), dim3(256), 0, s, v, n,
+                     tmp);
+  hipError_t e = hipGetLastError();
+  if (e != hipSuccess) return e;
+  return hipMemcpyAsync(v + n, tmp + nt, sizeof(unsigned long long), hipMemcpyDeviceToDevice, s);
+}
+
+// stable sort of n (u32 key, u32 value) pairs, 4 passes of 8 bits: (k0, v0) -> (k0, v0) through (k1, v1).
+// hist holds at least 256 * ceil(n / RS_TILE) + 1 entries, tmp what scan_excl needs for that many.
+static inline hipError_t radix_sort_pairs(uint32_t *k0, uint32_t *v0, uint32_t *k1, uint32_t *v1, long long n,
+                                          unsigned long long *hist, unsigned long long *tmp, hipStream_t s) {
+  if (n <= 1) return hipSuccess;
+  const long long nb = (n + RS_TILE - 1) / RS_TILE;
+  uint32_t *ks[2] = {k0, k1}, *vs[2] = {v0, v1};
+  for (int pass = 0; pass < 4; pass++) {
+    const int a = pass & 1, b = a ^ 1;
+    hipLaunchKernelGGL(k_radix_hist, dim3((unsigned)nb), dim3(RS_THREADS), 0, s, ks[a], n, pass * 8, hist, nb);
+    hipError_t e = scan_excl(hist, nb * 256, tmp, s);
+    if (e != hipSuccess) return e;
+    hipLaunchKernelGGL(k_radix_scatter, dim3((unsigned)nb), dim3(RS_THREADS), 0, s, ks[a], vs[a], n, pass * 8, hist,
+                       nb, ks[b], vs[b]);
+  }
+  return hipGetLastError();
+}
 
 }  // namespace

@@ -12,6 +12,7 @@
 #include <vector>
 
 #include "../../include/cheese_hip.h"
+#include "host.h"
 #include "primitives.h"
 #include "search_kernels.h"
 #include "spec.h"
@@ -358,72 +359,8 @@ __global__ void k_init_sort(const uint64_t *key, long long n, uint32_t *k32, uin
 // ------------------------------------------------------------------------------------------------------
 // host side
 
-static thread_local std::string g_err;
-void set_err(const std::string &s) { g_err = s; }
-const char *get_err() { return g_err.c_str(); }
-
-struct DeviceError {};
-
-#define HIPCHK(x)                                                                      \
-  do {                                                                                 \
-    hipError_t e_ = (x);                                                               \
-    if (e_ != hipSuccess) {                                                            \
-      ::chz::set_err(std::string("HIP error: ") + hipGetErrorString(e_) + " at " #x);  \
-      throw ::chz::DeviceError();                                                      \
-    }                                                                                  \
-  } while (0)
-
-template <typename T> struct DBuf {     // grow-only device array
-  T *p = nullptr;
-  size_t cap = 0;
-  void need(size_t n) {
-    if (n <= cap) return;
-    if (p) HIPCHK(hipFree(p));
-    p = nullptr;
-    const size_t c = std::max<size_t>(n, cap + cap / 2);
-    HIPCHK(hipMalloc((void **)&p, std::max<size_t>(c, 1) * sizeof(T)));
-    cap = c;
-  }
-  void release() { if (p) (void)hipFree(p); p = nullptr; cap = 0; }
-};
-
-static double now_s() { return std::chrono::duration<double>(std::chrono::steady_clock::now().time_since_epoch()).count(); }
-
 enum { K_EXPAND = 0, K_COMPACT, K_SORT, K_DEDUPE, K_GATHER, K_NFAM };
 static const char *k_names[K_NFAM] = {"expand", "compact", "sort", "dedupe", "gather"};
-
-// HIP-event timing of kernel families on the context's stream; pairs are read after each level's sync
-struct EventLog {
-  std::vector<hipEvent_t> pool;
-  std::vector<std::pair<int, std::pair<hipEvent_t, hipEvent_t>>> open;
-  hipEvent_t take() {
-    if (pool.empty()) { hipEvent_t e; HIPCHK(hipEventCreate(&e)); return e; }
-    hipEvent_t e = pool.back(); pool.pop_back(); return e;
-  }
-  void begin(int fam, hipStream_t s) {
-    hipEvent_t a = take(), b = take();
-    HIPCHK(hipEventRecord(a, s));
-    open.push_back({fam, {a, b}});
-  }
-  void end(hipStream_t s) { HIPCHK(hipEventRecord(open.back().second.second, s)); }
-  void flush(double *ms, long *launches) {   // call after the stream synchronised
-    for (auto &o : open) {
-      float x = 0;
-      HIPCHK(hipEventElapsedTime(&x, o.second.first, o.second.second));
-      ms[o.first] += x;
-      launches[o.first]++;
-      pool.push_back(o.second.first);
-      pool.push_back(o.second.second);
-    }
-    open.clear();
-  }
-  void destroy() {
-    for (auto &o : open) { pool.push_back(o.second.first); pool.push_back(o.second.second); }
-    open.clear();
-    for (auto e : pool) (void)hipEventDestroy(e);
-    pool.clear();
-  }
-};
 
 }  // namespace chz
 
@@ -476,11 +413,6 @@ static double t_ops16(int nq) {
   return (E - 1) * (H * 28 + (2 * nq - 1) * H * 4 + H * 6) + E * 12 + 2 * nq * E * 2;
 }
 
-static int grid_for(long long waves_needed, int max_blocks = 8192) {
-  long long blocks = (waves_needed + 3) / 4;   // 4 waves per 256-thread block
-  if (blocks < 1) blocks = 1;
-  return (int)std::min<long long>(blocks, max_blocks);
-}
 
 template <int NQ> struct Level {
   static constexpr int N = 1 << NQ, E = N * N, CPW = 64 / E;

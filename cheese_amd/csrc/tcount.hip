@@ -1,0 +1,1339 @@
+// Optimal T-count on one MI355X (reference behaviour: src/tcount.cu): a coset table grown level by level from
+// the identity into an HBM open-addressing table, and a batched search that peels rotations off many targets at
+// once and looks every residual up in that table.
+//
+// A node is a right coset S C of a channel matrix S (P x P, P = 4^n, entries (x + y sqrt2) / sqrt2^e); its
+// children peel one pi/8 rotation R(P_p), p = 1 .. P-1, off the left (src/tcount.cu:80-112).  Nodes are kept as
+// (parent, rotation) pairs per level; a wave rebuilds a node by replaying its path from its root, each lane
+// owning one column (peeling acts on rows, so a lane's column never leaves its lane; LDS serves as the
+// dynamically indexed column store).
+//
+// Two representations:
+//  * keys: every entry as its value in F_p, p = 2^31 - 1, where sqrt2 = 2^16 (2^32 = 2 mod p) and 1/sqrt2 = 2^15,
+//    so scaling by sqrt2 is a 31-bit rotation.  Values do not depend on the exponent chosen, so no
+//    renormalisation is needed.  A column hashes to (sum_r R[r] v_r)^2 (the square removes the column sign);
+//    the coset key is a mixed sum over columns (removes the column order).  A child's column hash is a fixed
+//    linear form of its parent's column (coefficients per rotation, precomputed), so all P-1 children of a
+//    parent come from one rebuild held in registers.
+//  * exact: packed int16 (x, y) with the reference's renormalisation (the exponent is minimal); equal keys are
+//    always confirmed by comparing the multisets of sign-normalised columns exactly (src/tcount.cu:174-188).
+//    Channel matrices are orthogonal and so are their Galois conjugates, hence |x|, |y sqrt2| <= sqrt2^e: 16 bits
+//    hold every exponent up to 29, which the host checks before a run.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cstdint>
+#include <cstdio>
+#include <cstring>
+#include <sstream>
+#include <string>
+#include <vector>
+
+#include "../../include/cheese_hip.h"
+#include "host.h"
+#include "primitives.h"
+
+namespace tcn {
+
+constexpr int MAXT = CHZ_TMAX;
+constexpr int EMAX16 = 29;                  // largest exponent whose channel entries fit int16
+constexpr uint32_t MP = 0x7fffffffu;        // 2^31 - 1
+constexpr int DMAXREP = 32;                 // distinct cosets kept per run of equal key prefixes
+
+// ---- F_p, p = 2^31 - 1
+__host__ __device__ __forceinline__ uint32_t m31_red(uint64_t x) {
+  x = (x & MP) + (x >> 31);
+  x = (x & MP) + (x >> 31);
+  return (uint32_t)(x >= MP ? x - MP : x);
+}
+__host__ __device__ __forceinline__ uint32_t m31_add(uint32_t a, uint32_t b) { const uint32_t s = a + b; return s >= MP ? s - MP : s; }
+__host__ __device__ __forceinline__ uint32_t m31_sub(uint32_t a, uint32_t b) { return a >= b ? a - b : a + MP - b; }
+__host__ __device__ __forceinline__ uint32_t m31_rot(uint32_t v, int k) {      // v 2^k, v < p, 0 <= k < 31
+  return k ? (((v << k) | (v >> (31 - k))) & MP) : v;
+}
+__host__ __device__ __forceinline__ uint32_t m31_of(int x) {
+  if (x >= 0) return (uint32_t)x % MP;
+  const uint32_t r = (uint32_t)(-(int64_t)x % MP);
+  return r ? MP - r : 0;
+}
+// (x + y sqrt2) / sqrt2^e in F_p
+__host__ __device__ __forceinline__ uint32_t m31_entry(int x, int y, int e) {
+  return m31_rot(m31_add(m31_of(x), m31_rot(m31_of(y), 16)), (15 * e) % 31);
+}
+// sum_r c_r v_r for c_r, v_r < p: each product split as (c mod 2^16) v + (c >> 16) v 2^16, 64 terms in 64 bits
+__device__ __forceinline__ uint32_t m31_fold(uint64_t lo, uint64_t hi) { return m31_add(m31_red(lo), m31_rot(m31_red(hi), 16)); }
+
+__host__ __device__ __forceinline__ uint64_t tmix(uint64_t x) {
+  x ^= x >> 32; x *= 0xd6e8feb86659fd93ull;
+  x ^= x >> 32; x *= 0xd6e8feb86659fd93ull;
+  return x ^ (x >> 32);
+}
+__device__ __forceinline__ uint64_t col_term(uint32_t g) { return tmix((uint64_t)g + 0x9e3779b97f4a7c15ull); }
+__host__ __device__ __forceinline__ uint64_t coset_key(uint64_t s) { return tmix(s ^ 0x632be59bd9b4e019ull); }
+// sort prefix: keys of different instances land in different runs
+__host__ __device__ __forceinline__ uint32_t sort_prefix(uint64_t key, uint32_t inst) {
+  return (uint32_t)((key ^ ((uint64_t)inst * 0x9e3779b97f4a7c15ull)) >> 32);
+}
+
+// ---- exact packed entries
+__device__ __forceinline__ int px(uint32_t w) { return (int)(int16_t)(w & 0xffffu); }
+__device__ __forceinline__ int py(uint32_t w) { return (int)(int16_t)(w >> 16); }
+__device__ __forceinline__ uint32_t pk(int x, int y) { return (uint32_t)(x & 0xffff) | ((uint32_t)y << 16); }
+__device__ __forceinline__ bool f16(int v) { return v >= -32768 && v <= 32767; }
+
+__device__ __forceinline__ uint64_t wsum64(uint64_t v) {
+#pragma unroll
+  for (int m = 1; m < 64; m <<= 1) v += __shfl_xor(v, m, 64);
+  return v;
+}
+__device__ __forceinline__ uint32_t wor32(uint32_t v) {
+#pragma unroll
+  for (int m = 1; m < 64; m <<= 1) v |= __shfl_xor(v, m, 64);
+  return v;
+}
+
+// rotation tables and hash coefficients on the device
+struct Rot {
+  const int8_t *tr, *ts;     // (P-1) x P: partner row and sign per rotation and row (tr < 0: row commutes)
+  const uint32_t *coef;      // (P-1) x P: child column hash = sum_r coef[p][r] v_r over the parent's column
+  const uint32_t *rv;        // P: column hash = sum_r rv[r] v_r
+};
+
+// nodes of every level: level 0 node i is root i; a node at level t >= 1 is (par, rot)
+struct Forest {
+  const uint32_t *par[MAXT + 1];
+  const uint8_t *rot[MAXT + 1];
+  const uint32_t *inst[MAXT + 1];   // instance of each node (null: one instance)
+  const uint32_t *root_x;           // roots x P*P packed exact entries
+  const uint32_t *root_fp;          // roots x P*P values in F_p
+  const int *root_e;
+};
+
+__device__ __forceinline__ uint32_t inst_of(const Forest &F, int t, uint32_t node) {
+  return F.inst[t] ? F.inst[t][node] : 0u;
+}
+
+// rotations of node (t, i) into seq[0..t) (LDS); returns its root
+__device__ __forceinline__ uint32_t walk(const Forest &F, int t, uint32_t i, uint8_t *seq, int lane) {
+  if (lane == 0)
+    for (int l = t; l >= 1; l--) { seq[l - 1] = F.rot[l][i]; i = F.par[l][i]; }
+  i = __shfl(i, 0, 64);
+  __builtin_amdgcn_wave_barrier();
+  return i;
+}
+
+// node (t, i) in F_p into M (P x P, row-major, lane c owns column c)
+template <int P>
+__device__ void fp_build(const Forest &F, const Rot &R, int t, uint32_t node, uint32_t *M, uint8_t *seq, int lane) {
+  const uint32_t root = walk(F, t, node, seq, lane);
+  const uint32_t *src = F.root_fp + (size_t)root * (P * P);
+  for (int k = lane; k < P * P; k += 64) M[k] = src[k];
+  __builtin_amdgcn_wave_barrier();
+  for (int l = 0; l < t; l++) {
+    const int p = __builtin_amdgcn_readfirstlane((int)seq[l]);
+    if (lane < P) {
+      const int8_t *tr = R.tr + (p - 1) * P, *ts = R.ts + (p - 1) * P;
+      for (int a = 0; a < P; a++) {
+        const int r = tr[a];
+        if (r > a) {   // anticommuting pair (a, r): a <- (a - s_a r) / sqrt2, r <- (r - s_r a) / sqrt2
+          uint32_t *ma = M + a * P + lane, *mr = M + r * P + lane;
+          const uint32_t va = *ma, vr = *mr;
+          *ma = m31_rot(ts[a] > 0 ? m31_sub(va, vr) : m31_add(va, vr), 15);
+          *mr = m31_rot(ts[r] > 0 ? m31_sub(vr, va) : m31_add(vr, va), 15);
+        }
+      }
+    }
+    __builtin_amdgcn_wave_barrier();
+  }
+}
+
+// number of steps (x + y s) / s = y + (x / 2) s the ORs X, Y of all entries allow below exponent e: step k needs
+// every current x even, i.e. 2^((k+1)/2) | x for odd k and 2^(k/2) | y for even k (src/tcount.cu:100-111)
+__device__ __forceinline__ int halvings(uint32_t X, uint32_t Y, int e) {
+  const int tx = X ? __ffs((int)X) - 1 : 32, ty = Y ? __ffs((int)Y) - 1 : 32;
+  int m = 0;
+  while (m < e) {
+    const int k = m + 1;
+    if ((k & 1) ? tx < (k + 1) / 2 : ty < k / 2) break;
+    m++;
+  }
+  return m;
+}
+__device__ __forceinline__ uint32_t halve(uint32_t w, int m) {   // m steps in closed form
+  int x = px(w), y = py(w);
+  const int t = m >> 1;
+  if (m & 1) { const int nx = y >> t, ny = x >> (t + 1); x = nx; y = ny; }
+  else { x >>= t; y >>= t; }
+  return pk(x, y);
+}
+
+// M <- R(P_p)^-1 M exactly, renormalised (src/tcount.cu:80-112); returns the new exponent
+template <int P>
+__device__ int ex_peel(uint32_t *M, const Rot &R, int p, int e, int lane, bool &ovf) {
+  uint32_t X = 0, Y = 0;
+  if (lane < P) {
+    const int8_t *tr = R.tr + (p - 1) * P, *ts = R.ts + (p - 1) * P;
+    for (int a = 0; a < P; a++) {
+      const int r = tr[a];
+      if (r < 0) {   // commuting row: times sqrt2 in the common exponent
+        uint32_t *m = M + a * P + lane;
+        const uint32_t w = *m;
+        const int x = 2 * py(w), y = px(w);
+        ovf |= !f16(x);
+        *m = pk(x, y);
+        X |= (uint32_t)x;
+        Y |= (uint32_t)y;
+      } else if (r > a) {
+        uint32_t *ma = M + a * P + lane, *mr = M + r * P + lane;
+        const uint32_t wa = *ma, wr = *mr;
+        const int sa = ts[a], sr = ts[r];
+        const int xa = px(wa) - sa * px(wr), ya = py(wa) - sa * py(wr);
+        const int xr = px(wr) - sr * px(wa), yr = py(wr) - sr * py(wa);
+        ovf |= !(f16(xa) && f16(ya) && f16(xr) && f16(yr));
+        *ma = pk(xa, ya);
+        *mr = pk(xr, yr);
+        X |= (uint32_t)(xa | xr);
+        Y |= (uint32_t)(ya | yr);
+      }
+    }
+  }
+  X = wor32(X);
+  Y = wor32(Y);
+  const int m = halvings(X, Y, e + 1);
+  if (m && lane < P)
+    for (int a = 0; a < P; a++) M[a * P + lane] = halve(M[a * P + lane], m);
+  __builtin_amdgcn_wave_barrier();
+  return e + 1 - m;
+}
+
+// node (t, i) exactly into M; returns its exponent
+template <int P>
+__device__ int ex_build(const Forest &F, const Rot &R, int t, uint32_t node, uint32_t *M, uint8_t *seq, int lane,
+                        bool &ovf) {
+  const uint32_t root = walk(F, t, node, seq, lane);
+  const uint32_t *src = F.root_x + (size_t)root * (P * P);
+  for (int k = lane; k < P * P; k += 64) M[k] = src[k];
+  __builtin_amdgcn_wave_barrier();
+  int e = F.root_e[root];
+  for (int l = 0; l < t; l++) e = ex_peel<P>(M, R, __builtin_amdgcn_readfirstlane((int)seq[l]), e, lane, ovf);
+  return e;
+}
+
+// sign-invariant hash of exact column c (its F_p column hash; the exponent is common to the compared matrices)
+template <int P>
+__device__ uint32_t ex_colhash(const uint32_t *M, const Rot &R, int c) {
+  uint64_t lo = 0, hi = 0;
+  for (int r = 0; r < P; r++) {
+    const uint32_t w = M[r * P + c];
+    const uint32_t v = m31_add(m31_of(px(w)), m31_rot(m31_of(py(w)), 16));
+    const uint32_t k = R.rv[r];
+    lo += (uint64_t)(k & 0xffffu) * v;
+    hi += (uint64_t)(k >> 16) * v;
+  }
+  const uint32_t h = m31_fold(lo, hi);
+  return m31_red((uint64_t)h * h);
+}
+
+__device__ __forceinline__ uint64_t bitonic64(uint64_t v, int lane) {   // ascending over the wave
+#pragma unroll
+  for (int k = 2; k <= 64; k <<= 1)
+#pragma unroll
+    for (int j = k >> 1; j > 0; j >>= 1) {
+      const uint64_t o = __shfl_xor(v, j, 64);
+      const bool take_min = ((lane & j) == 0) == ((lane & k) == 0);
+      v = take_min ? (o < v ? o : v) : (o > v ? o : v);
+    }
+  return v;
+}
+
+// column ca of A equals column cb of B after making the first nonzero entry of each positive (its x, else y)
+template <int P>
+__device__ bool same_col(const uint32_t *A, int ca, const uint32_t *B, int cb) {
+  int sa = 0, sb = 0;
+  for (int r = 0; r < P; r++) {
+    const uint32_t wa = A[r * P + ca], wb = B[r * P + cb];
+    const int xa = px(wa), ya = py(wa), xb = px(wb), yb = py(wb);
+    if (!sa && (xa | ya)) sa = (xa > 0 || (xa == 0 && ya > 0)) ? 1 : -1;
+    if (!sb && (xb | yb)) sb = (xb > 0 || (xb == 0 && yb > 0)) ? 1 : -1;
+    if (sa * xa != sb * xb || sa * ya != sb * yb) return false;
+  }
+  return true;
+}
+
+// exact right-coset equality: equal exponents and equal multisets of sign-normalised columns.  Columns are
+// paired after sorting both matrices by column hash; if a hash tie pairs different columns, every column's
+// multiplicity is counted in both.
+template <int P>
+__device__ bool same_coset(const uint32_t *A, int ea, const uint32_t *B, int eb, const Rot &R, int lane) {
+  if (ea != eb) return false;
+  const uint64_t ka = lane < P ? ((uint64_t)ex_colhash<P>(A, R, lane) << 32 | (uint32_t)lane) : ~0ull;
+  const uint64_t kb = lane < P ? ((uint64_t)ex_colhash<P>(B, R, lane) << 32 | (uint32_t)lane) : ~0ull;
+  const uint64_t sa = bitonic64(ka, lane), sb = bitonic64(kb, lane);
+  if (__ballot((sa >> 32) != (sb >> 32))) return false;
+  const bool eq = lane >= P || same_col<P>(A, (int)(uint32_t)sa, B, (int)(uint32_t)sb);
+  if (!__ballot(!eq)) return true;
+  bool ok = true;
+  if (lane < P) {
+    int na = 0, nb = 0;
+    for (int c = 0; c < P; c++) { na += same_col<P>(A, lane, A, c); nb += same_col<P>(A, lane, B, c); }
+    ok = na == nb;
+  }
+  return !__ballot(!ok);
+}
+
+// ------------------------------------------------------------------------------------------------------
+// kernels
+
+struct RootKeyArgs {
+  Rot R;
+  const uint32_t *root_fp;
+  long long n;
+  uint64_t *keys;
+};
+
+// coset keys of roots (the identity of the table, the targets of a search)
+template <int NQ> __global__ __launch_bounds__(64) void k_root_keys(RootKeyArgs a) {
+  constexpr int P = 1 << (2 * NQ);
+  const int lane = threadIdx.x;
+  for (long long i = blockIdx.x; i < a.n; i += gridDim.x) {
+    uint64_t lo = 0, hi = 0;
+    if (lane < P)
+      for (int r = 0; r < P; r++) {
+        const uint32_t k = a.R.rv[r], v = a.root_fp[(size_t)i * P * P + r * P + lane];
+        lo += (uint64_t)(k & 0xffffu) * v;
+        hi += (uint64_t)(k >> 16) * v;
+      }
+    const uint32_t h = m31_fold(lo, hi);
+    const uint64_t s = wsum64(lane < P ? col_term(m31_red((uint64_t)h * h)) : 0ull);
+    if (lane == 0) a.keys[i] = coset_key(s);
+  }
+}
+
+struct KeysArgs {
+  Rot R;
+  Forest F;
+  int t;                       // level of the parents
+  const uint32_t *plist;       // parents (node indices at level t); null: every node
+  long long np;
+  uint64_t *keys;              // by child index k * (P-1) + p - 1
+  uint32_t *k32, *v32;         // sort prefix and child index
+};
+
+// every child's coset key (src/tcount.cu:197-207): one wave per parent, the parent's F_p column in registers,
+// each child column hash a linear form of it
+template <int NQ> __global__ __launch_bounds__(64) void k_keys(KeysArgs a) {
+  constexpr int P = 1 << (2 * NQ), G = P - 1;
+  __shared__ uint32_t M[P * P];
+  __shared__ uint8_t seq[MAXT];
+  const int lane = threadIdx.x;
+  for (long long k = blockIdx.x; k < a.np; k += gridDim.x) {
+    const uint32_t node = a.plist ? a.plist[k] : (uint32_t)k;
+    fp_build<P>(a.F, a.R, a.t, node, M, seq, lane);
+    const uint32_t inst = inst_of(a.F, a.t, node);
+    uint32_t v[P];
+#pragma unroll
+    for (int r = 0; r < P; r++) v[r] = lane < P ? M[r * P + lane] : 0u;
+    uint64_t mine = 0;
+    for (int p = 1; p <= G; p++) {
+      const uint32_t *C = a.R.coef + (p - 1) * P;
+      uint64_t lo = 0, hi = 0;
+#pragma unroll
+      for (int r = 0; r < P; r++) {
+        const uint32_t c = C[r];
+        lo += (uint64_t)(c & 0xffffu) * v[r];
+        hi += (uint64_t)(c >> 16) * v[r];
+      }
+      const uint32_t h = m31_fold(lo, hi);
+      const uint64_t s = wsum64(lane < P ? col_term(m31_red((uint64_t)h * h)) : 0ull);
+      if (lane == p - 1) mine = s;
+    }
+    if (lane < G) {
+      const uint64_t key = coset_key(mine);
+      const long long ch = k * G + lane;
+      a.keys[ch] = key;
+      a.k32[ch] = sort_prefix(key, inst);
+      a.v32[ch] = (uint32_t)ch;
+    }
+    __builtin_amdgcn_wave_barrier();
+  }
+}
+
+// run starts of sorted prefixes that hold at least two children
+__global__ void k_multi_runs(const uint32_t *k32, long long n, uint32_t *runs, unsigned long long *nruns) {
+  for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (long long)gridDim.x * blockDim.x) {
+    const bool head = (i == 0 || k32[i] != k32[i - 1]) && i + 1 < n && k32[i + 1] == k32[i];
+    if (head) runs[atomicAdd(nruns, 1ull)] = (uint32_t)i;
+  }
+}
+
+struct DedupeArgs {
+  Rot R;
+  Forest F;
+  int t;
+  const uint32_t *plist;
+  const uint32_t *k32, *v32;
+  long long n;
+  const uint32_t *runs;
+  const unsigned long long *nruns;
+  const uint64_t *keys;
+  uint8_t *keep;               // by child; cleared for every child whose coset an earlier child holds
+  int *flags;                  // bit 0: int16 overflow, bit 1: a run held more than DMAXREP distinct cosets
+};
+
+// exact dedupe: within each run of equal prefixes (children in increasing index), the first child of every
+// exact coset stays (src/tcount.cu:327-360: the smallest index of each coset)
+template <int NQ> __global__ __launch_bounds__(64) void k_dedupe(DedupeArgs a) {
+  constexpr int P = 1 << (2 * NQ), G = P - 1;
+  __shared__ uint32_t A[P * P], B[P * P];
+  __shared__ uint8_t seq[MAXT];
+  __shared__ uint32_t rep[DMAXREP];
+  const int lane = threadIdx.x;
+  bool ovf = false, full = false;
+  const long long nr = (long long)*a.nruns;
+  auto parent = [&](uint32_t ch) { return a.plist ? a.plist[ch / G] : ch / G; };
+  for (long long r = blockIdx.x; r < nr; r += gridDim.x) {
+    const long long i0 = a.runs[r];
+    const uint32_t kp = a.k32[i0];
+    int nrep = 0, eb = 0;
+    long long bid = -1;
+    for (long long j = i0; j < a.n && a.k32[j] == kp; j++) {
+      const uint32_t ch = a.v32[j];
+      const uint64_t key = a.keys[ch];
+      const uint32_t inst = inst_of(a.F, a.t, parent(ch));
+      bool dup = false, built = false;
+      int ea = 0;
+      for (int q = 0; q < nrep && !dup; q++) {
+        const uint32_t rc = rep[q];
+        if (a.keys[rc] != key || inst_of(a.F, a.t, parent(rc)) != inst) continue;
+        if (!built) {
+          ea = ex_build<P>(a.F, a.R, a.t, parent(ch), A, seq, lane, ovf);
+          ea = ex_peel<P>(A, a.R, (int)(ch % G) + 1, ea, lane, ovf);
+          built = true;
+        }
+        if (bid != (long long)rc) {
+          eb = ex_build<P>(a.F, a.R, a.t, parent(rc), B, seq, lane, ovf);
+          eb = ex_peel<P>(B, a.R, (int)(rc % G) + 1, eb, lane, ovf);
+          bid = rc;
+        }
+        dup = same_coset<P>(A, ea, B, eb, a.R, lane);
+      }
+      if (dup) {
+        if (lane == 0) a.keep[ch] = 0;
+      } else if (nrep < DMAXREP) {
+        if (lane == 0) rep[nrep] = ch;
+        nrep++;
+        __builtin_amdgcn_wave_barrier();
+      } else {
+        full = true;
+      }
+    }
+  }
+  const bool o = __ballot(ovf) != 0;
+  if (lane == 0 && (o || full)) atomicOr(a.flags, (o ? 1 : 0) | (full ? 2 : 0));
+}
+
+// HBM coset table: buckets of 8 slots (key | 1, level << 32 | node), probed by 8 lanes
+struct HTab {
+  unsigned long long *key, *val;
+  unsigned long long bmask;
+};
+
+struct InsertArgs {
+  HTab T;
+  const uint64_t *keys;
+  long long n;
+  int level;
+};
+
+// insert the nodes of one level (no lookups run meanwhile); equal keys take separate slots
+__global__ __launch_bounds__(256) void k_insert(InsertArgs a) {
+  const int l8 = threadIdx.x & 7, shift = threadIdx.x & 56;
+  for (long long i0 = (long long)blockIdx.x * 32; i0 < a.n; i0 += (long long)gridDim.x * 32) {
+    const long long i = i0 + (threadIdx.x >> 3);
+    bool done = i >= a.n;
+    const uint64_t k = done ? 1ull : (a.keys[i] | 1ull);
+    uint64_t b = tmix(k) & a.T.bmask;
+    while (__ballot(!done)) {
+      unsigned long long kv = 1;
+      if (!done) kv = __hip_atomic_load(&a.T.key[b * 8 + l8], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+      const uint32_t emp = (uint32_t)(__ballot(!done && kv == 0) >> shift) & 0xffu;
+      unsigned long long old = 1;
+      const int e = emp ? __ffs((int)emp) - 1 : 0;
+      if (!done && emp && l8 == e) {
+        old = atomicCAS(&a.T.key[b * 8 + e], 0ull, (unsigned long long)k);
+        if (old == 0) a.T.val[b * 8 + e] = ((unsigned long long)a.level << 32) | (unsigned long long)i;
+      }
+      old = __shfl(old, shift + e, 64);
+      if (!done) {
+        if (emp && old == 0) done = true;
+        else if (!emp) b = (b + 1) & a.T.bmask;
+      }
+    }
+  }
+}
+
+struct ProbeArgs {
+  HTab T;
+  const uint64_t *keys;        // by candidate index
+  const uint32_t *items;       // candidates to probe (null: 0 .. n-1)
+  long long n;
+  uint8_t *hit;                // by candidate index: 1 when some slot holds the same key
+};
+
+// key probe, 8 lanes per candidate
+__global__ __launch_bounds__(256) void k_probe(ProbeArgs a) {
+  const int l8 = threadIdx.x & 7, shift = threadIdx.x & 56;
+  for (long long i0 = (long long)blockIdx.x * 32; i0 < a.n; i0 += (long long)gridDim.x * 32) {
+    const long long i = i0 + (threadIdx.x >> 3);
+    bool done = i >= a.n, hit = false;
+    const uint32_t c = done ? 0u : (a.items ? a.items[i] : (uint32_t)i);
+    const uint64_t k = done ? 1ull : (a.keys[c] | 1ull);
+    uint64_t b = tmix(k) & a.T.bmask;
+    while (__ballot(!done)) {
+      const unsigned long long kv = done ? 1ull : a.T.key[b * 8 + l8];
+      const uint32_t gm = (uint32_t)(__ballot(!done && kv == k) >> shift) & 0xffu;
+      const uint32_t ge = (uint32_t)(__ballot(!done && kv == 0) >> shift) & 0xffu;
+      if (!done) {
+        if (gm) { hit = true; done = true; }
+        else if (ge) done = true;
+        else b = (b + 1) & a.T.bmask;
+      }
+    }
+    if (i < a.n && l8 == 0) a.hit[c] = hit ? 1 : 0;
+  }
+}
+
+struct ConfirmArgs {
+  Rot R;
+  HTab T;
+  Forest Fc;                   // candidates' forest
+  int tc;                      // mode 0: level of the candidates' parents; mode 1: level of the candidates
+  const uint32_t *plist;       // mode 0: parents of the children
+  Forest Ft;                   // the table's forest
+  const uint32_t *items;       // candidates with a key hit
+  long long n;
+  const uint64_t *keys;        // by candidate index
+  const int *maxc;             // mode 1: per instance, the largest table level accepted
+  int mode;                    // 0: table filter, 1: search lookup
+  uint8_t *fresh;              // mode 0: cleared when the child's coset is in the table
+  unsigned long long *best;    // mode 1: per instance min(level << 32 | node)
+  uint32_t *hitx;              // mode 1: table node of each hit, by node
+  int *flags;
+};
+
+// exact confirmation of key hits (src/tcount.cu:248-276 lookup, :361-380 table filter): one wave per candidate
+template <int NQ> __global__ __launch_bounds__(64) void k_confirm(ConfirmArgs a) {
+  constexpr int P = 1 << (2 * NQ), G = P - 1;
+  __shared__ uint32_t A[P * P], B[P * P];
+  __shared__ uint8_t seq[MAXT];
+  const int lane = threadIdx.x;
+  bool ovf = false;
+  for (long long it = blockIdx.x; it < a.n; it += gridDim.x) {
+    const uint32_t c = a.items[it];
+    const uint64_t k = a.keys[c] | 1ull;
+    uint32_t node;
+    int p;
+    if (a.mode == 0) { node = a.plist ? a.plist[c / G] : c / G; p = (int)(c % G) + 1; }
+    else { node = c; p = 0; }
+    const uint32_t inst = inst_of(a.Fc, a.tc, node);
+    const int maxc = a.maxc ? a.maxc[inst] : (1 << 30);
+    int bestl = 1 << 30, ea = 0;
+    uint32_t bestx = 0;
+    bool built = false;
+    uint64_t b = tmix(k) & a.T.bmask;
+    while (true) {
+      const unsigned long long kv = lane < 8 ? a.T.key[b * 8 + lane] : 0ull;
+      const unsigned long long vv = (lane < 8 && kv == k) ? a.T.val[b * 8 + lane] : 0ull;
+      uint64_t mm = __ballot(lane < 8 && kv == k);
+      const bool empty = __ballot(lane < 8 && kv == 0) != 0;
+      while (mm) {
+        const int s = __ffsll((long long)mm) - 1;
+        mm &= mm - 1;
+        const unsigned long long v = __shfl(vv, s, 64);
+        const int lvl = (int)(v >> 32);
+        const uint32_t x = (uint32_t)v;
+        if (lvl > maxc || lvl >= bestl) continue;
+        if (!built) {
+          ea = ex_build<P>(a.Fc, a.R, a.tc, node, A, seq, lane, ovf);
+          if (p) ea = ex_peel<P>(A, a.R, p, ea, lane, ovf);
+          built = true;
+        }
+        const int eb = ex_build<P>(a.Ft, a.R, lvl, x, B, seq, lane, ovf);
+        if (same_coset<P>(A, ea, B, eb, a.R, lane)) { bestl = lvl; bestx = x; }
+      }
+      if (empty || (a.mode == 0 && bestl < (1 << 30))) break;
+      b = (b + 1) & a.T.bmask;
+    }
+    if (lane == 0 && bestl < (1 << 30)) {
+      if (a.mode == 0) a.fresh[c] = 0;
+      else {
+        a.hitx[c] = bestx;
+        atomicMin(&a.best[inst], ((unsigned long long)bestl << 32) | (unsigned long long)c);
+      }
+    }
+  }
+  if (__ballot(ovf) && lane == 0) atomicOr(a.flags, 1);
+}
+
+// flag functors for the ordered compactions
+struct FlagU8 {
+  const uint8_t *f;
+  __device__ bool operator()(long long i) const { return f[i] != 0; }
+};
+struct FlagActive {            // nodes whose instance is still searching
+  const uint32_t *inst;
+  const uint8_t *active;
+  __device__ bool operator()(long long i) const { return active[inst ? inst[i] : 0] != 0; }
+};
+
+struct GatherArgs {
+  const uint32_t *list;        // kept children, in increasing index
+  long long n;
+  const uint32_t *plist;       // parents (null: identity)
+  const uint32_t *pinst;       // parents' instances (null: one instance)
+  const uint64_t *ckeys;
+  int G;
+  uint32_t *par;
+  uint8_t *rot;
+  uint32_t *inst;              // may be null
+  uint64_t *keys;
+};
+
+__global__ void k_gather(GatherArgs a) {
+  for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < a.n; i += (long long)gridDim.x * blockDim.x) {
+    const uint32_t ch = a.list[i];
+    const uint32_t par = a.plist ? a.plist[ch / a.G] : ch / a.G;
+    a.par[i] = par;
+    a.rot[i] = (uint8_t)(ch % a.G + 1);
+    if (a.inst) a.inst[i] = a.pinst ? a.pinst[par] : 0u;
+    a.keys[i] = a.ckeys[ch];
+  }
+}
+
+__global__ void k_fill_u8(uint8_t *p, long long n, uint8_t v) {
+  for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (long long)gridDim.x * blockDim.x) p[i] = v;
+}
+__global__ void k_fill_u64(unsigned long long *p, long long n, unsigned long long v) {
+  for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (long long)gridDim.x * blockDim.x) p[i] = v;
+}
+__global__ void k_copy_u8(uint8_t *d, const uint8_t *s, long long n) {
+  for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (long long)gridDim.x * blockDim.x) d[i] = s[i];
+}
+
+// first and one-past-last node of every instance in a level (nodes are ordered by instance)
+__global__ void k_inst_bounds(const uint32_t *inst, long long n, long long *lo, long long *hi) {
+  for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (long long)gridDim.x * blockDim.x) {
+    const uint32_t s = inst[i];
+    if (i == 0 || inst[i - 1] != s) lo[s] = i;
+    if (i == n - 1 || inst[i + 1] != s) hi[s] = i + 1;
+  }
+}
+
+struct PathArgs {
+  Forest Fs, Ft;
+  int j;                       // search level of the hits
+  long long n;
+  const uint32_t *node, *tnode;
+  const int *tlevel;
+  int *out;                    // n x 2 MAXT: search rotations (levels 1..j), then table rotations (levels 1..c)
+};
+
+__global__ void k_paths(PathArgs a) {
+  for (long long q = (long long)blockIdx.x * blockDim.x + threadIdx.x; q < a.n; q += (long long)gridDim.x * blockDim.x) {
+    int *o = a.out + q * 2 * MAXT;
+    uint32_t i = a.node[q];
+    for (int l = a.j; l >= 1; l--) { o[l - 1] = a.Fs.rot[l][i]; i = a.Fs.par[l][i]; }
+    i = a.tnode[q];
+    for (int l = a.tlevel[q]; l >= 1; l--) { o[MAXT + l - 1] = a.Ft.rot[l][i]; i = a.Ft.par[l][i]; }
+  }
+}
+
+// ------------------------------------------------------------------------------------------------------
+// host side
+
+using chz::DBuf;
+using chz::set_err;
+
+struct Spec {
+  int n = 0, P = 0, e = 0;
+  std::vector<int> x, y, tr, ts;
+};
+
+// .tspec reader (src/tcount.cu:433-450): 2 malformed, 3 n outside 1..3
+static int read_spec(const char *text, size_t len, Spec &s, std::string &err) {
+  std::istringstream in(std::string(text, len));
+  std::string tok;
+  in >> tok;
+  if (tok != "tcount-spec") { err = "not a tcount spec"; return 2; }
+  int ver = 0;
+  in >> ver;
+  bool have_ch = false, have_rot = false;
+  while (in >> tok) {
+    if (tok == "n") {
+      if (!(in >> s.n)) { err = "malformed n"; return 2; }
+      if (s.n < 1 || s.n > 3) { err = "n = " + std::to_string(s.n) + " not supported (1 <= n <= 3)"; return 3; }
+      s.P = 1 << (2 * s.n);
+    } else if (tok == "channel") {
+      if (!s.P) { err = "channel before n"; return 2; }
+      in >> s.e;
+      s.x.resize(s.P * s.P);
+      s.y.resize(s.P * s.P);
+      for (int i = 0; i < s.P * s.P; i++) in >> s.x[i] >> s.y[i];
+      if (!in) { err = "malformed channel"; return 2; }
+      have_ch = true;
+    } else if (tok == "rotations") {
+      int m = 0;
+      in >> m;
+      if (!s.P || m != s.P - 1) { err = "malformed rotations"; return 2; }
+      s.tr.resize(m * s.P);
+      s.ts.resize(m * s.P);
+      for (int i = 0; i < m * s.P; i++) in >> s.tr[i] >> s.ts[i];
+      if (!in) { err = "malformed rotations"; return 2; }
+      have_rot = true;
+    } else if (tok == "end") {
+      break;
+    } else {
+      err = "unknown field " + tok;
+      return 2;
+    }
+  }
+  if (s.n < 1 || s.n > 3) { err = "n = " + std::to_string(s.n) + " not supported (1 <= n <= 3)"; return 3; }
+  if (!have_ch || !have_rot) { err = "spec lacks channel or rotations"; return 2; }
+  return 0;
+}
+
+// the peel pairs rows: every anticommuting row's partner must point back and carry a sign
+static bool tables_ok(const Spec &s) {
+  const int P = s.P;
+  for (int p = 0; p < P - 1; p++)
+    for (int a = 0; a < P; a++) {
+      const int r = s.tr[p * P + a];
+      if (r < -1 || r >= P || r == a) return false;
+      if (r >= 0 && (s.tr[p * P + r] != a || (s.ts[p * P + a] != 1 && s.ts[p * P + a] != -1))) return false;
+    }
+  return true;
+}
+
+enum { F_KEYS = 0, F_SORT, F_DEDUPE, F_PROBE, F_CONFIRM, F_COMPACT, F_INSERT, F_N };
+static const char *f_names[F_N] = {"keys", "sort", "dedupe", "probe", "confirm", "compact", "insert"};
+
+// per-level device arrays of one forest
+struct Levels {
+  std::vector<uint32_t *> par, inst;
+  std::vector<uint8_t *> rot;
+  std::vector<uint64_t *> keys;
+  std::vector<long long> count;
+  void release() {
+    for (auto p : par) if (p) (void)hipFree(p);
+    for (auto p : inst) if (p) (void)hipFree(p);
+    for (auto p : rot) if (p) (void)hipFree(p);
+    for (auto p : keys) if (p) (void)hipFree(p);
+    par.clear(); inst.clear(); rot.clear(); keys.clear(); count.clear();
+  }
+};
+
+template <typename T> static T *dalloc(size_t n) {
+  T *p = nullptr;
+  HIPCHK(hipMalloc((void **)&p, std::max<size_t>(n, 1) * sizeof(T)));
+  return p;
+}
+
+static unsigned grid1(long long n, long long cap = 65536) { return (unsigned)std::max<long long>(1, std::min(n, cap)); }
+static unsigned gridt(long long n, int threads = 256, long long cap = 16384) {
+  return (unsigned)std::max<long long>(1, std::min((n + threads - 1) / threads, cap));
+}
+
+}  // namespace tcn
+
+struct chz_tc {
+  int n = 0, P = 0, G = 0, K = 0, Kt = 0, device = 0;
+  hipStream_t stream = nullptr;
+  tcn::Spec spec;                            // the spec the table was built with (its rotation tables)
+  int8_t *d_tr = nullptr, *d_ts = nullptr;
+  uint32_t *d_coef = nullptr, *d_rv = nullptr;
+  // table forest: one root, the identity
+  tcn::Levels tab;
+  uint32_t *d_troot_x = nullptr, *d_troot_fp = nullptr;
+  int *d_troot_e = nullptr;
+  std::vector<chz_tc_level> tlog;
+  // HBM table
+  unsigned long long *d_hkey = nullptr, *d_hval = nullptr;
+  unsigned long long nslots = 0;
+  long long nentries = 0;
+  // work arrays
+  chz::DBuf<uint64_t> ckeys;
+  chz::DBuf<uint32_t> sk[2], sv[2], runs, list, list2;
+  chz::DBuf<uint8_t> keep, hit, fresh;
+  chz::DBuf<unsigned long long> cnt, bcount, hist, tmp;
+  chz::DBuf<int> flags;
+  // measurement
+  chz::EventLog ev;
+  long k_launch[tcn::F_N] = {};
+  double k_ms[tcn::F_N] = {}, k_work[tcn::F_N] = {};
+};
+
+namespace tcn {
+
+static Rot rot_of(const chz_tc *c) { return Rot{c->d_tr, c->d_ts, c->d_coef, c->d_rv}; }
+
+static Forest forest_of(const Levels &L, const uint32_t *root_x, const uint32_t *root_fp, const int *root_e) {
+  Forest F{};
+  for (size_t l = 0; l < L.par.size() && l <= (size_t)MAXT; l++) {
+    F.par[l] = L.par[l];
+    F.rot[l] = L.rot[l];
+    F.inst[l] = L.inst.size() > l ? L.inst[l] : nullptr;
+  }
+  F.root_x = root_x;
+  F.root_fp = root_fp;
+  F.root_e = root_e;
+  return F;
+}
+
+// ordered compaction of flagged indices 0..n-1 into out; returns the count
+template <typename F> static long long compact(chz_tc *c, F f, long long n, uint32_t *out) {
+  if (n == 0) return 0;
+  const long long nb = (n + CT_TILE - 1) / CT_TILE;
+  c->bcount.need(nb + 1);
+  c->tmp.need(nb / SC_TILE + 4);
+  hipLaunchKernelGGL(k_flag_count<F>, dim3((unsigned)nb), dim3(CT_THREADS), 0, c->stream, f, n, c->bcount.p);
+  HIPCHK(scan_excl(c->bcount.p, nb, c->tmp.p, c->stream));
+  unsigned long long tot = 0;
+  HIPCHK(hipMemcpyAsync(&tot, c->bcount.p + nb, 8, hipMemcpyDeviceToHost, c->stream));
+  HIPCHK(hipStreamSynchronize(c->stream));
+  if (tot && out)
+    hipLaunchKernelGGL(k_scatter_idx<F>, dim3((unsigned)nb), dim3(CT_THREADS), 0, c->stream, f, n, c->bcount.p, out);
+  return (long long)tot;
+}
+
+// sorted (prefix, child) pairs of nc children in sk[0]/sv[0]
+static void sort_children(chz_tc *c, long long nc) {
+  c->sk[1].need(nc);
+  c->sv[1].need(nc);
+  const long long nb = (nc + RS_TILE - 1) / RS_TILE;
+  c->hist.need(nb * 256 + 1);
+  c->tmp.need(nb * 256 / SC_TILE + 4);
+  HIPCHK(radix_sort_pairs(c->sk[0].p, c->sv[0].p, c->sk[1].p, c->sv[1].p, nc, c->hist.p, c->tmp.p, c->stream));
+}
+
+struct GrowOut {
+  long long kept = 0, fresh = 0;
+  int flags = 0;
+};
+
+// one level of growth of forest L (the table when table = true) from the nodes plist of level t (null: all np).
+// Children are keyed, sorted, deduplicated exactly, (table) filtered against the HBM table, and the survivors
+// become level t+1 (src/tcount.cu:327-395).
+template <int NQ>
+static GrowOut grow(chz_tc *c, Levels &L, const uint32_t *root_x, const uint32_t *root_fp, const int *root_e, int t,
+                    const uint32_t *plist, long long np, bool table) {
+  constexpr int P = 1 << (2 * NQ), G = P - 1;
+  GrowOut out;
+  const long long nc = np * G;
+  if (nc >= (1ll << 32) - 1) { set_err("a level holds more than 2^32 children"); throw chz::DeviceError(); }
+  const Rot R = rot_of(c);
+  const Forest F = forest_of(L, root_x, root_fp, root_e);
+  c->ckeys.need(nc);
+  c->sk[0].need(nc);
+  c->sv[0].need(nc);
+  c->keep.need(nc);
+  c->flags.need(1);
+  HIPCHK(hipMemsetAsync(c->flags.p, 0, sizeof(int), c->stream));
+
+  c->ev.begin(F_KEYS, c->stream);
+  KeysArgs ka{R, F, t, plist, np, c->ckeys.p, c->sk[0].p, c->sv[0].p};
+  hipLaunchKernelGGL(k_keys<NQ>, dim3(grid1(np)), dim3(64), 0, c->stream, ka);
+  c->ev.end(c->stream);
+  c->k_work[F_KEYS] += (double)nc * P * 2;
+
+  c->ev.begin(F_SORT, c->stream);
+  sort_children(c, nc);
+  c->ev.end(c->stream);
+  c->k_work[F_SORT] += (double)nc * 8 * 4 * 2;
+
+  c->ev.begin(F_DEDUPE, c->stream);
+  c->runs.need(nc);
+  c->cnt.need(4);
+  HIPCHK(hipMemsetAsync(c->cnt.p, 0, 8, c->stream));
+  hipLaunchKernelGGL(k_fill_u8, dim3(gridt(nc)), dim3(256), 0, c->stream, c->keep.p, nc, (uint8_t)1);
+  hipLaunchKernelGGL(k_multi_runs, dim3(gridt(nc)), dim3(256), 0, c->stream, c->sk[0].p, nc, c->runs.p, c->cnt.p);
+  unsigned long long nruns = 0;
+  HIPCHK(hipMemcpyAsync(&nruns, c->cnt.p, 8, hipMemcpyDeviceToHost, c->stream));
+  HIPCHK(hipStreamSynchronize(c->stream));
+  if (nruns) {
+    DedupeArgs da{R, F, t, plist, c->sk[0].p, c->sv[0].p, nc, c->runs.p, c->cnt.p, c->ckeys.p, c->keep.p, c->flags.p};
+    hipLaunchKernelGGL(k_dedupe<NQ>, dim3(grid1((long long)nruns)), dim3(64), 0, c->stream, da);
+  }
+  c->ev.end(c->stream);
+
+  c->list.need(nc);
+  c->ev.begin(F_COMPACT, c->stream);
+  out.kept = compact(c, FlagU8{c->keep.p}, nc, c->list.p);
+  c->ev.end(c->stream);
+  if (table && out.kept) {
+    c->hit.need(nc);
+    c->fresh.need(nc);
+    c->ev.begin(F_PROBE, c->stream);
+    hipLaunchKernelGGL(k_fill_u8, dim3(gridt(nc)), dim3(256), 0, c->stream, c->hit.p, nc, (uint8_t)0);
+    ProbeArgs pa{HTab{c->d_hkey, c->d_hval, c->nslots / 8 - 1}, c->ckeys.p, c->list.p, out.kept, c->hit.p};
+    hipLaunchKernelGGL(k_probe, dim3(gridt(out.kept, 32)), dim3(256), 0, c->stream, pa);
+    hipLaunchKernelGGL(k_copy_u8, dim3(gridt(nc)), dim3(256), 0, c->stream, c->fresh.p, c->keep.p, nc);
+    c->ev.end(c->stream);
+    c->list2.need(nc);
+    const long long nh = compact(c, FlagU8{c->hit.p}, nc, c->list2.p);
+    if (nh) {
+      c->ev.begin(F_CONFIRM, c->stream);
+      ConfirmArgs ca{R, HTab{c->d_hkey, c->d_hval, c->nslots / 8 - 1}, F, t, plist,
+                     forest_of(c->tab, c->d_troot_x, c->d_troot_fp, c->d_troot_e), c->list2.p, nh, c->ckeys.p, nullptr,
+                     0, c->fresh.p, nullptr, nullptr, c->flags.p};
+      hipLaunchKernelGGL(k_confirm<NQ>, dim3(grid1(nh)), dim3(64), 0, c->stream, ca);
+      c->ev.end(c->stream);
+    }
+    c->ev.begin(F_COMPACT, c->stream);
+    out.fresh = compact(c, FlagU8{c->fresh.p}, nc, c->list.p);
+    c->ev.end(c->stream);
+  } else {
+    out.fresh = out.kept;
+  }
+
+  // level t+1
+  const long long nn = out.fresh;
+  uint32_t *par = dalloc<uint32_t>(nn);
+  uint8_t *rot = dalloc<uint8_t>(nn);
+  uint64_t *keys = dalloc<uint64_t>(nn);
+  uint32_t *inst = table ? nullptr : dalloc<uint32_t>(nn);
+  L.par.push_back(par);
+  L.rot.push_back(rot);
+  L.keys.push_back(keys);
+  L.inst.push_back(inst);
+  L.count.push_back(nn);
+  if (nn) {
+    c->ev.begin(F_COMPACT, c->stream);
+    GatherArgs ga{c->list.p, nn, plist, L.inst[t], c->ckeys.p, G, par, rot, inst, keys};
+    hipLaunchKernelGGL(k_gather, dim3(gridt(nn)), dim3(256), 0, c->stream, ga);
+    c->ev.end(c->stream);
+  }
+  HIPCHK(hipMemcpyAsync(&out.flags, c->flags.p, sizeof(int), hipMemcpyDeviceToHost, c->stream));
+  HIPCHK(hipStreamSynchronize(c->stream));
+  c->ev.flush(c->k_ms, c->k_launch);
+  return out;
+}
+
+// (re)build the HBM table for `need` entries from every level's keys
+static void table_reserve(chz_tc *c, long long need) {
+  if (c->d_hkey && (double)need <= 0.6 * (double)c->nslots) return;
+  unsigned long long slots = 1ull << 12;
+  while ((double)need > 0.45 * (double)slots) slots <<= 1;
+  if (c->d_hkey) { (void)hipFree(c->d_hkey); (void)hipFree(c->d_hval); c->d_hkey = c->d_hval = nullptr; }
+  c->d_hkey = dalloc<unsigned long long>(slots);
+  c->d_hval = dalloc<unsigned long long>(slots);
+  c->nslots = slots;
+  hipLaunchKernelGGL(k_fill_u64, dim3(gridt((long long)slots)), dim3(256), 0, c->stream, c->d_hkey, (long long)slots, 0ull);
+  c->nentries = 0;
+  for (size_t l = 0; l < c->tab.keys.size(); l++) {
+    const long long n = c->tab.count[l];
+    if (!n) continue;
+    InsertArgs ia{HTab{c->d_hkey, c->d_hval, slots / 8 - 1}, c->tab.keys[l], n, (int)l};
+    hipLaunchKernelGGL(k_insert, dim3(gridt(n, 32)), dim3(256), 0, c->stream, ia);
+    c->nentries += n;
+  }
+}
+
+static void table_insert_level(chz_tc *c, int l) {
+  const long long n = c->tab.count[l];
+  if (!c->d_hkey || (double)(c->nentries + n) > 0.6 * (double)c->nslots) {
+    table_reserve(c, c->nentries + n + (c->nentries + n) / 2);
+    return;   // the rebuild inserted every level, this one included
+  }
+  if (!n) return;
+  c->ev.begin(F_INSERT, c->stream);
+  InsertArgs ia{HTab{c->d_hkey, c->d_hval, c->nslots / 8 - 1}, c->tab.keys[l], n, l};
+  hipLaunchKernelGGL(k_insert, dim3(gridt(n, 32)), dim3(256), 0, c->stream, ia);
+  c->ev.end(c->stream);
+  c->nentries += n;
+}
+
+static void upload_rotations(chz_tc *c) {
+  const int P = c->P, G = c->G;
+  std::vector<int8_t> tr(G * P), ts(G * P);
+  for (int i = 0; i < G * P; i++) { tr[i] = (int8_t)c->spec.tr[i]; ts[i] = (int8_t)c->spec.ts[i]; }
+  // column hash weights R[r] (a fixed pseudo-random sequence) and per-rotation child coefficients: the child's
+  // value is v'_a = v_a (commuting row a) or (v_a - s_a v_b) / sqrt2 (b its partner), so its column hash
+  // sum_a R[a] v'_a equals sum_a coef[p][a] v_a with coef = R[a] or (R[a] - s_b R[b]) / sqrt2
+  std::vector<uint32_t> rv(P), coef(G * P);
+  uint64_t s = 0x5eed7c0ull;
+  for (int r = 0; r < P; r++) {
+    s += 0x9e3779b97f4a7c15ull;
+    rv[r] = (uint32_t)(tmix(s) % MP);
+  }
+  for (int p = 0; p < G; p++)
+    for (int a = 0; a < P; a++) {
+      const int b = c->spec.tr[p * P + a];
+      if (b < 0) coef[p * P + a] = rv[a];
+      else {
+        const int sb = c->spec.ts[p * P + b];
+        coef[p * P + a] = m31_rot(sb > 0 ? m31_sub(rv[a], rv[b]) : m31_add(rv[a], rv[b]), 15);
+      }
+    }
+  c->d_tr = dalloc<int8_t>(G * P);
+  c->d_ts = dalloc<int8_t>(G * P);
+  c->d_coef = dalloc<uint32_t>(G * P);
+  c->d_rv = dalloc<uint32_t>(P);
+  HIPCHK(hipMemcpy(c->d_tr, tr.data(), G * P, hipMemcpyHostToDevice));
+  HIPCHK(hipMemcpy(c->d_ts, ts.data(), G * P, hipMemcpyHostToDevice));
+  HIPCHK(hipMemcpy(c->d_coef, coef.data(), G * P * 4, hipMemcpyHostToDevice));
+  HIPCHK(hipMemcpy(c->d_rv, rv.data(), P * 4, hipMemcpyHostToDevice));
+}
+
+// roots: packed exact entries and F_p values
+static void pack_root(const std::vector<int> &x, const std::vector<int> &y, int e, uint32_t *px_, uint32_t *pf) {
+  for (size_t i = 0; i < x.size(); i++) {
+    px_[i] = (uint32_t)(x[i] & 0xffff) | ((uint32_t)y[i] << 16);
+    pf[i] = m31_entry(x[i], y[i], e);
+  }
+}
+static bool fits16(const std::vector<int> &v) {
+  for (int a : v) if (a < -32768 || a > 32767) return false;
+  return true;
+}
+
+template <int NQ> static void root_keys(chz_tc *c, const uint32_t *root_fp, long long n, uint64_t *keys) {
+  RootKeyArgs ra{rot_of(c), root_fp, n, keys};
+  hipLaunchKernelGGL(k_root_keys<NQ>, dim3(grid1(n)), dim3(64), 0, c->stream, ra);
+}
+
+template <int NQ> static void build_table(chz_tc *c) {
+  const int P = c->P;
+  std::vector<int> ix(P * P, 0), iy(P * P, 0);
+  for (int i = 0; i < P; i++) ix[i * P + i] = 1;
+  std::vector<uint32_t> hx(P * P), hf(P * P);
+  pack_root(ix, iy, 0, hx.data(), hf.data());
+  c->d_troot_x = dalloc<uint32_t>(P * P);
+  c->d_troot_fp = dalloc<uint32_t>(P * P);
+  c->d_troot_e = dalloc<int>(1);
+  const int zero = 0;
+  HIPCHK(hipMemcpy(c->d_troot_x, hx.data(), P * P * 4, hipMemcpyHostToDevice));
+  HIPCHK(hipMemcpy(c->d_troot_fp, hf.data(), P * P * 4, hipMemcpyHostToDevice));
+  HIPCHK(hipMemcpy(c->d_troot_e, &zero, 4, hipMemcpyHostToDevice));
+  Levels &T = c->tab;
+  T.par.push_back(nullptr);
+  T.rot.push_back(nullptr);
+  T.inst.push_back(nullptr);
+  T.keys.push_back(dalloc<uint64_t>(1));
+  T.count.push_back(1);
+  root_keys<NQ>(c, c->d_troot_fp, 1, T.keys[0]);
+  table_reserve(c, 1 << 10);
+  c->Kt = 0;
+  for (int l = 1; l <= c->K; l++) {
+    const double t0 = chz::now_s();
+    const long long np = T.count[l - 1];
+    const GrowOut g = grow<NQ>(c, T, c->d_troot_x, c->d_troot_fp, c->d_troot_e, l - 1, nullptr, np, true);
+    if (g.flags & 1) { set_err("table entries exceed 16 bits"); throw chz::DeviceError(); }
+    if (g.flags & 2) { set_err("a dedupe run holds more distinct cosets than its buffer"); throw chz::DeviceError(); }
+    table_insert_level(c, l);
+    HIPCHK(hipStreamSynchronize(c->stream));
+    c->ev.flush(c->k_ms, c->k_launch);
+    chz_tc_level lv{l, (long)np, (long)(np * c->G), (long)g.kept, (long)g.fresh, chz::now_s() - t0};
+    c->tlog.push_back(lv);
+    c->Kt = l;
+    if (!g.fresh) break;
+  }
+}
+
+// one batch of searches against the table (src/tcount.cu:497-531 per instance)
+struct Inst {
+  int slot;                    // index into the caller's arrays
+  Spec s;
+};
+
+template <int NQ>
+static void search_batch(chz_tc *c, std::vector<Inst> &ins, int maxt, chz_tc_result *res, double t0) {
+  constexpr int P = 1 << (2 * NQ), G = P - 1;
+  const int ni = (int)ins.size();
+  if (!ni) return;
+  Levels S;
+  std::vector<void *> extra;
+  struct Guard {
+    Levels &S;
+    std::vector<void *> &extra;
+    ~Guard() { S.release(); for (void *p : extra) if (p) (void)hipFree(p); }
+  } guard{S, extra};
+  auto keep_ptr = [&](auto *p) { extra.push_back((void *)p); return p; };
+  // roots
+  std::vector<uint32_t> hx((size_t)ni * P * P), hf((size_t)ni * P * P), hinst(ni);
+  std::vector<int> he(ni);
+  for (int i = 0; i < ni; i++) {
+    pack_root(ins[i].s.x, ins[i].s.y, ins[i].s.e, hx.data() + (size_t)i * P * P, hf.data() + (size_t)i * P * P);
+    he[i] = ins[i].s.e;
+    hinst[i] = (uint32_t)i;
+  }
+  uint32_t *rx = keep_ptr(dalloc<uint32_t>(hx.size()));
+  uint32_t *rf = keep_ptr(dalloc<uint32_t>(hf.size()));
+  int *re = keep_ptr(dalloc<int>(ni));
+  int *d_maxc = keep_ptr(dalloc<int>(ni));
+  uint8_t *d_active = keep_ptr(dalloc<uint8_t>(ni));
+  unsigned long long *d_best = keep_ptr(dalloc<unsigned long long>(ni));
+  long long *d_lo = keep_ptr(dalloc<long long>(ni));
+  long long *d_hi = keep_ptr(dalloc<long long>(ni));
+  uint32_t *d_hitx = nullptr;
+  HIPCHK(hipMemcpy(rx, hx.data(), hx.size() * 4, hipMemcpyHostToDevice));
+  HIPCHK(hipMemcpy(rf, hf.data(), hf.size() * 4, hipMemcpyHostToDevice));
+  HIPCHK(hipMemcpy(re, he.data(), ni * 4, hipMemcpyHostToDevice));
+  S.par.push_back(nullptr);
+  S.rot.push_back(nullptr);
+  S.inst.push_back(dalloc<uint32_t>(ni));
+  S.keys.push_back(dalloc<uint64_t>(ni));
+  S.count.push_back(ni);
+  HIPCHK(hipMemcpy(S.inst[0], hinst.data(), ni * 4, hipMemcpyHostToDevice));
+  root_keys<NQ>(c, rf, ni, S.keys[0]);
+
+  std::vector<int> state(ni, -1);             // -1 searching, else the result code
+  std::vector<long long> cnt_prev(ni, 1), cnt(ni, 1);
+  std::vector<unsigned long long> best(ni);
+  std::vector<uint8_t> active(ni, 1);
+  const Rot R = rot_of(c);
+  const Forest Ft = forest_of(c->tab, c->d_troot_x, c->d_troot_fp, c->d_troot_e);
+  const HTab HT{c->d_hkey, c->d_hval, c->nslots / 8 - 1};
+  int running = ni;
+  for (int j = 0; j <= maxt && running; j++) {
+    const double tl = chz::now_s();
+    if (j > 0) {
+      // parents: level j-1 nodes of instances still searching
+      const long long np_all = S.count[j - 1];
+      const uint32_t *plist = nullptr;
+      long long np = np_all;
+      bool all = true;
+      for (int i = 0; i < ni; i++) all = all && active[i];
+      if (!all) {
+        HIPCHK(hipMemcpyAsync(d_active, active.data(), ni, hipMemcpyHostToDevice, c->stream));
+        c->list2.need(np_all);
+        np = compact(c, FlagActive{S.inst[j - 1], d_active}, np_all, c->list2.p);
+        uint32_t *pl = keep_ptr(dalloc<uint32_t>(np));
+        if (np) HIPCHK(hipMemcpyAsync(pl, c->list2.p, np * 4, hipMemcpyDeviceToDevice, c->stream));
+        plist = pl;
+      }
+      const GrowOut g = grow<NQ>(c, S, rx, rf, re, j - 1, plist, np, false);
+      if (g.flags & 2) { set_err("a dedupe run holds more distinct cosets than its buffer"); throw chz::DeviceError(); }
+      // per-instance counts of the new level
+      const long long nn = S.count[j];
+      hipLaunchKernelGGL(k_fill_u64, dim3(1), dim3(256), 0, c->stream, (unsigned long long *)d_lo, (long long)ni, 0ull);
+      hipLaunchKernelGGL(k_fill_u64, dim3(1), dim3(256), 0, c->stream, (unsigned long long *)d_hi, (long long)ni, 0ull);
+      if (nn) hipLaunchKernelGGL(k_inst_bounds, dim3(gridt(nn)), dim3(256), 0, c->stream, S.inst[j], nn, d_lo, d_hi);
+      std::vector<long long> lo(ni), hi(ni);
+      HIPCHK(hipMemcpyAsync(lo.data(), d_lo, ni * 8, hipMemcpyDeviceToHost, c->stream));
+      HIPCHK(hipMemcpyAsync(hi.data(), d_hi, ni * 8, hipMemcpyDeviceToHost, c->stream));
+      HIPCHK(hipStreamSynchronize(c->stream));
+      const double secs = chz::now_s() - tl;
+      for (int i = 0; i < ni; i++) {
+        if (state[i] >= 0) continue;
+        cnt[i] = hi[i] - lo[i];
+        chz_tc_result &r = res[ins[i].slot];
+        if (r.nlevels <= MAXT) {
+          r.levels[r.nlevels] = chz_tc_level{j, (long)cnt_prev[i], (long)(cnt_prev[i] * G), (long)cnt[i], (long)cnt[i], secs};
+          r.nlevels++;
+        }
+        if (g.flags & 1) state[i] = 4;
+        else if (!cnt[i]) state[i] = 1;   // nothing left to peel: T-count above maxt
+      }
+    }
+    // lookup of level j
+    const long long nn = S.count[j];
+    std::vector<int> maxc(ni, std::min(c->Kt, maxt - j));
+    HIPCHK(hipMemcpyAsync(d_maxc, maxc.data(), ni * 4, hipMemcpyHostToDevice, c->stream));
+    hipLaunchKernelGGL(k_fill_u64, dim3(1), dim3(256), 0, c->stream, d_best, (long long)ni, ~0ull);
+    if (nn) {
+      c->hit.need(nn);
+      c->ev.begin(F_PROBE, c->stream);
+      ProbeArgs pa{HT, S.keys[j], nullptr, nn, c->hit.p};
+      hipLaunchKernelGGL(k_probe, dim3(gridt(nn, 32)), dim3(256), 0, c->stream, pa);
+      c->ev.end(c->stream);
+      c->list2.need(nn);
+      const long long nh = compact(c, FlagU8{c->hit.p}, nn, c->list2.p);
+      if (nh) {
+        d_hitx = keep_ptr(dalloc<uint32_t>(nn));
+        c->flags.need(1);
+        HIPCHK(hipMemsetAsync(c->flags.p, 0, sizeof(int), c->stream));
+        c->ev.begin(F_CONFIRM, c->stream);
+        ConfirmArgs ca{R, HT, forest_of(S, rx, rf, re), j, nullptr, Ft, c->list2.p, nh, S.keys[j], d_maxc, 1,
+                       nullptr, d_best, d_hitx, c->flags.p};
+        hipLaunchKernelGGL(k_confirm<NQ>, dim3(grid1(nh)), dim3(64), 0, c->stream, ca);
+        c->ev.end(c->stream);
+      }
+    }
+    HIPCHK(hipMemcpyAsync(best.data(), d_best, ni * 8, hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(hipStreamSynchronize(c->stream));
+    c->ev.flush(c->k_ms, c->k_launch);
+    // results of this level
+    std::vector<uint32_t> qn, qt;
+    std::vector<int> ql, qi;
+    for (int i = 0; i < ni; i++)
+      if (state[i] < 0 && best[i] != ~0ull) {
+        qn.push_back((uint32_t)best[i]);
+        ql.push_back((int)(best[i] >> 32));
+        qi.push_back(i);
+      }
+    if (!qn.empty()) {
+      const long long nq = (long long)qn.size();
+      qt.resize(nq);
+      for (long long q = 0; q < nq; q++) HIPCHK(hipMemcpy(&qt[q], d_hitx + qn[q], 4, hipMemcpyDeviceToHost));
+      uint32_t *dn = keep_ptr(dalloc<uint32_t>(nq));
+      uint32_t *dt = keep_ptr(dalloc<uint32_t>(nq));
+      int *dl = keep_ptr(dalloc<int>(nq));
+      int *dout = keep_ptr(dalloc<int>(nq * 2 * MAXT));
+      HIPCHK(hipMemcpy(dn, qn.data(), nq * 4, hipMemcpyHostToDevice));
+      HIPCHK(hipMemcpy(dt, qt.data(), nq * 4, hipMemcpyHostToDevice));
+      HIPCHK(hipMemcpy(dl, ql.data(), nq * 4, hipMemcpyHostToDevice));
+      PathArgs pa{forest_of(S, rx, rf, re), Ft, j, nq, dn, dt, dl, dout};
+      hipLaunchKernelGGL(k_paths, dim3(gridt(nq)), dim3(256), 0, c->stream, pa);
+      std::vector<int> paths(nq * 2 * MAXT);
+      HIPCHK(hipMemcpyAsync(paths.data(), dout, paths.size() * 4, hipMemcpyDeviceToHost, c->stream));
+      HIPCHK(hipStreamSynchronize(c->stream));
+      for (long long q = 0; q < nq; q++) {
+        const int i = qi[q], lv = ql[q];
+        chz_tc_result &r = res[ins[i].slot];
+        r.tcount = j + lv;
+        r.split[0] = j;
+        r.split[1] = lv;
+        int k = 0;
+        for (int l = 0; l < j; l++) { r.rotations[2 * k] = paths[q * 2 * MAXT + l]; r.rotations[2 * k + 1] = 0; k++; }
+        for (int l = lv - 1; l >= 0; l--) { r.rotations[2 * k] = paths[q * 2 * MAXT + MAXT + l]; r.rotations[2 * k + 1] = 1; k++; }
+        state[i] = 0;
+      }
+    }
+    for (int i = 0; i < ni; i++) {
+      if (state[i] < 0 && j + c->Kt >= maxt) state[i] = 1;
+      if (state[i] >= 0 && active[i]) {
+        active[i] = 0;
+        running--;
+        res[ins[i].slot].rc = state[i];
+        res[ins[i].slot].seconds = chz::now_s() - t0;
+      }
+      cnt_prev[i] = cnt[i];
+    }
+  }
+  for (int i = 0; i < ni; i++)
+    if (state[i] < 0) { res[ins[i].slot].rc = 1; res[ins[i].slot].seconds = chz::now_s() - t0; }
+}
+
+}  // namespace tcn
+
+// ------------------------------------------------------------------------------------------------------
+// C ABI
+
+extern "C" {
+
+const char *chz_tc_last_error(void) { return chz::get_err(); }
+
+int chz_tc_table(const char *spec_text, size_t len, int K, int device, chz_tc **out) {
+  using namespace tcn;
+  if (!spec_text || !out) { set_err("null argument"); return 3; }
+  *out = nullptr;
+  std::string err;
+  Spec s;
+  if (const int rc = read_spec(spec_text, len, s, err)) { set_err(err); return rc; }
+  if (!tables_ok(s)) { set_err("rotation tables do not pair anticommuting rows"); return 3; }
+  if (K < 0 || K + 1 > EMAX16 || K > MAXT) { set_err("table depth outside 0.." + std::to_string(EMAX16 - 1)); return 3; }
+  chz_tc *c = new chz_tc();
+  c->n = s.n;
+  c->P = s.P;
+  c->G = s.P - 1;
+  c->K = K;
+  c->device = device;
+  c->spec = s;
+  try {
+    HIPCHK(hipSetDevice(device));
+    HIPCHK(hipStreamCreate(&c->stream));
+    upload_rotations(c);
+    switch (c->n) {
+      case 1: build_table<1>(c); break;
+      case 2: build_table<2>(c); break;
+      default: build_table<3>(c); break;
+    }
+  } catch (const chz::DeviceError &) {
+    const std::string e = chz::get_err();
+    chz_tc_free(c);
+    set_err(e);
+    return e.find("16 bits") != std::string::npos || e.find("buffer") != std::string::npos ? 4 : 5;
+  }
+  *out = c;
+  return 0;
+}
+
+int chz_tc_table_levels(const chz_tc *c, chz_tc_level *lv, int max) {
+  if (!c) return 0;
+  const int n = (int)c->tlog.size();
+  for (int i = 0; i < n && i < max && lv; i++) lv[i] = c->tlog[i];
+  return n;
+}
+
+long long chz_tc_table_size(const chz_tc *c) {
+  if (!c) return 0;
+  long long s = 0;
+  for (long long n : c->tab.count) s += n;
+  return s;
+}
+
+int chz_tc_table_depth(const chz_tc *c) { return c ? c->Kt : 0; }
+int chz_tc_num_qubits(const chz_tc *c) { return c ? c->n : 0; }
+
+int chz_tc_search(chz_tc *c, const char *const *specs, const size_t *lens, int ninst, int maxt, chz_tc_result *res) {
+  using namespace tcn;
+  if (!c || ninst < 0 || (ninst > 0 && (!specs || !lens || !res))) { set_err("null argument"); return 3; }
+  if (maxt < 0 || maxt > MAXT) { set_err("maxt outside 0..32"); return 3; }
+  const double t0 = chz::now_s();
+  std::vector<Inst> ins;
+  for (int i = 0; i < ninst; i++) {
+    std::memset(&res[i], 0, sizeof(chz_tc_result));
+    res[i].rc = -1;
+    Inst in;
+    in.slot = i;
+    std::string err;
+    int rc = read_spec(specs[i], lens[i], in.s, err);
+    if (!rc && (in.s.n != c->n || in.s.tr != c->spec.tr || in.s.ts != c->spec.ts)) {
+      err = "spec's qubit count or rotation tables differ from the table's";
+      rc = 3;
+    }
+    if (!rc && (!fits16(in.s.x) || !fits16(in.s.y) || in.s.e < 0 ||
+                in.s.e + std::max(0, maxt - c->Kt) + 1 > EMAX16)) {
+      err = "channel exponent or entries beyond the 16-bit exact form";
+      rc = 3;
+    }
+    if (rc) { res[i].rc = rc; set_err(err); continue; }
+    ins.push_back(std::move(in));
+  }
+  try {
+    HIPCHK(hipSetDevice(c->device));
+    switch (c->n) {
+      case 1: search_batch<1>(c, ins, maxt, res, t0); break;
+      case 2: search_batch<2>(c, ins, maxt, res, t0); break;
+      default: search_batch<3>(c, ins, maxt, res, t0); break;
+    }
+  } catch (const chz::DeviceError &) {
+    return 5;
+  }
+  return 0;
+}
+
+int chz_tc_kernel_stats(const chz_tc *c, chz_kstat *out, int n) {
+  if (!c) return 0;
+  for (int f = 0; f < tcn::F_N && f < n; f++)
+    if (out) out[f] = chz_kstat{tcn::f_names[f], c->k_launch[f], c->k_ms[f], c->k_work[f]};
+  return tcn::F_N;
+}
+
+void chz_tc_free(chz_tc *c) {
+  if (!c) return;
+  (void)hipSetDevice(c->device);
+  c->tab.release();
+  void *ps[] = {c->d_tr, c->d_ts, c->d_coef, c->d_rv, c->d_troot_x, c->d_troot_fp, c->d_troot_e, c->d_hkey, c->d_hval};
+  for (void *p : ps) if (p) (void)hipFree(p);
+  c->ckeys.release();
+  for (int i = 0; i < 2; i++) { c->sk[i].release(); c->sv[i].release(); }
+  c->runs.release(); c->list.release(); c->list2.release();
+  c->keep.release(); c->hit.release(); c->fresh.release();
+  c->cnt.release(); c->bcount.release(); c->hist.release(); c->tmp.release(); c->flags.release();
+  c->ev.destroy();
+  if (c->stream) (void)hipStreamDestroy(c->stream);
+  delete c;
+}
+
+}  // extern "C"

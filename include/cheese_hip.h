@@ -114,6 +114,49 @@ int chz_cliff_readback(chz_cliff *c, const uint64_t *tab, long long nq, int maxl
 void chz_cliff_free(chz_cliff *c);
 const char *chz_cliff_last_error(void);
 
+/* ---------------- optimal T-count (src/tcount.cu) ----------------
+ * Input is the .tspec text (channel representation of U and the Pauli rotation tables, src/tcount.cu:433-450).
+ * A table holds every right coset reachable from the identity with <= K rotations, each at the level of its
+ * T-count; a search peels rotations off U level by level and looks each residual up (src/tcount.cu:452-532).
+ * One table serves any number of searches for the same qubit count; a search call runs a batch of targets
+ * together, level by level, each with its own result. */
+#define CHZ_TMAX 32 /* deepest rotation sequence, src/tcount.cu:44 */
+
+typedef struct chz_tc chz_tc;
+
+/* one level line: {"dir": ..., "t", "parents", "children", "cosets", "new"} (src/tcount.cu:385-390) */
+typedef struct chz_tc_level {
+  int t;
+  long parents, children, cosets, fresh; /* fresh: the "new" field */
+  double seconds;
+} chz_tc_level;
+
+typedef struct chz_tc_result {
+  int rc;                        /* 0 found, 1 none with T-count <= maxt ("tcount_above"), 2 bad spec,
+                                    3 unsupported, 4 overflow */
+  int tcount;                    /* with rc 0 */
+  int split[2];                  /* rotations peeled off U, table level of the residual */
+  int rotations[2 * CHZ_TMAX];   /* tcount pairs (p, inv) in circuit order (the "rotations" field) */
+  int nlevels;                   /* search level lines before the result line */
+  chz_tc_level levels[CHZ_TMAX + 1];
+  double seconds;
+} chz_tc_result;
+
+/* builds the table for the spec's qubit count with levels <= K from the identity (the table part of main,
+ * src/tcount.cu:473-495): 0, 2 (bad spec), 3 (n outside 1..3, K outside 0..28), 4 (overflow), 5 (device) */
+int chz_tc_table(const char *spec_text, size_t len, int K, int device, chz_tc **out);
+int chz_tc_table_levels(const chz_tc *t, chz_tc_level *levels, int max); /* returns the number of levels */
+long long chz_tc_table_size(const chz_tc *t);                           /* cosets in the table, identity included */
+int chz_tc_table_depth(const chz_tc *t);                                 /* levels built (D.top()) */
+int chz_tc_num_qubits(const chz_tc *t);
+/* searches ninst targets (the search part of main, src/tcount.cu:497-531, `--max maxt`); results per target.
+ * Returns 0 when the batch ran (see res[i].rc), 3 on bad arguments, 5 on a device error. */
+int chz_tc_search(chz_tc *t, const char *const *specs, const size_t *lens, int ninst, int maxt, chz_tc_result *res);
+/* device-side kernel statistics of the table build and every search so far (no reference counterpart) */
+int chz_tc_kernel_stats(const chz_tc *t, chz_kstat *out, int n);
+void chz_tc_free(chz_tc *t);
+const char *chz_tc_last_error(void);
+
 #ifdef __cplusplus
 }
 #endif

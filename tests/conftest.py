@@ -95,3 +95,26 @@ def flags_to_kwargs(flags):
             i += 1
         i += 1
     return kw
+
+
+@pytest.fixture(scope="session")
+def tc_oracle():
+    """the T-count oracle (test infrastructure) built from oracle/ if it is not there yet"""
+    import ctypes
+    path = os.path.join(ROOT, "oracle", "libtcount_oracle.so")
+    if not os.path.exists(path):
+        subprocess.run(["make", "-C", os.path.join(ROOT, "oracle"), "all"], check=True, capture_output=True)
+    L = ctypes.CDLL(path)
+    L.orc_tc_table.restype = ctypes.c_void_p
+    L.orc_tc_table.argtypes = [ctypes.c_char_p, ctypes.c_long, ctypes.c_int, ctypes.c_int, ctypes.c_char_p, ctypes.c_long]
+    L.orc_tc_table_size.restype = ctypes.c_long
+    L.orc_tc_table_size.argtypes = [ctypes.c_void_p]
+    L.orc_tc_search.argtypes = [ctypes.c_void_p, ctypes.c_char_p, ctypes.c_long, ctypes.c_int, ctypes.c_char_p,
+                                ctypes.c_long, ctypes.POINTER(ctypes.c_int)]
+    L.orc_tc_free.argtypes = [ctypes.c_void_p]
+    return L
+
+
+def read_tspec(name):
+    with open(os.path.join(GOLDEN, "tspec", name + ".tspec")) as f:
+        return f.read()

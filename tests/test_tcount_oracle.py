@@ -18,20 +18,8 @@ NAMES3 = ["relative-phase-toffoli", "toffoli", "ccz", "peres", "fredkin"]
 
 
 @pytest.fixture(scope="session")
-def tco():
-    path = os.path.join(ROOT, "oracle", "libtcount_oracle.so")
-    if not os.path.exists(path):
-        import subprocess
-        subprocess.run(["make", "-C", os.path.join(ROOT, "oracle"), "all"], check=True, capture_output=True)
-    L = ctypes.CDLL(path)
-    L.orc_tc_table.restype = ctypes.c_void_p
-    L.orc_tc_table.argtypes = [ctypes.c_char_p, ctypes.c_long, ctypes.c_int, ctypes.c_int, ctypes.c_char_p, ctypes.c_long]
-    L.orc_tc_table_size.restype = ctypes.c_long
-    L.orc_tc_table_size.argtypes = [ctypes.c_void_p]
-    L.orc_tc_search.argtypes = [ctypes.c_void_p, ctypes.c_char_p, ctypes.c_long, ctypes.c_int, ctypes.c_char_p,
-                                ctypes.c_long, ctypes.POINTER(ctypes.c_int)]
-    L.orc_tc_free.argtypes = [ctypes.c_void_p]
-    return L
+def tco(tc_oracle):
+    return tc_oracle
 
 
 def _strip(text):

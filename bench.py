@@ -83,20 +83,51 @@ def random_circuits(n=32):
             "lengths_input_found": lens}
 
 
-def clifford_table(nq=100000):
+def clifford_table(T, wall, nq=100000):
     """C4: the whole 3-qubit Clifford group tabulated on the GPU (BFS into the HBM hash table), then 1e5 seeded
     random words of 5..40 gates answered in one batched launch"""
     from cheese_amd import clifford
-    t = time.perf_counter()
-    T = clifford.CliffordTable(3)
-    wall = time.perf_counter() - t
     entries = T.digest()[1]
     words = clifford.random_words(7, nq, T.ng)
     dist, ms = T.query_words(words)
-    out = {"elements": entries, "levels": T.levels(), "build_ms_device": round(T.build_ms, 2),
-           "build_s_wall": round(wall, 3), "queries": nq, "query_ms_device": round(ms, 3),
-           "queries_per_s": round(nq / (ms / 1e3), 1), "max_distance": max(dist)}
-    T.close()
+    return {"elements": entries, "levels": T.levels(), "build_ms_device": round(T.build_ms, 2),
+            "build_s_wall": round(wall, 3), "queries": nq, "query_ms_device": round(ms, 3),
+            "queries_per_s": round(nq / (ms / 1e3), 1), "max_distance": max(dist)}
+
+
+def tcount_batch(cliff, count=256):
+    """C3: the 3-qubit coset table (K = 4) built once, then `count` seeded targets (k T/Tdg gates, k in 3..7,
+    between uniformly random Cliffords from the Clifford table) searched in one batched call; every
+    decomposition is verified exactly and its T-count is at most k"""
+    from cheese_amd import tcount as T
+    from cheese_amd import workloads as W
+    with open(os.path.join(ROOT, "tests", "golden", "tspec", "toffoli.tspec")) as f:
+        base = f.read()
+    t = time.perf_counter()
+    table = T.CosetTable(base, K=4)
+    build = time.perf_counter() - t
+    cases = T.c3_targets(cliff, count)
+    t = time.perf_counter()
+    specs = [T.tspec_text(3, W.unitary(3, circ)) for _, circ in cases]
+    gen = time.perf_counter() - t
+    ks0 = table.kernel_stats()
+    t = time.perf_counter()
+    res = table.search(specs, maxt=16)
+    search = time.perf_counter() - t
+    ks1 = table.kernel_stats()
+    hist = {}
+    for (k, circ), r in zip(cases, res):
+        if r["rc"] != 0 or r["tcount"] > k or not T.verify(3, W.unitary(3, circ), [list(x) for x in r["rotations"]]):
+            print("bench.py: a T-count target was not decomposed exactly", file=sys.stderr)
+            sys.exit(1)
+        hist[r["tcount"]] = hist.get(r["tcount"], 0) + 1
+    out = {"table_cosets": table.size, "table_build_s": round(build, 3),
+           "table_levels": [[lv["t"], lv["new"], round(lv["seconds"], 3)] for lv in table.levels],
+           "targets": count, "search_s": round(search, 3), "solves_per_s": round(count / search, 1),
+           "tcount_histogram": {str(k): v for k, v in sorted(hist.items())}, "verified": count,
+           "target_generation_s_host": round(gen, 2),
+           "search_kernels_ms": {k: round(ks1[k]["ms"] - ks0[k]["ms"], 3) for k in ks1}}
+    table.close()
     return out
 
 
@@ -112,6 +143,8 @@ def main():
                     help="also time the 32 seeded random 3-qubit circuits (budget = length)")
     ap.add_argument("--clifford", action=argparse.BooleanOptionalAction, default=True,
                     help="also build the 3-qubit Clifford table and answer 1e5 queries")
+    ap.add_argument("--tcount", action=argparse.BooleanOptionalAction, default=True,
+                    help="also build the 3-qubit T-count table and search 256 batched targets")
     a = ap.parse_args()
     if a.gpus != 1:
         print("bench.py: this build runs on one GPU; multi-GPU sharding is a design note only", file=sys.stderr)
@@ -167,8 +200,16 @@ def main():
     }
     if a.random:
         line["random32"] = random_circuits()
-    if a.clifford:
-        line["clifford"] = clifford_table()
+    if a.clifford or a.tcount:
+        from cheese_amd import clifford
+        t = time.perf_counter()
+        cliff = clifford.CliffordTable(3)
+        wall = time.perf_counter() - t
+        if a.clifford:
+            line["clifford"] = clifford_table(cliff, wall)
+        if a.tcount:
+            line["tcount256"] = tcount_batch(cliff)
+        cliff.close()
     if not a.no_cpu:
         line["cpu_baseline"] = cpu_baseline(a.cpu_sample)
     print(json.dumps(line), flush=True)

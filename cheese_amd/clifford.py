@@ -27,6 +27,7 @@ def _lib():
             "chz_cliff_word_tableau": (ctypes.c_uint64, [ctypes.c_void_p, P(ctypes.c_uint8), ctypes.c_int]),
             "chz_cliff_readback": (ctypes.c_int, [ctypes.c_void_p, P(ctypes.c_uint64), ctypes.c_longlong, ctypes.c_int,
                                                   P(ctypes.c_uint8), P(ctypes.c_int)]),
+            "chz_cliff_sample": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_uint64, ctypes.c_longlong, P(ctypes.c_uint64)]),
             "chz_cliff_free": (None, [ctypes.c_void_p]),
             "chz_cliff_last_error": (ctypes.c_char_p, []),
         }
@@ -43,6 +44,16 @@ def gate_names(n):
     for q in range(n):
         out += [f"h {q}", f"s {q}", f"sdg {q}", f"x {q}", f"z {q}"]
     out += [f"cx {a} {b}" for a in range(n) for b in range(n) if a != b]
+    return out
+
+
+def word_gates(n, word):
+    """a gate-set word as a circuit [(gate, qubits)] (letters applied in order)"""
+    names = gate_names(n)
+    out = []
+    for l in word:
+        parts = names[l].split()
+        out.append((parts[0], tuple(int(q) for q in parts[1:])))
     return out
 
 
@@ -105,6 +116,14 @@ class CliffordTable:
         if rc:
             raise RuntimeError(f"readback exit {rc}")
         return [list(words[i * maxlen:i * maxlen + lens[i]]) if lens[i] >= 0 else None for i in range(n)]
+
+    def sample(self, seed, count):
+        """count uniformly random group elements (tableaux) of the table, seeded"""
+        out = (ctypes.c_uint64 * max(count, 1))()
+        rc = _lib().chz_cliff_sample(self.h, seed, count, out)
+        if rc:
+            raise RuntimeError(f"chz_cliff_sample exit {rc}: {_lib().chz_cliff_last_error().decode()}")
+        return list(out[:count])
 
     def close(self):
         if self.h:

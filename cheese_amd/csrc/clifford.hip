@@ -14,6 +14,7 @@
 #include <vector>
 
 #include "../../include/cheese_hip.h"
+#include "primitives.h"
 
 namespace chz_cl {
 
@@ -218,6 +219,43 @@ __global__ __launch_bounds__(64) void k_readback(Gates G, Table T, const uint64_
   }
 }
 
+// elements in a canonical order (sorted by tableau), so that seeded samples do not depend on where concurrent
+// inserts happened to place keys: occupied slots are compacted, then stably sorted by the low and then the high
+// 32 bits of the tableau
+struct FlagSlot {
+  const unsigned long long *s;
+  __device__ bool operator()(long long i) const { return s[i] != 0; }
+};
+__global__ void k_key_lo(const unsigned long long *slot, const uint32_t *idx, long long n, uint32_t *k, uint32_t *v) {
+  for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (long long)gridDim.x * blockDim.x) {
+    k[i] = (uint32_t)(slot[idx[i]] >> 8);
+    v[i] = (uint32_t)i;
+  }
+}
+__global__ void k_key_hi(const unsigned long long *slot, const uint32_t *idx, const uint32_t *perm, long long n,
+                         uint32_t *k, uint32_t *v) {
+  for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (long long)gridDim.x * blockDim.x) {
+    k[i] = (uint32_t)(slot[idx[perm[i]]] >> 40);
+    v[i] = perm[i];
+  }
+}
+__global__ void k_key_final(const unsigned long long *slot, const uint32_t *idx, const uint32_t *perm, long long n,
+                            uint64_t *out) {
+  for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (long long)gridDim.x * blockDim.x)
+    out[i] = slot[idx[perm[i]]] >> 8;
+}
+
+// seeded uniform elements: splitmix64 indices into the sorted element list
+__global__ void k_sample(const uint64_t *sorted, long long order, unsigned long long seed, long long n, uint64_t *out) {
+  for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (long long)gridDim.x * blockDim.x) {
+    uint64_t z = seed + (uint64_t)(i + 1) * 0x9e3779b97f4a7c15ull;
+    z = (z ^ (z >> 30)) * 0xbf58476d1ce4e5b9ull;
+    z = (z ^ (z >> 27)) * 0x94d049bb133111ebull;
+    z ^= z >> 31;
+    out[i] = sorted[z % (uint64_t)order];
+  }
+}
+
 // order-independent digest: sum over entries of hmix-free mix of (tableau * 64 + distance), and the count
 __global__ __launch_bounds__(256) void k_digest(Table T, long long nslots, unsigned long long *out) {
   unsigned long long s = 0, c = 0;
@@ -243,6 +281,8 @@ struct chz_cliff {
   hipStream_t stream = nullptr;
   double build_ms = 0;
   double probes = 0;
+  uint64_t *sorted = nullptr;          // elements sorted by tableau (built on the first sample)
+  long long order = 0;
 };
 
 #define CLCHK(x)                                                                         \
@@ -346,6 +386,58 @@ int chz_cliff_digest(const chz_cliff *c, uint64_t *digest, long long *entries) {
   return 0;
 }
 
+// the sorted element list, built once
+static int cliff_sorted(chz_cliff *c) {
+  if (c->sorted) return 0;
+  const long long ns = c->nslots;
+  const long long nb = (ns + CT_TILE - 1) / CT_TILE;
+  unsigned long long *bc = nullptr, *tmp = nullptr, *hist = nullptr;
+  uint32_t *idx = nullptr, *k0 = nullptr, *v0 = nullptr, *k1 = nullptr, *v1 = nullptr;
+  CLCHK(hipMalloc((void **)&bc, (nb + 1) * 8));
+  CLCHK(hipMalloc((void **)&tmp, (ns / SC_TILE + 8) * 8));
+  chz_cl::FlagSlot f{c->T.slot};
+  hipLaunchKernelGGL(k_flag_count<chz_cl::FlagSlot>, dim3((unsigned)nb), dim3(CT_THREADS), 0, c->stream, f, ns, bc);
+  CLCHK(scan_excl(bc, nb, tmp, c->stream));
+  unsigned long long n = 0;
+  CLCHK(hipMemcpyAsync(&n, bc + nb, 8, hipMemcpyDeviceToHost, c->stream));
+  CLCHK(hipStreamSynchronize(c->stream));
+  const long long rb = ((long long)n + RS_TILE - 1) / RS_TILE;
+  CLCHK(hipMalloc((void **)&idx, std::max<unsigned long long>(n, 1) * 4));
+  CLCHK(hipMalloc((void **)&k0, std::max<unsigned long long>(n, 1) * 4));
+  CLCHK(hipMalloc((void **)&v0, std::max<unsigned long long>(n, 1) * 4));
+  CLCHK(hipMalloc((void **)&k1, std::max<unsigned long long>(n, 1) * 4));
+  CLCHK(hipMalloc((void **)&v1, std::max<unsigned long long>(n, 1) * 4));
+  CLCHK(hipMalloc((void **)&hist, (rb * 256 + 1) * 8));
+  CLCHK(hipMalloc((void **)&c->sorted, std::max<unsigned long long>(n, 1) * 8));
+  hipLaunchKernelGGL(k_scatter_idx<chz_cl::FlagSlot>, dim3((unsigned)nb), dim3(CT_THREADS), 0, c->stream, f, ns, bc, idx);
+  const unsigned g = (unsigned)std::min<long long>(((long long)n + 255) / 256 + 1, 16384);
+  hipLaunchKernelGGL(chz_cl::k_key_lo, dim3(g), dim3(256), 0, c->stream, c->T.slot, idx, (long long)n, k0, v0);
+  CLCHK(radix_sort_pairs(k0, v0, k1, v1, (long long)n, hist, tmp, c->stream));
+  hipLaunchKernelGGL(chz_cl::k_key_hi, dim3(g), dim3(256), 0, c->stream, c->T.slot, idx, v0, (long long)n, k1, v1);
+  CLCHK(radix_sort_pairs(k1, v1, k0, v0, (long long)n, hist, tmp, c->stream));
+  hipLaunchKernelGGL(chz_cl::k_key_final, dim3(g), dim3(256), 0, c->stream, c->T.slot, idx, v1, (long long)n, c->sorted);
+  CLCHK(hipStreamSynchronize(c->stream));
+  c->order = (long long)n;
+  void *fr[] = {bc, tmp, hist, idx, k0, v0, k1, v1};
+  for (void *p : fr) (void)hipFree(p);
+  return 0;
+}
+
+// count uniformly random group elements (tableaux), seeded: uniform indices into the elements sorted by tableau
+int chz_cliff_sample(chz_cliff *c, uint64_t seed, long long count, uint64_t *tab) {
+  CLCHK(hipSetDevice(c->device));
+  if (const int rc = cliff_sorted(c)) return rc;
+  if (c->order <= 0) { chz_cl::g_err = "empty table"; return 4; }
+  uint64_t *d = nullptr;
+  CLCHK(hipMalloc((void **)&d, std::max<long long>(count, 1) * 8));
+  hipLaunchKernelGGL(chz_cl::k_sample, dim3((unsigned)std::min<long long>((count + 255) / 256 + 1, 4096)), dim3(256), 0,
+                     c->stream, c->sorted, c->order, (unsigned long long)seed, count, d);
+  CLCHK(hipMemcpyAsync(tab, d, count * 8, hipMemcpyDeviceToHost, c->stream));
+  CLCHK(hipStreamSynchronize(c->stream));
+  (void)hipFree(d);
+  return 0;
+}
+
 // distances of nq words: letters[offs[i] .. offs[i+1]) are gate-set indices applied in order.
 // ms (may be NULL) receives the device time of the word and lookup kernels.
 int chz_cliff_query_words(chz_cliff *c, const uint8_t *letters, const long long *offs, long long nq, int *dist,
@@ -421,6 +513,7 @@ void chz_cliff_free(chz_cliff *c) {
   (void)hipSetDevice(c->device);
   if (c->stream) (void)hipStreamSynchronize(c->stream);
   if (c->T.slot) (void)hipFree(c->T.slot);
+  if (c->sorted) (void)hipFree(c->sorted);
   if (c->stream) (void)hipStreamDestroy(c->stream);
   delete c;
 }

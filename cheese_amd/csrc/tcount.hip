@@ -4,24 +4,29 @@
 //
 // A node is a right coset S C of a channel matrix S (P x P, P = 4^n, entries (x + y sqrt2) / sqrt2^e); its
 // children peel one pi/8 rotation R(P_p), p = 1 .. P-1, off the left (src/tcount.cu:80-112).  Nodes are kept as
-// (parent, rotation) pairs per level; a wave rebuilds a node by replaying its path from its root, each lane
-// owning one column (peeling acts on rows, so a lane's column never leaves its lane; LDS serves as the
-// dynamically indexed column store).
+// (parent, rotation) pairs per level.  A level that is about to be expanded is materialised in HBM (P*P words per
+// node, row-major, so that lane c reading column c of every row is one coalesced 256-byte access per row);
+// otherwise a wave replays a node's path from the nearest materialised level.  Each lane owns one column:
+// peeling acts on rows, so a lane's column never leaves its lane.
 //
-// Two representations:
-//  * keys: every entry as its value in F_p, p = 2^31 - 1, where sqrt2 = 2^16 (2^32 = 2 mod p) and 1/sqrt2 = 2^15,
-//    so scaling by sqrt2 is a 31-bit rotation.  Values do not depend on the exponent chosen, so no
-//    renormalisation is needed.  A column hashes to (sum_r R[r] v_r)^2 (the square removes the column sign);
-//    the coset key is a mixed sum over columns (removes the column order).  A child's column hash is a fixed
-//    linear form of its parent's column (coefficients per rotation, precomputed), so all P-1 children of a
-//    parent come from one rebuild held in registers.
-//  * exact: packed int16 (x, y) with the reference's renormalisation (the exponent is minimal); equal keys are
-//    always confirmed by comparing the multisets of sign-normalised columns exactly (src/tcount.cu:174-188).
-//    Channel matrices are orthogonal and so are their Galois conjugates, hence |x|, |y sqrt2| <= sqrt2^e: 16 bits
-//    hold every exponent up to 29, which the host checks before a run.
+// Representation: every entry is held as its value in F_p, p = 2^31 - 1, where sqrt2 = 2^16 (2^32 = 2 mod p)
+// and 1/sqrt2 = 2^15, so scaling by sqrt2 is a 31-bit rotation and a residue does not depend on the exponent
+// chosen (no renormalisation).  This is exact, not a hash: a channel matrix is orthogonal and so is its Galois
+// conjugate (sqrt2 -> -sqrt2), so at any common exponent e both x + y sqrt2 and x - y sqrt2 lie in
+// [-sqrt2^e, sqrt2^e]; for e <= 28 that bounds the integer x + 2^16 y strictly inside (-p/2, p/2), where the
+// residue determines it, and |x| < 2^15 then determines (x, y).  Two nodes whose exponents are <= 28 are
+// therefore equal exactly when their residues are.  The host checks the exponent bound before a run.
+//  * keys: a column hashes to (sum_r R[r] v_r)^2 (the square removes the column sign); the coset key is a mixed
+//    sum over columns (removes the column order).  A child's column hash is a fixed linear form of its
+//    parent's column (coefficients per rotation, precomputed), so all P-1 children of a parent come from one
+//    parent column held in registers.
+//  * equality (dedupe and table hits, src/tcount.cu:174-188): columns are sign-normalised (the residue of the
+//    first nonzero entry made to stand for a positive value), sorted by a column hash and compared entry by
+//    entry.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <cmath>
 #include <cstdint>
 #include <cstdio>
 #include <cstring>
@@ -36,8 +41,9 @@
 namespace tcn {
 
 constexpr int MAXT = CHZ_TMAX;
-constexpr int EMAX16 = 29;                  // largest exponent whose channel entries fit int16
+constexpr int EMAX = 28;                    // largest exponent at which residues identify channel entries
 constexpr uint32_t MP = 0x7fffffffu;        // 2^31 - 1
+constexpr uint32_t HALF = MP / 2;           // residues above it stand for negative values
 constexpr int DMAXREP = 32;                 // distinct cosets kept per run of equal key prefixes
 
 // ---- F_p, p = 2^31 - 1
@@ -75,28 +81,18 @@ __host__ __device__ __forceinline__ uint32_t sort_prefix(uint64_t key, uint32_t 
   return (uint32_t)((key ^ ((uint64_t)inst * 0x9e3779b97f4a7c15ull)) >> 32);
 }
 
-// ---- exact packed entries
-__device__ __forceinline__ int px(uint32_t w) { return (int)(int16_t)(w & 0xffffu); }
-__device__ __forceinline__ int py(uint32_t w) { return (int)(int16_t)(w >> 16); }
-__device__ __forceinline__ uint32_t pk(int x, int y) { return (uint32_t)(x & 0xffff) | ((uint32_t)y << 16); }
-__device__ __forceinline__ bool f16(int v) { return v >= -32768 && v <= 32767; }
-
 __device__ __forceinline__ uint64_t wsum64(uint64_t v) {
 #pragma unroll
   for (int m = 1; m < 64; m <<= 1) v += __shfl_xor(v, m, 64);
   return v;
 }
-__device__ __forceinline__ uint32_t wor32(uint32_t v) {
-#pragma unroll
-  for (int m = 1; m < 64; m <<= 1) v |= __shfl_xor(v, m, 64);
-  return v;
-}
-
 // rotation tables and hash coefficients on the device
 struct Rot {
-  const int8_t *tr, *ts;     // (P-1) x P: partner row and sign per rotation and row (tr < 0: row commutes)
+  const uint64_t *anti;      // (P-1): bit a set when row a anticommutes with P_p (its partner is row a ^ p)
+  const uint64_t *plus;      // (P-1): bit a set when the sign s_a of row a is +1
   const uint32_t *coef;      // (P-1) x P: child column hash = sum_r coef[p][r] v_r over the parent's column
   const uint32_t *rv;        // P: column hash = sum_r rv[r] v_r
+  const uint32_t *rw;        // P: second weights of the column hash used to pair columns
 };
 
 // nodes of every level: level 0 node i is root i; a node at level t >= 1 is (par, rot)
@@ -104,134 +100,91 @@ struct Forest {
   const uint32_t *par[MAXT + 1];
   const uint8_t *rot[MAXT + 1];
   const uint32_t *inst[MAXT + 1];   // instance of each node (null: one instance)
-  const uint32_t *root_x;           // roots x P*P packed exact entries
-  const uint32_t *root_fp;          // roots x P*P values in F_p
-  const int *root_e;
+  const uint32_t *mat[MAXT + 1];    // materialised levels (node x P*P residues) or null; mat[0]: the roots
 };
 
 __device__ __forceinline__ uint32_t inst_of(const Forest &F, int t, uint32_t node) {
   return F.inst[t] ? F.inst[t][node] : 0u;
 }
 
-// rotations of node (t, i) into seq[0..t) (LDS); returns its root
-__device__ __forceinline__ uint32_t walk(const Forest &F, int t, uint32_t i, uint8_t *seq, int lane) {
+// one rotation peeled off M in place, lane c rewriting its column c: rows that commute with P_p keep their
+// values, an anticommuting pair (a, a ^ p) becomes ((a - s_a r) / sqrt2, (r - s_r a) / sqrt2)
+// (src/tcount.cu:80-99).  Pairs are enumerated as the rows a with the top bit of p clear.
+template <int P>
+__device__ __forceinline__ void fp_peel(uint32_t *M, const Rot &R, int p, int lane) {
+  const uint64_t anti = R.anti[p - 1], plus = R.plus[p - 1];
+  const int h = 31 - __clz(p), lowm = (1 << h) - 1;
+  if (lane < P) {
+#pragma unroll 4
+    for (int j = 0; j < P / 2; j++) {
+      const int a = ((j >> h) << (h + 1)) | (j & lowm), r = a ^ p;
+      if ((anti >> a) & 1) {
+        uint32_t *ma = M + a * P + lane, *mr = M + r * P + lane;
+        const uint32_t va = *ma, vr = *mr;
+        *ma = m31_rot(((plus >> a) & 1) ? m31_sub(va, vr) : m31_add(va, vr), 15);
+        *mr = m31_rot(((plus >> r) & 1) ? m31_sub(vr, va) : m31_add(vr, va), 15);
+      }
+    }
+  }
+  __builtin_amdgcn_wave_barrier();
+}
+
+// P*P words from src (global) into M (LDS), loads issued in batches ahead of the stores
+template <int P>
+__device__ __forceinline__ void copy_in(const uint32_t *__restrict__ src, uint32_t *__restrict__ M, int lane) {
+  constexpr int N = P * P / 64 > 0 ? P * P / 64 : 1, B = N < 16 ? N : 16;
+  if (P * P < 64) {
+    if (lane < P * P) M[lane] = src[lane];
+  } else {
+#pragma unroll
+    for (int k0 = 0; k0 < N; k0 += B) {
+      uint32_t t[B];
+#pragma unroll
+      for (int j = 0; j < B; j++) t[j] = src[(k0 + j) * 64 + lane];
+#pragma unroll
+      for (int j = 0; j < B; j++) M[(k0 + j) * 64 + lane] = t[j];
+    }
+  }
+  __builtin_amdgcn_wave_barrier();
+}
+
+// node (t, i) into M (LDS): copied from the nearest materialised level at or below t, then the remaining
+// rotations of its path peeled off
+template <int P>
+__device__ void load_node(const Forest &F, const Rot &R, int t, uint32_t node, uint32_t *M, uint8_t *seq, int lane) {
+  int l0 = t;
+  uint32_t i = node;
   if (lane == 0)
-    for (int l = t; l >= 1; l--) { seq[l - 1] = F.rot[l][i]; i = F.par[l][i]; }
+    while (!F.mat[l0]) { seq[l0 - 1] = F.rot[l0][i]; i = F.par[l0][i]; l0--; }
+  l0 = __shfl(l0, 0, 64);
   i = __shfl(i, 0, 64);
   __builtin_amdgcn_wave_barrier();
-  return i;
+  copy_in<P>(F.mat[l0] + (size_t)i * (P * P), M, lane);
+  for (int l = l0; l < t; l++) fp_peel<P>(M, R, __builtin_amdgcn_readfirstlane((int)seq[l]), lane);
 }
 
-// node (t, i) in F_p into M (P x P, row-major, lane c owns column c)
+// lane's column of M sign-normalised in place (the residue of its first nonzero entry made to stand for a positive
+// value) and its sort key: a 58-bit hash of the normalised column above the 6-bit column index
 template <int P>
-__device__ void fp_build(const Forest &F, const Rot &R, int t, uint32_t node, uint32_t *M, uint8_t *seq, int lane) {
-  const uint32_t root = walk(F, t, node, seq, lane);
-  const uint32_t *src = F.root_fp + (size_t)root * (P * P);
-  for (int k = lane; k < P * P; k += 64) M[k] = src[k];
-  __builtin_amdgcn_wave_barrier();
-  for (int l = 0; l < t; l++) {
-    const int p = __builtin_amdgcn_readfirstlane((int)seq[l]);
-    if (lane < P) {
-      const int8_t *tr = R.tr + (p - 1) * P, *ts = R.ts + (p - 1) * P;
-      for (int a = 0; a < P; a++) {
-        const int r = tr[a];
-        if (r > a) {   // anticommuting pair (a, r): a <- (a - s_a r) / sqrt2, r <- (r - s_r a) / sqrt2
-          uint32_t *ma = M + a * P + lane, *mr = M + r * P + lane;
-          const uint32_t va = *ma, vr = *mr;
-          *ma = m31_rot(ts[a] > 0 ? m31_sub(va, vr) : m31_add(va, vr), 15);
-          *mr = m31_rot(ts[r] > 0 ? m31_sub(vr, va) : m31_add(vr, va), 15);
-        }
-      }
-    }
-    __builtin_amdgcn_wave_barrier();
-  }
-}
-
-// number of steps (x + y s) / s = y + (x / 2) s the ORs X, Y of all entries allow below exponent e: step k needs
-// every current x even, i.e. 2^((k+1)/2) | x for odd k and 2^(k/2) | y for even k (src/tcount.cu:100-111)
-__device__ __forceinline__ int halvings(uint32_t X, uint32_t Y, int e) {
-  const int tx = X ? __ffs((int)X) - 1 : 32, ty = Y ? __ffs((int)Y) - 1 : 32;
-  int m = 0;
-  while (m < e) {
-    const int k = m + 1;
-    if ((k & 1) ? tx < (k + 1) / 2 : ty < k / 2) break;
-    m++;
-  }
-  return m;
-}
-__device__ __forceinline__ uint32_t halve(uint32_t w, int m) {   // m steps in closed form
-  int x = px(w), y = py(w);
-  const int t = m >> 1;
-  if (m & 1) { const int nx = y >> t, ny = x >> (t + 1); x = nx; y = ny; }
-  else { x >>= t; y >>= t; }
-  return pk(x, y);
-}
-
-// M <- R(P_p)^-1 M exactly, renormalised (src/tcount.cu:80-112); returns the new exponent
-template <int P>
-__device__ int ex_peel(uint32_t *M, const Rot &R, int p, int e, int lane, bool &ovf) {
-  uint32_t X = 0, Y = 0;
-  if (lane < P) {
-    const int8_t *tr = R.tr + (p - 1) * P, *ts = R.ts + (p - 1) * P;
-    for (int a = 0; a < P; a++) {
-      const int r = tr[a];
-      if (r < 0) {   // commuting row: times sqrt2 in the common exponent
-        uint32_t *m = M + a * P + lane;
-        const uint32_t w = *m;
-        const int x = 2 * py(w), y = px(w);
-        ovf |= !f16(x);
-        *m = pk(x, y);
-        X |= (uint32_t)x;
-        Y |= (uint32_t)y;
-      } else if (r > a) {
-        uint32_t *ma = M + a * P + lane, *mr = M + r * P + lane;
-        const uint32_t wa = *ma, wr = *mr;
-        const int sa = ts[a], sr = ts[r];
-        const int xa = px(wa) - sa * px(wr), ya = py(wa) - sa * py(wr);
-        const int xr = px(wr) - sr * px(wa), yr = py(wr) - sr * py(wa);
-        ovf |= !(f16(xa) && f16(ya) && f16(xr) && f16(yr));
-        *ma = pk(xa, ya);
-        *mr = pk(xr, yr);
-        X |= (uint32_t)(xa | xr);
-        Y |= (uint32_t)(ya | yr);
-      }
-    }
-  }
-  X = wor32(X);
-  Y = wor32(Y);
-  const int m = halvings(X, Y, e + 1);
-  if (m && lane < P)
-    for (int a = 0; a < P; a++) M[a * P + lane] = halve(M[a * P + lane], m);
-  __builtin_amdgcn_wave_barrier();
-  return e + 1 - m;
-}
-
-// node (t, i) exactly into M; returns its exponent
-template <int P>
-__device__ int ex_build(const Forest &F, const Rot &R, int t, uint32_t node, uint32_t *M, uint8_t *seq, int lane,
-                        bool &ovf) {
-  const uint32_t root = walk(F, t, node, seq, lane);
-  const uint32_t *src = F.root_x + (size_t)root * (P * P);
-  for (int k = lane; k < P * P; k += 64) M[k] = src[k];
-  __builtin_amdgcn_wave_barrier();
-  int e = F.root_e[root];
-  for (int l = 0; l < t; l++) e = ex_peel<P>(M, R, __builtin_amdgcn_readfirstlane((int)seq[l]), e, lane, ovf);
-  return e;
-}
-
-// sign-invariant hash of exact column c (its F_p column hash; the exponent is common to the compared matrices)
-template <int P>
-__device__ uint32_t ex_colhash(const uint32_t *M, const Rot &R, int c) {
-  uint64_t lo = 0, hi = 0;
+__device__ uint64_t norm_col(uint32_t *M, const Rot &R, int lane) {
+  if (lane >= P) return ~0ull;
+  bool seen = false, neg = false;
   for (int r = 0; r < P; r++) {
-    const uint32_t w = M[r * P + c];
-    const uint32_t v = m31_add(m31_of(px(w)), m31_rot(m31_of(py(w)), 16));
-    const uint32_t k = R.rv[r];
-    lo += (uint64_t)(k & 0xffffu) * v;
-    hi += (uint64_t)(k >> 16) * v;
+    const uint32_t v = M[r * P + lane];
+    if (!seen && v) { seen = true; neg = v > HALF; }
   }
-  const uint32_t h = m31_fold(lo, hi);
-  return m31_red((uint64_t)h * h);
+  uint64_t lo1 = 0, hi1 = 0, lo2 = 0, hi2 = 0;
+  for (int r = 0; r < P; r++) {
+    uint32_t v = M[r * P + lane];
+    if (neg && v) { v = MP - v; M[r * P + lane] = v; }
+    const uint32_t k1 = R.rv[r], k2 = R.rw[r];
+    lo1 += (uint64_t)(k1 & 0xffffu) * v;
+    hi1 += (uint64_t)(k1 >> 16) * v;
+    lo2 += (uint64_t)(k2 & 0xffffu) * v;
+    hi2 += (uint64_t)(k2 >> 16) * v;
+  }
+  const uint64_t h = (uint64_t)m31_fold(lo1, hi1) | ((uint64_t)(m31_fold(lo2, hi2) & 0x7ffffffu) << 31);
+  return (h << 6) | (uint32_t)lane;
 }
 
 __device__ __forceinline__ uint64_t bitonic64(uint64_t v, int lane) {   // ascending over the wave
@@ -246,39 +199,40 @@ __device__ __forceinline__ uint64_t bitonic64(uint64_t v, int lane) {   // ascen
   return v;
 }
 
-// column ca of A equals column cb of B after making the first nonzero entry of each positive (its x, else y)
 template <int P>
-__device__ bool same_col(const uint32_t *A, int ca, const uint32_t *B, int cb) {
-  int sa = 0, sb = 0;
-  for (int r = 0; r < P; r++) {
-    const uint32_t wa = A[r * P + ca], wb = B[r * P + cb];
-    const int xa = px(wa), ya = py(wa), xb = px(wb), yb = py(wb);
-    if (!sa && (xa | ya)) sa = (xa > 0 || (xa == 0 && ya > 0)) ? 1 : -1;
-    if (!sb && (xb | yb)) sb = (xb > 0 || (xb == 0 && yb > 0)) ? 1 : -1;
-    if (sa * xa != sb * xb || sa * ya != sb * yb) return false;
-  }
+__device__ __forceinline__ bool col_eq(const uint32_t *A, int ca, const uint32_t *B, int cb) {
+  for (int r = 0; r < P; r++)
+    if (A[r * P + ca] != B[r * P + cb]) return false;
   return true;
 }
 
-// exact right-coset equality: equal exponents and equal multisets of sign-normalised columns.  Columns are
-// paired after sorting both matrices by column hash; if a hash tie pairs different columns, every column's
-// multiplicity is counted in both.
+// right-coset equality of two sign-normalised matrices (keys from norm_col): equal multisets of columns.
+// Columns are paired after sorting by hash and compared entry by entry; if a hash tie paired different
+// columns, every column's multiplicity is counted in both.
 template <int P>
-__device__ bool same_coset(const uint32_t *A, int ea, const uint32_t *B, int eb, const Rot &R, int lane) {
-  if (ea != eb) return false;
-  const uint64_t ka = lane < P ? ((uint64_t)ex_colhash<P>(A, R, lane) << 32 | (uint32_t)lane) : ~0ull;
-  const uint64_t kb = lane < P ? ((uint64_t)ex_colhash<P>(B, R, lane) << 32 | (uint32_t)lane) : ~0ull;
+__device__ bool same_cols(const uint32_t *A, uint64_t ka, const uint32_t *B, uint64_t kb, int lane) {
   const uint64_t sa = bitonic64(ka, lane), sb = bitonic64(kb, lane);
-  if (__ballot((sa >> 32) != (sb >> 32))) return false;
-  const bool eq = lane >= P || same_col<P>(A, (int)(uint32_t)sa, B, (int)(uint32_t)sb);
+  if (__ballot((sa >> 6) != (sb >> 6))) return false;
+  const bool eq = lane >= P || col_eq<P>(A, (int)(sa & 63), B, (int)(sb & 63));
   if (!__ballot(!eq)) return true;
   bool ok = true;
   if (lane < P) {
     int na = 0, nb = 0;
-    for (int c = 0; c < P; c++) { na += same_col<P>(A, lane, A, c); nb += same_col<P>(A, lane, B, c); }
+    for (int c = 0; c < P; c++) { na += col_eq<P>(A, lane, A, c); nb += col_eq<P>(A, lane, B, c); }
     ok = na == nb;
   }
   return !__ballot(!ok);
+}
+
+// child (parent at level t, rotation p) of F, sign-normalised, into M; returns the lane's column key
+template <int P>
+__device__ uint64_t load_child(const Forest &F, const Rot &R, int t, uint32_t parent, int p, uint32_t *M,
+                               uint8_t *seq, int lane) {
+  load_node<P>(F, R, t, parent, M, seq, lane);
+  if (p) fp_peel<P>(M, R, p, lane);
+  const uint64_t k = norm_col<P>(M, R, lane);
+  __builtin_amdgcn_wave_barrier();
+  return k;
 }
 
 // ------------------------------------------------------------------------------------------------------
@@ -328,11 +282,13 @@ template <int NQ> __global__ __launch_bounds__(64) void k_keys(KeysArgs a) {
   const int lane = threadIdx.x;
   for (long long k = blockIdx.x; k < a.np; k += gridDim.x) {
     const uint32_t node = a.plist ? a.plist[k] : (uint32_t)k;
-    fp_build<P>(a.F, a.R, a.t, node, M, seq, lane);
+    const uint32_t *src = M;
+    if (a.F.mat[a.t]) src = a.F.mat[a.t] + (size_t)node * (P * P);
+    else load_node<P>(a.F, a.R, a.t, node, M, seq, lane);
     const uint32_t inst = inst_of(a.F, a.t, node);
     uint32_t v[P];
 #pragma unroll
-    for (int r = 0; r < P; r++) v[r] = lane < P ? M[r * P + lane] : 0u;
+    for (int r = 0; r < P; r++) v[r] = lane < P ? src[r * P + lane] : 0u;
     uint64_t mine = 0;
     for (int p = 1; p <= G; p++) {
       const uint32_t *C = a.R.coef + (p - 1) * P;
@@ -377,7 +333,7 @@ struct DedupeArgs {
   const unsigned long long *nruns;
   const uint64_t *keys;
   uint8_t *keep;               // by child; cleared for every child whose coset an earlier child holds
-  int *flags;                  // bit 0: int16 overflow, bit 1: a run held more than DMAXREP distinct cosets
+  int *flags;                  // bit 1: a run held more than DMAXREP distinct cosets
 };
 
 // exact dedupe: within each run of equal prefixes (children in increasing index), the first child of every
@@ -388,34 +344,33 @@ template <int NQ> __global__ __launch_bounds__(64) void k_dedupe(DedupeArgs a) {
   __shared__ uint8_t seq[MAXT];
   __shared__ uint32_t rep[DMAXREP];
   const int lane = threadIdx.x;
-  bool ovf = false, full = false;
+  bool full = false;
   const long long nr = (long long)*a.nruns;
   auto parent = [&](uint32_t ch) { return a.plist ? a.plist[ch / G] : ch / G; };
   for (long long r = blockIdx.x; r < nr; r += gridDim.x) {
     const long long i0 = a.runs[r];
     const uint32_t kp = a.k32[i0];
-    int nrep = 0, eb = 0;
+    int nrep = 0;
     long long bid = -1;
+    uint64_t kb = 0;
     for (long long j = i0; j < a.n && a.k32[j] == kp; j++) {
       const uint32_t ch = a.v32[j];
       const uint64_t key = a.keys[ch];
       const uint32_t inst = inst_of(a.F, a.t, parent(ch));
       bool dup = false, built = false;
-      int ea = 0;
+      uint64_t ka = 0;
       for (int q = 0; q < nrep && !dup; q++) {
         const uint32_t rc = rep[q];
         if (a.keys[rc] != key || inst_of(a.F, a.t, parent(rc)) != inst) continue;
         if (!built) {
-          ea = ex_build<P>(a.F, a.R, a.t, parent(ch), A, seq, lane, ovf);
-          ea = ex_peel<P>(A, a.R, (int)(ch % G) + 1, ea, lane, ovf);
+          ka = load_child<P>(a.F, a.R, a.t, parent(ch), (int)(ch % G) + 1, A, seq, lane);
           built = true;
         }
         if (bid != (long long)rc) {
-          eb = ex_build<P>(a.F, a.R, a.t, parent(rc), B, seq, lane, ovf);
-          eb = ex_peel<P>(B, a.R, (int)(rc % G) + 1, eb, lane, ovf);
+          kb = load_child<P>(a.F, a.R, a.t, parent(rc), (int)(rc % G) + 1, B, seq, lane);
           bid = rc;
         }
-        dup = same_coset<P>(A, ea, B, eb, a.R, lane);
+        dup = same_cols<P>(A, ka, B, kb, lane);
       }
       if (dup) {
         if (lane == 0) a.keep[ch] = 0;
@@ -428,8 +383,30 @@ template <int NQ> __global__ __launch_bounds__(64) void k_dedupe(DedupeArgs a) {
       }
     }
   }
-  const bool o = __ballot(ovf) != 0;
-  if (lane == 0 && (o || full)) atomicOr(a.flags, (o ? 1 : 0) | (full ? 2 : 0));
+  if (lane == 0 && full) atomicOr(a.flags, 2);
+}
+
+struct MatArgs {
+  Rot R;
+  Forest F;                    // holds level t (par, rot) and a materialised level below it
+  int t;
+  long long n;
+  uint32_t *out;               // n x P*P
+};
+
+// materialise the nodes of level t: each replayed from the nearest materialised level and written row-major
+template <int NQ> __global__ __launch_bounds__(64) void k_mat(MatArgs a) {
+  constexpr int P = 1 << (2 * NQ);
+  __shared__ uint32_t M[P * P];
+  __shared__ uint8_t seq[MAXT];
+  const int lane = threadIdx.x;
+  for (long long i = blockIdx.x; i < a.n; i += gridDim.x) {
+    load_node<P>(a.F, a.R, a.t, (uint32_t)i, M, seq, lane);
+    uint32_t *o = a.out + (size_t)i * (P * P);
+#pragma unroll 16
+    for (int k = lane; k < P * P; k += 64) o[k] = M[k];
+    __builtin_amdgcn_wave_barrier();
+  }
 }
 
 // HBM coset table: buckets of 8 slots (key | 1, level << 32 | node), probed by 8 lanes
@@ -527,7 +504,6 @@ template <int NQ> __global__ __launch_bounds__(64) void k_confirm(ConfirmArgs a)
   __shared__ uint32_t A[P * P], B[P * P];
   __shared__ uint8_t seq[MAXT];
   const int lane = threadIdx.x;
-  bool ovf = false;
   for (long long it = blockIdx.x; it < a.n; it += gridDim.x) {
     const uint32_t c = a.items[it];
     const uint64_t k = a.keys[c] | 1ull;
@@ -537,9 +513,10 @@ template <int NQ> __global__ __launch_bounds__(64) void k_confirm(ConfirmArgs a)
     else { node = c; p = 0; }
     const uint32_t inst = inst_of(a.Fc, a.tc, node);
     const int maxc = a.maxc ? a.maxc[inst] : (1 << 30);
-    int bestl = 1 << 30, ea = 0;
+    int bestl = 1 << 30;
     uint32_t bestx = 0;
     bool built = false;
+    uint64_t ka = 0;
     uint64_t b = tmix(k) & a.T.bmask;
     while (true) {
       const unsigned long long kv = lane < 8 ? a.T.key[b * 8 + lane] : 0ull;
@@ -554,12 +531,11 @@ template <int NQ> __global__ __launch_bounds__(64) void k_confirm(ConfirmArgs a)
         const uint32_t x = (uint32_t)v;
         if (lvl > maxc || lvl >= bestl) continue;
         if (!built) {
-          ea = ex_build<P>(a.Fc, a.R, a.tc, node, A, seq, lane, ovf);
-          if (p) ea = ex_peel<P>(A, a.R, p, ea, lane, ovf);
+          ka = load_child<P>(a.Fc, a.R, a.tc, node, p, A, seq, lane);
           built = true;
         }
-        const int eb = ex_build<P>(a.Ft, a.R, lvl, x, B, seq, lane, ovf);
-        if (same_coset<P>(A, ea, B, eb, a.R, lane)) { bestl = lvl; bestx = x; }
+        const uint64_t kb = load_child<P>(a.Ft, a.R, lvl, x, 0, B, seq, lane);
+        if (same_cols<P>(A, ka, B, kb, lane)) { bestl = lvl; bestx = x; }
       }
       if (empty || (a.mode == 0 && bestl < (1 << 30))) break;
       b = (b + 1) & a.T.bmask;
@@ -572,7 +548,6 @@ template <int NQ> __global__ __launch_bounds__(64) void k_confirm(ConfirmArgs a)
       }
     }
   }
-  if (__ballot(ovf) && lane == 0) atomicOr(a.flags, 1);
 }
 
 // flag functors for the ordered compactions
@@ -702,24 +677,27 @@ static int read_spec(const char *text, size_t len, Spec &s, std::string &err) {
   return 0;
 }
 
-// the peel pairs rows: every anticommuting row's partner must point back and carry a sign
+// the Pauli multiplication tables the peel relies on: with digits 0 I, 1 X, 2 Y, 3 Z per qubit, P_p P_a is a
+// multiple of P_(a ^ p), so an anticommuting row's partner is row a ^ p, which anticommutes too, and its sign is
+// +-1 (tcount.py's writer produces exactly these)
 static bool tables_ok(const Spec &s) {
   const int P = s.P;
-  for (int p = 0; p < P - 1; p++)
+  for (int p = 1; p < P; p++)
     for (int a = 0; a < P; a++) {
-      const int r = s.tr[p * P + a];
-      if (r < -1 || r >= P || r == a) return false;
-      if (r >= 0 && (s.tr[p * P + r] != a || (s.ts[p * P + a] != 1 && s.ts[p * P + a] != -1))) return false;
+      const int r = s.tr[(p - 1) * P + a], sg = s.ts[(p - 1) * P + a];
+      if (r != -1 && r != (a ^ p)) return false;
+      if (r >= 0 && (s.tr[(p - 1) * P + r] != a || (sg != 1 && sg != -1))) return false;
     }
   return true;
 }
 
-enum { F_KEYS = 0, F_SORT, F_DEDUPE, F_PROBE, F_CONFIRM, F_COMPACT, F_INSERT, F_N };
-static const char *f_names[F_N] = {"keys", "sort", "dedupe", "probe", "confirm", "compact", "insert"};
+enum { F_KEYS = 0, F_SORT, F_DEDUPE, F_PROBE, F_CONFIRM, F_COMPACT, F_INSERT, F_MAT, F_N };
+static const char *f_names[F_N] = {"keys", "sort", "dedupe", "probe", "confirm", "compact", "insert", "materialise"};
 
-// per-level device arrays of one forest
+// per-level device arrays of one forest; mat[l]: the level's nodes materialised (mat[0]: the roots, owned
+// by the caller), kept only for the level being expanded and the one below it
 struct Levels {
-  std::vector<uint32_t *> par, inst;
+  std::vector<uint32_t *> par, inst, mat;
   std::vector<uint8_t *> rot;
   std::vector<uint64_t *> keys;
   std::vector<long long> count;
@@ -728,7 +706,8 @@ struct Levels {
     for (auto p : inst) if (p) (void)hipFree(p);
     for (auto p : rot) if (p) (void)hipFree(p);
     for (auto p : keys) if (p) (void)hipFree(p);
-    par.clear(); inst.clear(); rot.clear(); keys.clear(); count.clear();
+    for (size_t l = 1; l < mat.size(); l++) if (mat[l]) (void)hipFree(mat[l]);
+    par.clear(); inst.clear(); rot.clear(); keys.clear(); count.clear(); mat.clear();
   }
 };
 
@@ -749,12 +728,12 @@ struct chz_tc {
   int n = 0, P = 0, G = 0, K = 0, Kt = 0, device = 0;
   hipStream_t stream = nullptr;
   tcn::Spec spec;                            // the spec the table was built with (its rotation tables)
-  int8_t *d_tr = nullptr, *d_ts = nullptr;
-  uint32_t *d_coef = nullptr, *d_rv = nullptr;
+  uint64_t *d_anti = nullptr, *d_plus = nullptr;
+  uint32_t *d_coef = nullptr, *d_rv = nullptr, *d_rw = nullptr;
   // table forest: one root, the identity
   tcn::Levels tab;
-  uint32_t *d_troot_x = nullptr, *d_troot_fp = nullptr;
-  int *d_troot_e = nullptr;
+  uint32_t *d_troot_fp = nullptr;
+  double mat_budget = 32e9;                  // bytes of materialised levels one forest may hold
   std::vector<chz_tc_level> tlog;
   // HBM table
   unsigned long long *d_hkey = nullptr, *d_hval = nullptr;
@@ -774,19 +753,37 @@ struct chz_tc {
 
 namespace tcn {
 
-static Rot rot_of(const chz_tc *c) { return Rot{c->d_tr, c->d_ts, c->d_coef, c->d_rv}; }
+static Rot rot_of(const chz_tc *c) { return Rot{c->d_anti, c->d_plus, c->d_coef, c->d_rv, c->d_rw}; }
 
-static Forest forest_of(const Levels &L, const uint32_t *root_x, const uint32_t *root_fp, const int *root_e) {
+static Forest forest_of(const Levels &L) {
   Forest F{};
   for (size_t l = 0; l < L.par.size() && l <= (size_t)MAXT; l++) {
     F.par[l] = L.par[l];
     F.rot[l] = L.rot[l];
     F.inst[l] = L.inst.size() > l ? L.inst[l] : nullptr;
+    F.mat[l] = L.mat.size() > l ? L.mat[l] : nullptr;
   }
-  F.root_x = root_x;
-  F.root_fp = root_fp;
-  F.root_e = root_e;
   return F;
+}
+
+// materialise level t of L (its parents must be reachable from a materialised level below) when it fits the
+// budget, and drop the materialised level t-1, which nothing reads any more
+template <int NQ> static void materialise(chz_tc *c, Levels &L, int t) {
+  constexpr int P = 1 << (2 * NQ);
+  while (L.mat.size() <= (size_t)t) L.mat.push_back(nullptr);
+  if (t >= 1 && !L.mat[t] && (double)L.count[t] * P * P * 4 <= c->mat_budget && L.count[t] > 0) {
+    uint32_t *m = dalloc<uint32_t>((size_t)L.count[t] * P * P);
+    c->ev.begin(F_MAT, c->stream);
+    MatArgs ma{rot_of(c), forest_of(L), t, L.count[t], m};
+    hipLaunchKernelGGL(k_mat<NQ>, dim3(grid1(L.count[t])), dim3(64), 0, c->stream, ma);
+    c->ev.end(c->stream);
+    L.mat[t] = m;
+  }
+  if (t >= 2 && L.mat[t - 1] && L.mat[t]) {
+    HIPCHK(hipStreamSynchronize(c->stream));
+    (void)hipFree(L.mat[t - 1]);
+    L.mat[t - 1] = nullptr;
+  }
 }
 
 // ordered compaction of flagged indices 0..n-1 into out; returns the count
@@ -824,14 +821,14 @@ struct GrowOut {
 // Children are keyed, sorted, deduplicated exactly, (table) filtered against the HBM table, and the survivors
 // become level t+1 (src/tcount.cu:327-395).
 template <int NQ>
-static GrowOut grow(chz_tc *c, Levels &L, const uint32_t *root_x, const uint32_t *root_fp, const int *root_e, int t,
-                    const uint32_t *plist, long long np, bool table) {
+static GrowOut grow(chz_tc *c, Levels &L, int t, const uint32_t *plist, long long np, bool table) {
   constexpr int P = 1 << (2 * NQ), G = P - 1;
   GrowOut out;
   const long long nc = np * G;
   if (nc >= (1ll << 32) - 1) { set_err("a level holds more than 2^32 children"); throw chz::DeviceError(); }
+  materialise<NQ>(c, L, t);
   const Rot R = rot_of(c);
-  const Forest F = forest_of(L, root_x, root_fp, root_e);
+  const Forest F = forest_of(L);
   c->ckeys.need(nc);
   c->sk[0].need(nc);
   c->sv[0].need(nc);
@@ -882,9 +879,8 @@ static GrowOut grow(chz_tc *c, Levels &L, const uint32_t *root_x, const uint32_t
     const long long nh = compact(c, FlagU8{c->hit.p}, nc, c->list2.p);
     if (nh) {
       c->ev.begin(F_CONFIRM, c->stream);
-      ConfirmArgs ca{R, HTab{c->d_hkey, c->d_hval, c->nslots / 8 - 1}, F, t, plist,
-                     forest_of(c->tab, c->d_troot_x, c->d_troot_fp, c->d_troot_e), c->list2.p, nh, c->ckeys.p, nullptr,
-                     0, c->fresh.p, nullptr, nullptr, c->flags.p};
+      ConfirmArgs ca{R, HTab{c->d_hkey, c->d_hval, c->nslots / 8 - 1}, F, t, plist, forest_of(c->tab), c->list2.p,
+                     nh, c->ckeys.p, nullptr, 0, c->fresh.p, nullptr, nullptr, c->flags.p};
       hipLaunchKernelGGL(k_confirm<NQ>, dim3(grid1(nh)), dim3(64), 0, c->stream, ca);
       c->ev.end(c->stream);
     }
@@ -906,6 +902,7 @@ static GrowOut grow(chz_tc *c, Levels &L, const uint32_t *root_x, const uint32_t
   L.keys.push_back(keys);
   L.inst.push_back(inst);
   L.count.push_back(nn);
+  L.mat.resize(L.par.size(), nullptr);
   if (nn) {
     c->ev.begin(F_COMPACT, c->stream);
     GatherArgs ga{c->list.p, nn, plist, L.inst[t], c->ckeys.p, G, par, rot, inst, keys};
@@ -954,16 +951,22 @@ static void table_insert_level(chz_tc *c, int l) {
 
 static void upload_rotations(chz_tc *c) {
   const int P = c->P, G = c->G;
-  std::vector<int8_t> tr(G * P), ts(G * P);
-  for (int i = 0; i < G * P; i++) { tr[i] = (int8_t)c->spec.tr[i]; ts[i] = (int8_t)c->spec.ts[i]; }
+  std::vector<uint64_t> anti(G, 0), plus(G, 0);
+  for (int p = 0; p < G; p++)
+    for (int a = 0; a < P; a++) {
+      if (c->spec.tr[p * P + a] >= 0) anti[p] |= 1ull << a;
+      if (c->spec.ts[p * P + a] > 0) plus[p] |= 1ull << a;
+    }
   // column hash weights R[r] (a fixed pseudo-random sequence) and per-rotation child coefficients: the child's
   // value is v'_a = v_a (commuting row a) or (v_a - s_a v_b) / sqrt2 (b its partner), so its column hash
   // sum_a R[a] v'_a equals sum_a coef[p][a] v_a with coef = R[a] or (R[a] - s_b R[b]) / sqrt2
-  std::vector<uint32_t> rv(P), coef(G * P);
+  std::vector<uint32_t> rv(P), rw(P), coef(G * P);
   uint64_t s = 0x5eed7c0ull;
   for (int r = 0; r < P; r++) {
     s += 0x9e3779b97f4a7c15ull;
     rv[r] = (uint32_t)(tmix(s) % MP);
+    s += 0x9e3779b97f4a7c15ull;
+    rw[r] = (uint32_t)(tmix(s) % MP);
   }
   for (int p = 0; p < G; p++)
     for (int a = 0; a < P; a++) {
@@ -974,25 +977,41 @@ static void upload_rotations(chz_tc *c) {
         coef[p * P + a] = m31_rot(sb > 0 ? m31_sub(rv[a], rv[b]) : m31_add(rv[a], rv[b]), 15);
       }
     }
-  c->d_tr = dalloc<int8_t>(G * P);
-  c->d_ts = dalloc<int8_t>(G * P);
+  c->d_anti = dalloc<uint64_t>(G);
+  c->d_plus = dalloc<uint64_t>(G);
   c->d_coef = dalloc<uint32_t>(G * P);
   c->d_rv = dalloc<uint32_t>(P);
-  HIPCHK(hipMemcpy(c->d_tr, tr.data(), G * P, hipMemcpyHostToDevice));
-  HIPCHK(hipMemcpy(c->d_ts, ts.data(), G * P, hipMemcpyHostToDevice));
+  c->d_rw = dalloc<uint32_t>(P);
+  HIPCHK(hipMemcpy(c->d_anti, anti.data(), G * 8, hipMemcpyHostToDevice));
+  HIPCHK(hipMemcpy(c->d_plus, plus.data(), G * 8, hipMemcpyHostToDevice));
   HIPCHK(hipMemcpy(c->d_coef, coef.data(), G * P * 4, hipMemcpyHostToDevice));
   HIPCHK(hipMemcpy(c->d_rv, rv.data(), P * 4, hipMemcpyHostToDevice));
+  HIPCHK(hipMemcpy(c->d_rw, rw.data(), P * 4, hipMemcpyHostToDevice));
 }
 
-// roots: packed exact entries and F_p values
-static void pack_root(const std::vector<int> &x, const std::vector<int> &y, int e, uint32_t *px_, uint32_t *pf) {
-  for (size_t i = 0; i < x.size(); i++) {
-    px_[i] = (uint32_t)(x[i] & 0xffff) | ((uint32_t)y[i] << 16);
-    pf[i] = m31_entry(x[i], y[i], e);
-  }
+// roots as residues
+static void root_residues(const Spec &sp, uint32_t *pf) {
+  for (size_t i = 0; i < sp.x.size(); i++) pf[i] = m31_entry(sp.x[i], sp.y[i], sp.e);
 }
-static bool fits16(const std::vector<int> &v) {
-  for (int a : v) if (a < -32768 || a > 32767) return false;
+
+// the channel matrix is orthogonal (sum_a M[a][b] M[a][c] = delta_bc exactly), which the exactness of the
+// residues relies on; entries must lie within the bound sqrt2^e
+static bool orthogonal(const Spec &sp) {
+  const int P = sp.P;
+  if (sp.e < 0 || sp.e > EMAX) return false;
+  const double lim = std::ldexp(1.0, (sp.e + 1) / 2) + 1;
+  for (int i = 0; i < P * P; i++)
+    if (std::abs((double)sp.x[i]) > lim || std::abs((double)sp.y[i]) > lim) return false;
+  for (int b = 0; b < P; b++)
+    for (int c = b; c < P; c++) {
+      long long A = 0, B = 0;
+      for (int a = 0; a < P; a++) {
+        const long long x1 = sp.x[a * P + b], y1 = sp.y[a * P + b], x2 = sp.x[a * P + c], y2 = sp.y[a * P + c];
+        A += x1 * x2 + 2 * y1 * y2;
+        B += x1 * y2 + y1 * x2;
+      }
+      if (B != 0 || A != (b == c ? (1ll << sp.e) : 0)) return false;
+    }
   return true;
 }
 
@@ -1003,21 +1022,21 @@ template <int NQ> static void root_keys(chz_tc *c, const uint32_t *root_fp, long
 
 template <int NQ> static void build_table(chz_tc *c) {
   const int P = c->P;
-  std::vector<int> ix(P * P, 0), iy(P * P, 0);
-  for (int i = 0; i < P; i++) ix[i * P + i] = 1;
-  std::vector<uint32_t> hx(P * P), hf(P * P);
-  pack_root(ix, iy, 0, hx.data(), hf.data());
-  c->d_troot_x = dalloc<uint32_t>(P * P);
+  Spec id;
+  id.n = c->n;
+  id.P = P;
+  id.x.assign(P * P, 0);
+  id.y.assign(P * P, 0);
+  for (int i = 0; i < P; i++) id.x[i * P + i] = 1;
+  std::vector<uint32_t> hf(P * P);
+  root_residues(id, hf.data());
   c->d_troot_fp = dalloc<uint32_t>(P * P);
-  c->d_troot_e = dalloc<int>(1);
-  const int zero = 0;
-  HIPCHK(hipMemcpy(c->d_troot_x, hx.data(), P * P * 4, hipMemcpyHostToDevice));
   HIPCHK(hipMemcpy(c->d_troot_fp, hf.data(), P * P * 4, hipMemcpyHostToDevice));
-  HIPCHK(hipMemcpy(c->d_troot_e, &zero, 4, hipMemcpyHostToDevice));
   Levels &T = c->tab;
   T.par.push_back(nullptr);
   T.rot.push_back(nullptr);
   T.inst.push_back(nullptr);
+  T.mat.push_back(c->d_troot_fp);
   T.keys.push_back(dalloc<uint64_t>(1));
   T.count.push_back(1);
   root_keys<NQ>(c, c->d_troot_fp, 1, T.keys[0]);
@@ -1026,8 +1045,7 @@ template <int NQ> static void build_table(chz_tc *c) {
   for (int l = 1; l <= c->K; l++) {
     const double t0 = chz::now_s();
     const long long np = T.count[l - 1];
-    const GrowOut g = grow<NQ>(c, T, c->d_troot_x, c->d_troot_fp, c->d_troot_e, l - 1, nullptr, np, true);
-    if (g.flags & 1) { set_err("table entries exceed 16 bits"); throw chz::DeviceError(); }
+    const GrowOut g = grow<NQ>(c, T, l - 1, nullptr, np, true);
     if (g.flags & 2) { set_err("a dedupe run holds more distinct cosets than its buffer"); throw chz::DeviceError(); }
     table_insert_level(c, l);
     HIPCHK(hipStreamSynchronize(c->stream));
@@ -1037,6 +1055,10 @@ template <int NQ> static void build_table(chz_tc *c) {
     c->Kt = l;
     if (!g.fresh) break;
   }
+  // the table's nodes are replayed from the identity when a lookup needs them; drop materialised levels
+  HIPCHK(hipStreamSynchronize(c->stream));
+  for (size_t l = 1; l < T.mat.size(); l++)
+    if (T.mat[l]) { (void)hipFree(T.mat[l]); T.mat[l] = nullptr; }
 }
 
 // one batch of searches against the table (src/tcount.cu:497-531 per instance)
@@ -1059,27 +1081,22 @@ static void search_batch(chz_tc *c, std::vector<Inst> &ins, int maxt, chz_tc_res
   } guard{S, extra};
   auto keep_ptr = [&](auto *p) { extra.push_back((void *)p); return p; };
   // roots
-  std::vector<uint32_t> hx((size_t)ni * P * P), hf((size_t)ni * P * P), hinst(ni);
-  std::vector<int> he(ni);
+  std::vector<uint32_t> hf((size_t)ni * P * P), hinst(ni);
   for (int i = 0; i < ni; i++) {
-    pack_root(ins[i].s.x, ins[i].s.y, ins[i].s.e, hx.data() + (size_t)i * P * P, hf.data() + (size_t)i * P * P);
-    he[i] = ins[i].s.e;
+    root_residues(ins[i].s, hf.data() + (size_t)i * P * P);
     hinst[i] = (uint32_t)i;
   }
-  uint32_t *rx = keep_ptr(dalloc<uint32_t>(hx.size()));
   uint32_t *rf = keep_ptr(dalloc<uint32_t>(hf.size()));
-  int *re = keep_ptr(dalloc<int>(ni));
   int *d_maxc = keep_ptr(dalloc<int>(ni));
   uint8_t *d_active = keep_ptr(dalloc<uint8_t>(ni));
   unsigned long long *d_best = keep_ptr(dalloc<unsigned long long>(ni));
   long long *d_lo = keep_ptr(dalloc<long long>(ni));
   long long *d_hi = keep_ptr(dalloc<long long>(ni));
   uint32_t *d_hitx = nullptr;
-  HIPCHK(hipMemcpy(rx, hx.data(), hx.size() * 4, hipMemcpyHostToDevice));
   HIPCHK(hipMemcpy(rf, hf.data(), hf.size() * 4, hipMemcpyHostToDevice));
-  HIPCHK(hipMemcpy(re, he.data(), ni * 4, hipMemcpyHostToDevice));
   S.par.push_back(nullptr);
   S.rot.push_back(nullptr);
+  S.mat.push_back(rf);
   S.inst.push_back(dalloc<uint32_t>(ni));
   S.keys.push_back(dalloc<uint64_t>(ni));
   S.count.push_back(ni);
@@ -1091,7 +1108,7 @@ static void search_batch(chz_tc *c, std::vector<Inst> &ins, int maxt, chz_tc_res
   std::vector<unsigned long long> best(ni);
   std::vector<uint8_t> active(ni, 1);
   const Rot R = rot_of(c);
-  const Forest Ft = forest_of(c->tab, c->d_troot_x, c->d_troot_fp, c->d_troot_e);
+  const Forest Ft = forest_of(c->tab);
   const HTab HT{c->d_hkey, c->d_hval, c->nslots / 8 - 1};
   int running = ni;
   for (int j = 0; j <= maxt && running; j++) {
@@ -1111,7 +1128,7 @@ static void search_batch(chz_tc *c, std::vector<Inst> &ins, int maxt, chz_tc_res
         if (np) HIPCHK(hipMemcpyAsync(pl, c->list2.p, np * 4, hipMemcpyDeviceToDevice, c->stream));
         plist = pl;
       }
-      const GrowOut g = grow<NQ>(c, S, rx, rf, re, j - 1, plist, np, false);
+      const GrowOut g = grow<NQ>(c, S, j - 1, plist, np, false);
       if (g.flags & 2) { set_err("a dedupe run holds more distinct cosets than its buffer"); throw chz::DeviceError(); }
       // per-instance counts of the new level
       const long long nn = S.count[j];
@@ -1131,8 +1148,7 @@ static void search_batch(chz_tc *c, std::vector<Inst> &ins, int maxt, chz_tc_res
           r.levels[r.nlevels] = chz_tc_level{j, (long)cnt_prev[i], (long)(cnt_prev[i] * G), (long)cnt[i], (long)cnt[i], secs};
           r.nlevels++;
         }
-        if (g.flags & 1) state[i] = 4;
-        else if (!cnt[i]) state[i] = 1;   // nothing left to peel: T-count above maxt
+        if (!cnt[i]) state[i] = 1;   // nothing left to peel: T-count above maxt
       }
     }
     // lookup of level j
@@ -1153,8 +1169,8 @@ static void search_batch(chz_tc *c, std::vector<Inst> &ins, int maxt, chz_tc_res
         c->flags.need(1);
         HIPCHK(hipMemsetAsync(c->flags.p, 0, sizeof(int), c->stream));
         c->ev.begin(F_CONFIRM, c->stream);
-        ConfirmArgs ca{R, HT, forest_of(S, rx, rf, re), j, nullptr, Ft, c->list2.p, nh, S.keys[j], d_maxc, 1,
-                       nullptr, d_best, d_hitx, c->flags.p};
+        ConfirmArgs ca{R, HT, forest_of(S), j, nullptr, Ft, c->list2.p, nh, S.keys[j], d_maxc, 1, nullptr, d_best,
+                       d_hitx, c->flags.p};
         hipLaunchKernelGGL(k_confirm<NQ>, dim3(grid1(nh)), dim3(64), 0, c->stream, ca);
         c->ev.end(c->stream);
       }
@@ -1182,7 +1198,7 @@ static void search_batch(chz_tc *c, std::vector<Inst> &ins, int maxt, chz_tc_res
       HIPCHK(hipMemcpy(dn, qn.data(), nq * 4, hipMemcpyHostToDevice));
       HIPCHK(hipMemcpy(dt, qt.data(), nq * 4, hipMemcpyHostToDevice));
       HIPCHK(hipMemcpy(dl, ql.data(), nq * 4, hipMemcpyHostToDevice));
-      PathArgs pa{forest_of(S, rx, rf, re), Ft, j, nq, dn, dt, dl, dout};
+      PathArgs pa{forest_of(S), Ft, j, nq, dn, dt, dl, dout};
       hipLaunchKernelGGL(k_paths, dim3(gridt(nq)), dim3(256), 0, c->stream, pa);
       std::vector<int> paths(nq * 2 * MAXT);
       HIPCHK(hipMemcpyAsync(paths.data(), dout, paths.size() * 4, hipMemcpyDeviceToHost, c->stream));
@@ -1231,7 +1247,7 @@ int chz_tc_table(const char *spec_text, size_t len, int K, int device, chz_tc **
   Spec s;
   if (const int rc = read_spec(spec_text, len, s, err)) { set_err(err); return rc; }
   if (!tables_ok(s)) { set_err("rotation tables do not pair anticommuting rows"); return 3; }
-  if (K < 0 || K + 1 > EMAX16 || K > MAXT) { set_err("table depth outside 0.." + std::to_string(EMAX16 - 1)); return 3; }
+  if (K < 0 || K + 1 > EMAX || K > MAXT) { set_err("table depth outside 0.." + std::to_string(EMAX - 1)); return 3; }
   chz_tc *c = new chz_tc();
   c->n = s.n;
   c->P = s.P;
@@ -1252,7 +1268,7 @@ int chz_tc_table(const char *spec_text, size_t len, int K, int device, chz_tc **
     const std::string e = chz::get_err();
     chz_tc_free(c);
     set_err(e);
-    return e.find("16 bits") != std::string::npos || e.find("buffer") != std::string::npos ? 4 : 5;
+    return e.find("buffer") != std::string::npos ? 4 : 5;
   }
   *out = c;
   return 0;
@@ -1292,9 +1308,12 @@ int chz_tc_search(chz_tc *c, const char *const *specs, const size_t *lens, int n
       err = "spec's qubit count or rotation tables differ from the table's";
       rc = 3;
     }
-    if (!rc && (!fits16(in.s.x) || !fits16(in.s.y) || in.s.e < 0 ||
-                in.s.e + std::max(0, maxt - c->Kt) + 1 > EMAX16)) {
-      err = "channel exponent or entries beyond the 16-bit exact form";
+    if (!rc && !orthogonal(in.s)) {
+      err = "channel matrix is not orthogonal";
+      rc = 2;
+    }
+    if (!rc && in.s.e + std::max(0, maxt - c->Kt) + 1 > EMAX) {
+      err = "channel exponent plus search depth beyond " + std::to_string(EMAX);
       rc = 3;
     }
     if (rc) { res[i].rc = rc; set_err(err); continue; }
@@ -1324,7 +1343,7 @@ void chz_tc_free(chz_tc *c) {
   if (!c) return;
   (void)hipSetDevice(c->device);
   c->tab.release();
-  void *ps[] = {c->d_tr, c->d_ts, c->d_coef, c->d_rv, c->d_troot_x, c->d_troot_fp, c->d_troot_e, c->d_hkey, c->d_hval};
+  void *ps[] = {c->d_anti, c->d_plus, c->d_coef, c->d_rv, c->d_rw, c->d_troot_fp, c->d_hkey, c->d_hval};
   for (void *p : ps) if (p) (void)hipFree(p);
   c->ckeys.release();
   for (int i = 0; i < 2; i++) { c->sk[i].release(); c->sv[i].release(); }

@@ -353,3 +353,24 @@ class CosetTable:
             self.close()
         except Exception:
             pass
+
+
+def c3_targets(cliff, count=256, seed0=0, n=3, kmin=3, kmax=7):
+    """seeded batched T-count workload: target s is U = C_k X_k ... C_1 X_1 C_0 with k uniform in [kmin, kmax],
+    each X_i a T or Tdg (uniform) on a uniform qubit and each C_i a uniformly random n-qubit Clifford drawn from
+    the Clifford table `cliff` and realised by its optimal word; the T-count is at most k.
+    Returns [(k, circuit)]."""
+    from . import clifford as CL
+    out = []
+    for s in range(count):
+        r = W.SplitMix64(seed0 + s)
+        k = kmin + r.below(kmax - kmin + 1)
+        tabs = cliff.sample(((seed0 + s) * 0x9E3779B97F4A7C15 + 1) & ((1 << 64) - 1), k + 1)
+        words = cliff.readback(tabs, maxlen=48)
+        circ = []
+        for i in range(k + 1):
+            circ += CL.word_gates(n, words[i])
+            if i < k:
+                circ.append(("t" if r.below(2) == 0 else "tdg", (r.below(n),)))
+        out.append((k, circ))
+    return out

@@ -111,6 +111,9 @@ uint64_t chz_cliff_word_tableau(const chz_cliff *c, const uint8_t *letters, int 
 /* optimal words of tableaux by walking distances down, first gate in gate order (src/clifford.cu:354-372);
  * words: nq x maxlen letters, lens[i] = length or -1 */
 int chz_cliff_readback(chz_cliff *c, const uint64_t *tab, long long nq, int maxlen, uint8_t *words, int *lens);
+/* count uniformly random group elements (uniform over the table's entries), seeded (no reference counterpart:
+ * generates the Clifford factors of random T-count targets); 0 or 4 */
+int chz_cliff_sample(chz_cliff *c, uint64_t seed, long long count, uint64_t *tab);
 void chz_cliff_free(chz_cliff *c);
 const char *chz_cliff_last_error(void);
 

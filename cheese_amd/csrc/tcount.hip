@@ -334,6 +334,7 @@ struct DedupeArgs {
   const uint64_t *keys;
   uint8_t *keep;               // by child; cleared for every child whose coset an earlier child holds
   int *flags;                  // bit 1: a run held more than DMAXREP distinct cosets
+  unsigned long long *stat;    // [0] children visited in runs, [1] exact comparisons, [2] duplicates
 };
 
 // exact dedupe: within each run of equal prefixes (children in increasing index), the first child of every
@@ -345,6 +346,7 @@ template <int NQ> __global__ __launch_bounds__(64) void k_dedupe(DedupeArgs a) {
   __shared__ uint32_t rep[DMAXREP];
   const int lane = threadIdx.x;
   bool full = false;
+  unsigned long long n_vis = 0, n_cmp = 0, n_dup = 0;
   const long long nr = (long long)*a.nruns;
   auto parent = [&](uint32_t ch) { return a.plist ? a.plist[ch / G] : ch / G; };
   for (long long r = blockIdx.x; r < nr; r += gridDim.x) {
@@ -371,8 +373,11 @@ template <int NQ> __global__ __launch_bounds__(64) void k_dedupe(DedupeArgs a) {
           bid = rc;
         }
         dup = same_cols<P>(A, ka, B, kb, lane);
+        n_cmp++;
       }
+      n_vis++;
       if (dup) {
+        n_dup++;
         if (lane == 0) a.keep[ch] = 0;
       } else if (nrep < DMAXREP) {
         if (lane == 0) rep[nrep] = ch;
@@ -384,6 +389,11 @@ template <int NQ> __global__ __launch_bounds__(64) void k_dedupe(DedupeArgs a) {
     }
   }
   if (lane == 0 && full) atomicOr(a.flags, 2);
+  if (lane == 0 && n_vis) {
+    atomicAdd(&a.stat[0], n_vis);
+    atomicAdd(&a.stat[1], n_cmp);
+    atomicAdd(&a.stat[2], n_dup);
+  }
 }
 
 struct MatArgs {
@@ -748,6 +758,7 @@ struct chz_tc {
   // measurement
   chz::EventLog ev;
   long k_launch[tcn::F_N] = {};
+  double dcount[4] = {};                      // dedupe: runs, children visited, exact comparisons, duplicates
   double k_ms[tcn::F_N] = {}, k_work[tcn::F_N] = {};
 };
 
@@ -850,14 +861,15 @@ static GrowOut grow(chz_tc *c, Levels &L, int t, const uint32_t *plist, long lon
   c->ev.begin(F_DEDUPE, c->stream);
   c->runs.need(nc);
   c->cnt.need(4);
-  HIPCHK(hipMemsetAsync(c->cnt.p, 0, 8, c->stream));
+  HIPCHK(hipMemsetAsync(c->cnt.p, 0, 32, c->stream));
   hipLaunchKernelGGL(k_fill_u8, dim3(gridt(nc)), dim3(256), 0, c->stream, c->keep.p, nc, (uint8_t)1);
   hipLaunchKernelGGL(k_multi_runs, dim3(gridt(nc)), dim3(256), 0, c->stream, c->sk[0].p, nc, c->runs.p, c->cnt.p);
   unsigned long long nruns = 0;
   HIPCHK(hipMemcpyAsync(&nruns, c->cnt.p, 8, hipMemcpyDeviceToHost, c->stream));
   HIPCHK(hipStreamSynchronize(c->stream));
   if (nruns) {
-    DedupeArgs da{R, F, t, plist, c->sk[0].p, c->sv[0].p, nc, c->runs.p, c->cnt.p, c->ckeys.p, c->keep.p, c->flags.p};
+    DedupeArgs da{R, F, t, plist, c->sk[0].p, c->sv[0].p, nc, c->runs.p, c->cnt.p, c->ckeys.p, c->keep.p, c->flags.p,
+                  c->cnt.p + 1};
     hipLaunchKernelGGL(k_dedupe<NQ>, dim3(grid1((long long)nruns)), dim3(64), 0, c->stream, da);
   }
   c->ev.end(c->stream);
@@ -910,8 +922,15 @@ static GrowOut grow(chz_tc *c, Levels &L, int t, const uint32_t *plist, long lon
     c->ev.end(c->stream);
   }
   HIPCHK(hipMemcpyAsync(&out.flags, c->flags.p, sizeof(int), hipMemcpyDeviceToHost, c->stream));
+  unsigned long long st[4] = {0, 0, 0, 0};
+  HIPCHK(hipMemcpyAsync(st, c->cnt.p, 32, hipMemcpyDeviceToHost, c->stream));
   HIPCHK(hipStreamSynchronize(c->stream));
   c->ev.flush(c->k_ms, c->k_launch);
+  c->k_work[F_DEDUPE] += (double)st[2];
+  c->dcount[0] += (double)st[0];
+  c->dcount[1] += (double)st[1];
+  c->dcount[2] += (double)st[2];
+  c->dcount[3] += (double)st[3];
   return out;
 }
 
@@ -1337,6 +1356,12 @@ int chz_tc_kernel_stats(const chz_tc *c, chz_kstat *out, int n) {
   for (int f = 0; f < tcn::F_N && f < n; f++)
     if (out) out[f] = chz_kstat{tcn::f_names[f], c->k_launch[f], c->k_ms[f], c->k_work[f]};
   return tcn::F_N;
+}
+
+int chz_tc_dedupe_counts(const chz_tc *c, double *out) {
+  if (!c || !out) return 0;
+  for (int i = 0; i < 4; i++) out[i] = c->dcount[i];
+  return 4;
 }
 
 void chz_tc_free(chz_tc *c) {

@@ -245,6 +245,7 @@ def _tc_lib():
             "chz_tc_search": (ctypes.c_int, [ctypes.c_void_p, P(ctypes.c_char_p), P(ctypes.c_size_t), ctypes.c_int,
                                              ctypes.c_int, P(TcResult)]),
             "chz_tc_kernel_stats": (ctypes.c_int, [ctypes.c_void_p, P(KStat), ctypes.c_int]),
+            "chz_tc_dedupe_counts": (ctypes.c_int, [ctypes.c_void_p, P(ctypes.c_double)]),
             "chz_tc_free": (None, [ctypes.c_void_p]),
             "chz_tc_last_error": (ctypes.c_char_p, []),
         }
@@ -342,6 +343,12 @@ class CosetTable:
         ks = (KStat * 16)()
         n = _tc_lib().chz_tc_kernel_stats(self._h, ks, 16)
         return {ks[i].name.decode(): {"launches": ks[i].launches, "ms": ks[i].ms, "work": ks[i].work} for i in range(n)}
+
+    def dedupe_counts(self):
+        import ctypes
+        out = (ctypes.c_double * 4)()
+        _tc_lib().chz_tc_dedupe_counts(self._h, out)
+        return dict(zip(("runs", "visited", "compared", "duplicates"), [int(x) for x in out]))
 
     def close(self):
         if getattr(self, "_h", None):

@@ -157,6 +157,9 @@ int chz_tc_num_qubits(const chz_tc *t);
 int chz_tc_search(chz_tc *t, const char *const *specs, const size_t *lens, int ninst, int maxt, chz_tc_result *res);
 /* device-side kernel statistics of the table build and every search so far (no reference counterpart) */
 int chz_tc_kernel_stats(const chz_tc *t, chz_kstat *out, int n);
+/* dedupe work so far (no reference counterpart; for measurement): runs of equal key prefixes, children visited
+ * in them, exact comparisons, duplicates removed; returns 4 */
+int chz_tc_dedupe_counts(const chz_tc *t, double *out);
 void chz_tc_free(chz_tc *t);
 const char *chz_tc_last_error(void);
 

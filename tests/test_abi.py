@@ -48,3 +48,18 @@ def test_cli_unsupported_exit_3(tmp_path):
     assert _cli(str(p)).returncode == 3                           # state targets are not built
     p.write_text(spec.replace("\nn 2\n", "\nn 4\n"))
     assert _cli(str(p)).returncode == 3                           # n outside 1..3
+
+
+TCOUNT_CLI = os.path.join(ROOT, "cheese_amd", "bin", "tcount")
+
+
+def test_tcount_cli_exit_codes(tmp_path):
+    """the T-count command line's usage, bad-spec and unsupported exits (src/tcount.cu:436-457), before any
+    device work"""
+    assert subprocess.run([TCOUNT_CLI], capture_output=True).returncode == 3
+    p = tmp_path / "bad.tspec"
+    p.write_text("not a tspec\n")
+    assert subprocess.run([TCOUNT_CLI, str(p)], capture_output=True).returncode == 2
+    p.write_text("tcount-spec 1\nn 4\nend\n")
+    assert subprocess.run([TCOUNT_CLI, str(p)], capture_output=True).returncode == 3
+    assert subprocess.run([TCOUNT_CLI, str(tmp_path / "missing.tspec")], capture_output=True).returncode == 2

@@ -151,3 +151,16 @@ def test_batched_targets_from_clifford_samples(table3):
     for (k, circ), r in zip(cases, got):
         assert r["rc"] == 0 and r["tcount"] <= k
         assert T.verify(3, W.unitary(3, circ), [list(x) for x in r["rotations"]])
+
+
+def test_tcount_command_line_matches_reference():
+    """`tcount toffoli.tspec --table 4`: the table lines, search lines and result line of the reference"""
+    import subprocess
+    from conftest import ROOT
+    exe = os.path.join(ROOT, "cheese_amd", "bin", "tcount")
+    p = subprocess.run([exe, os.path.join(GOLDEN, "tspec", "toffoli.tspec"), "--table", "4"], capture_output=True,
+                       text=True, timeout=300)
+    assert p.returncode == 0
+    lines = p.stdout.strip().splitlines()
+    assert strip_seconds("\n".join(lines[:4])) == _golden("table-n3-k4.oracle.log")
+    assert strip_seconds("\n".join(lines[4:])) == _golden("toffoli.json")

@@ -40,6 +40,10 @@ class KStat(ctypes.Structure):
                 ("work", ctypes.c_double)]
 
 
+class Event(ctypes.Structure):
+    _fields_ = [("tag", ctypes.c_int), ("kind", ctypes.c_int), ("level", Level), ("result", Result)]
+
+
 LEVEL_CB = ctypes.CFUNCTYPE(None, ctypes.POINTER(Level), ctypes.c_void_p)
 
 # entry points declared in include/cheese_hip.h, with their ctypes signatures
@@ -56,6 +60,12 @@ SIGNATURES = {
     "chz_kernel_stats": (ctypes.c_int, [ctypes.c_void_p, ctypes.POINTER(KStat), ctypes.c_int]),
     "chz_verify": (ctypes.c_int, [ctypes.c_void_p, ctypes.POINTER(ctypes.c_int), ctypes.c_int]),
     "chz_last_error": (ctypes.c_char_p, []),
+    "chz_verify_spec": (ctypes.c_int, [ctypes.c_char_p, ctypes.c_size_t, ctypes.POINTER(ctypes.c_int), ctypes.c_int]),
+    "chz_batch_add": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_char_p, ctypes.c_size_t, ctypes.c_int, ctypes.c_int]),
+    "chz_batch_step": (ctypes.c_int, [ctypes.c_void_p]),
+    "chz_batch_events": (ctypes.c_int, [ctypes.c_void_p, ctypes.POINTER(Event), ctypes.c_int]),
+    "chz_batch_live": (ctypes.c_int, [ctypes.c_void_p]),
+    "chz_batch_nodes": (ctypes.c_longlong, [ctypes.c_void_p]),
 }
 
 
@@ -138,3 +148,98 @@ def result_line(out: dict) -> str:
     names = ", ".join(json.dumps(c) for c in out["circuit"])
     return '{"t": %d, "found": %d, "circuit": [%s], "seconds": %.2f}' % (
         len(out["circuit"]), out["found"] or 1, names, out["seconds"])
+
+
+def verify_spec(spec_text: str, circuit_idx) -> bool:
+    """exact check of a candidate-index list against the spec's target, on the host (no device work)"""
+    raw = spec_text.encode()
+    arr = (ctypes.c_int * max(1, len(circuit_idx)))(*circuit_idx)
+    return lib().chz_verify_spec(raw, len(raw), arr, len(circuit_idx)) == 1
+
+
+class Batch:
+    """searches that share a gate set advancing level by level in the same kernel launches (chz_batch_*).
+    The context is opened on `base_spec` (any spec of that gate set); add() queues searches by tag."""
+
+    def __init__(self, base_spec: str, device: int = 0, chunk: int = 0):
+        L = lib()
+        self._ctx = ctypes.c_void_p()
+        raw = base_spec.encode()
+        o = Opts(-1, 1, 1, chunk, 0, device)
+        rc = L.chz_open(raw, len(raw), ctypes.byref(o), ctypes.byref(self._ctx))
+        if rc:
+            raise SearchError(rc, L.chz_last_error().decode())
+        self.names = [L.chz_gate_name(self._ctx, g).decode() for g in range(L.chz_num_gates(self._ctx))]
+
+    def add(self, spec_text: str, tag: int, depth: int | None = None) -> int:
+        raw = spec_text.encode()
+        return lib().chz_batch_add(self._ctx, raw, len(raw), -1 if depth is None else depth, tag)
+
+    def step(self) -> int:
+        return lib().chz_batch_step(self._ctx)
+
+    def events(self):
+        """queued events: (tag, "level", level dict) or (tag, "result", result dict)"""
+        out = []
+        buf = (Event * 256)()
+        while True:
+            n = lib().chz_batch_events(self._ctx, buf, 256)
+            for i in range(n):
+                e = buf[i]
+                if e.kind == 0:
+                    out.append((e.tag, "level", {f: getattr(e.level, f) for f, _ in Level._fields_}))
+                else:
+                    r = e.result
+                    idx = [r.circuit[j] for j in range(r.len)] if r.rc == 0 else None
+                    out.append((e.tag, "result", {"rc": r.rc, "circuit_idx": idx, "found": r.found,
+                                                  "seconds": r.seconds,
+                                                  "circuit": [self.names[g] for g in idx] if idx is not None else None}))
+            if n < 256:
+                return out
+
+    @property
+    def live(self) -> int:
+        return lib().chz_batch_live(self._ctx)
+
+    @property
+    def nodes(self) -> int:
+        return lib().chz_batch_nodes(self._ctx)
+
+    def close(self):
+        if getattr(self, "_ctx", None):
+            lib().chz_close(self._ctx)
+            self._ctx = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+def search_many(specs, device: int = 0):
+    """all searches in one batch (they must share a gate set); returns per spec the dict search() returns"""
+    b = Batch(specs[0], device)
+    outs = [{"rc": None, "levels": [], "circuit": None} for _ in specs]
+    try:
+        for i, sp in enumerate(specs):
+            rc = b.add(sp, i)
+            if rc:
+                outs[i]["rc"] = rc
+        while True:
+            for tag, kind, d in b.events():
+                o = outs[tag]
+                if kind == "level":
+                    o["levels"].append(d)
+                else:
+                    o.update(d)
+                    if d["rc"] == 0:
+                        o["verified"] = verify_spec(specs[tag], d["circuit_idx"])
+            if not b.live:
+                break
+            rc = b.step()
+            if rc:
+                raise SearchError(rc, lib().chz_last_error().decode())
+        return outs
+    finally:
+        b.close()

@@ -7,6 +7,7 @@
 #include <chrono>
 #include <cstdio>
 #include <cstring>
+#include <deque>
 #include <memory>
 #include <string>
 #include <vector>
@@ -22,6 +23,8 @@ namespace chz {
 // ------------------------------------------------------------------------------------------------------
 // counters of one level (device): indices into a u64 array
 enum { C_W2 = 0, C_W3, C_DEAD, C_FOUND, C_ALIVE, C_OVF, C_NEEDT, C_FIRST, C_RUNS, C_REPOVF, C_KEPT, C_EVAL, C_N };
+// per-instance counters of one level (a batch carries several searches that share the spec's gate set)
+enum { IC_W2 = 0, IC_W3, IC_DEAD, IC_FOUND, IC_ALIVE, IC_OVF, IC_FIRST, IC_N };
 
 struct ExpandArgs {
   const int4 *res;          // parents' residuals, E entries each
@@ -33,7 +36,11 @@ struct ExpandArgs {
   uint8_t *status;          // [ncc]
   uint64_t *keys;           // [ncc] class keys of kept children
   uint32_t *info;           // [ncc] packed cheap bounds of children that need the T bound
-  unsigned long long *cnt;
+  unsigned long long *cnt;  // global counters (C_EVAL, C_NEEDT)
+  const uint32_t *pinst;    // instance slot of every parent of the level (null: one instance, slot 0)
+  const int *rem;           // per slot: gate budget left after this level's gate (depth - t)
+  const uint8_t *keepi;     // per slot: this level's survivors become parents (t < depth)
+  unsigned long long *icnt; // per slot: IC_N counters
 };
 
 // One level's children, fused (src/cheese.cu:662-720, :959-1021): a lane group per parent holds the parent's
@@ -72,13 +79,30 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) void k
   int4 *P = spar + (threadIdx.x & ~(GS - 1));
   const long long wave = ((long long)blockIdx.x * blockDim.x + threadIdx.x) >> 6;
   const long long nwaves = ((long long)gridDim.x * blockDim.x) >> 6;
-  const int rem = sp.depth - a.t;
   const int G = sp.G;
+  // per-instance tallies live in each group's leader lane and are flushed when the group's instance changes
   unsigned long long n_w2 = 0, n_w3 = 0, n_dead = 0, n_found = 0, n_ovf = 0, n_alive = 0, n_eval = 0, n_t = 0;
+  int cur = -1;
+  auto flush = [&]() {
+    if (le == 0 && cur >= 0) {
+      unsigned long long *ic = a.icnt + (size_t)cur * IC_N;
+      if (n_w2) atomicAdd(&ic[IC_W2], n_w2);
+      if (n_w3) atomicAdd(&ic[IC_W3], n_w3);
+      if (n_dead) atomicAdd(&ic[IC_DEAD], n_dead);
+      if (n_found) atomicAdd(&ic[IC_FOUND], n_found);
+      if (n_ovf) atomicAdd(&ic[IC_OVF], n_ovf);
+      if (n_alive) atomicAdd(&ic[IC_ALIVE], n_alive);
+    }
+    n_w2 = n_w3 = n_dead = n_found = n_ovf = n_alive = 0;
+  };
   for (long long base = wave * CPW; base < np; base += nwaves * CPW) {
     const long long pl = base + slot;
     if (pl >= np) continue;
     const long long pi = a.off + pl;
+    const int inst = a.pinst ? (int)a.pinst[pi] : 0;
+    if (inst != cur) { flush(); cur = inst; }
+    const int rem = a.rem[inst];
+    const bool keep = a.keepi[inst] != 0;
     const int4 S = a.res[pi * E + le];
     const int pk = a.pk[pi], g1 = a.g1[pi], g2 = a.g2[pi];
     const bool small = gall<GS>(zmag(S) < (1u << 28));
@@ -169,7 +193,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) void k
                 alive = combine_bound(hh, T, c2, supp) <= rem;
               }
             }
-            if (alive && a.keep) key = class_key<NQ>(s, k, le, sp);
+            if (alive && keep) key = class_key<NQ>(s, k, le, sp) ^ ((uint64_t)inst * 0x9e3779b97f4a7c15ull);
             stt = alive ? ST_ALIVE : ST_DEAD;
           }
         }
@@ -181,38 +205,44 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) void k
           case ST_W3: n_w3++; break;
           case ST_DEAD: n_dead++; break;
           case ST_OVF: n_ovf++; break;
-          case ST_ALIVE: n_alive++; if (a.keep) a.keys[cc] = key; break;
+          case ST_ALIVE: n_alive++; if (keep) a.keys[cc] = key; break;
           case ST_FOUND:
             n_found++;
-            atomicMin(&a.cnt[C_FIRST], (unsigned long long)(a.off * G + cc));
+            atomicMin(&a.icnt[(size_t)inst * IC_N + IC_FIRST], (unsigned long long)(a.off * G + cc));
             break;
         }
       }
     }
     __builtin_amdgcn_wave_barrier();
   }
-  n_w2 = gsum<64>(n_w2); n_w3 = gsum<64>(n_w3); n_dead = gsum<64>(n_dead);
-  n_found = gsum<64>(n_found); n_ovf = gsum<64>(n_ovf); n_alive = gsum<64>(n_alive);
-  n_eval = gsum<64>(n_eval); n_t = gsum<64>(n_t);
+  flush();
+  n_eval = gsum<64>(n_eval);
+  n_t = gsum<64>(n_t);
   if (lane == 0) {
     if (n_eval) atomicAdd(&a.cnt[C_EVAL], n_eval);
     if (n_t) atomicAdd(&a.cnt[C_NEEDT], n_t);
-    if (n_w2) atomicAdd(&a.cnt[C_W2], n_w2);
-    if (n_w3) atomicAdd(&a.cnt[C_W3], n_w3);
-    if (n_dead) atomicAdd(&a.cnt[C_DEAD], n_dead);
-    if (n_found) atomicAdd(&a.cnt[C_FOUND], n_found);
-    if (n_ovf) atomicAdd(&a.cnt[C_OVF], n_ovf);
-    if (n_alive) atomicAdd(&a.cnt[C_ALIVE], n_alive);
   }
 }
 
-struct FlagAlive {      // survivors of a chunk, in child order
+struct FlagAlive {      // survivors of a chunk that become parents, in child order
   const uint8_t *status;
-  __device__ bool operator()(long long i) const { return status[i] == ST_ALIVE; }
+  const uint32_t *pinst;
+  const uint8_t *keepi;
+  long long off;          // first parent of the chunk
+  int G;
+  __device__ bool operator()(long long i) const {
+    return status[i] == ST_ALIVE && keepi[pinst ? pinst[off + i / G] : 0] != 0;
+  }
 };
-struct FlagKept {
+struct FlagKept {       // kept survivors of instances still searching
   const uint8_t *kept;
-  __device__ bool operator()(long long i) const { return kept[i] != 0; }
+  const uint64_t *surv_o;
+  const uint32_t *pinst;
+  const uint8_t *active;
+  int G;
+  __device__ bool operator()(long long i) const {
+    return kept[i] != 0 && active[pinst ? pinst[surv_o[i] / (uint64_t)G] : 0] != 0;
+  }
 };
 struct FlagNeed {       // children that need the T bound, in child order
   const uint8_t *status;
@@ -258,6 +288,7 @@ struct DedupeArgs {
   const int *pk;
   uint8_t *kept;
   unsigned long long *cnt;
+  const uint32_t *pinst;          // parents' instance slots (null: one instance)
 };
 constexpr int MAXREP = 32;
 
@@ -291,6 +322,7 @@ __global__ __launch_bounds__(256) void k_dedupe_runs(KSpec sp, DedupeArgs a) {
         if (a.surv_key[rr] != key64) continue;
         const uint64_t ro = a.surv_o[rr];
         const long long rpi = (long long)(ro / (uint64_t)sp.G);
+        if (a.pinst && a.pinst[rpi] != a.pinst[pi]) continue;
         int4 b;
         int kb;
         make_child<NQ>(a.res + rpi * E, a.pk[rpi], sp.gates[(int)(ro % (uint64_t)sp.G)], le, b, kb);
@@ -318,6 +350,8 @@ struct GatherArgs {
   uint8_t *g1_next, *g2_next;
   uint32_t *tree_par;
   uint8_t *tree_gate;
+  const uint32_t *pinst;          // parents' instance slots (null: one instance)
+  uint32_t *inst_next;
 };
 
 // next frontier: tree node, window state and materialised residual of every kept child
@@ -343,6 +377,7 @@ __global__ __launch_bounds__(256) void k_gather(KSpec sp, GatherArgs a) {
       a.g2_next[p] = a.g1[pi];
       a.tree_par[p] = (uint32_t)pi;
       a.tree_gate[p] = (uint8_t)g;
+      a.inst_next[p] = a.pinst ? a.pinst[pi] : 0u;
     }
   }
 }
@@ -362,6 +397,20 @@ __global__ void k_init_sort(const uint64_t *key, long long n, uint32_t *k32, uin
 enum { K_EXPAND = 0, K_COMPACT, K_SORT, K_DEDUPE, K_GATHER, K_NFAM };
 static const char *k_names[K_NFAM] = {"expand", "compact", "sort", "dedupe", "gather"};
 
+}  // namespace chz
+
+namespace chz {
+// one search of a batch (searches that share a spec's gate set advance level by level together)
+struct BInst {
+  int tag = 0, depth = 0, t = 0, slot = 0, root_step = -1;
+  bool live = false;
+  long long nodes = 0;
+  double t0 = 0;
+};
+struct Pending {               // a search waiting to join the frontier as a root
+  int inst, k;
+  std::vector<int4> U;
+};
 }  // namespace chz
 
 struct chz_ctx {
@@ -390,6 +439,17 @@ struct chz_ctx {
   chz::DBuf<uint32_t> info;
   chz::DBuf<unsigned long long> cnt, bcount, hist;
   chz::DBuf<uint32_t> sk[2], sv[2], runs, kidx;
+  // batch state: searches, frontier instance slots, per-slot device arrays, queued events
+  std::vector<chz::BInst> binst;
+  std::vector<chz::Pending> pending;
+  std::deque<chz_event> events;
+  chz::DBuf<uint32_t> finst[2];
+  long long m = 0;             // nodes in the current frontier (buffer cur)
+  int cur = 0, gstep = -1, nslots = 0;
+  chz::DBuf<int> d_rem;
+  chz::DBuf<uint8_t> d_keepi, d_active;
+  chz::DBuf<unsigned long long> icnt, scan_tmp;
+  chz::DBuf<long long> d_lo, d_hi;
   // measurement
   chz::EventLog ev;
   long k_launch[chz::K_NFAM] = {};
@@ -414,220 +474,330 @@ static double t_ops16(int nq) {
 }
 
 
-template <int NQ> struct Level {
-  static constexpr int N = 1 << NQ, E = N * N, CPW = 64 / E;
+constexpr uint32_t NO_PARENT = 0xffffffffu;
 
-  // stable sort of (key prefix, rank) pairs of n survivors; the result lands in sk[0]/sv[0]
-  static void sort_pairs(chz_ctx *c, long long n) {
-    const long long nb = (n + RS_TILE - 1) / RS_TILE;
-    c->hist.need((size_t)nb * 256 + 1);
-    for (int pass = 0; pass < 4; pass++) {
-      const int src = pass & 1, dst = src ^ 1;
-      hipLaunchKernelGGL(k_radix_hist, dim3((unsigned)nb), dim3(RS_THREADS), 0, c->stream, c->sk[src].p, n, pass * 8,
-                         c->hist.p, nb);
-      hipLaunchKernelGGL(k_scan_one_block, dim3(1), dim3(1024), 0, c->stream, c->hist.p, nb * 256);
-      hipLaunchKernelGGL(k_radix_scatter, dim3((unsigned)nb), dim3(RS_THREADS), 0, c->stream, c->sk[src].p,
-                         c->sv[src].p, n, pass * 8, c->hist.p, nb, c->sk[dst].p, c->sv[dst].p);
-    }
+// stable sort of (key prefix, rank) pairs of n survivors; the result lands in sk[0]/sv[0]
+static void sort_pairs(chz_ctx *c, long long n) {
+  const long long nb = (n + RS_TILE - 1) / RS_TILE;
+  c->hist.need((size_t)nb * 256 + 1);
+  c->scan_tmp.need((size_t)nb * 256 / SC_TILE + 4);
+  HIPCHK(radix_sort_pairs(c->sk[0].p, c->sv[0].p, c->sk[1].p, c->sv[1].p, n, c->hist.p, c->scan_tmp.p, c->stream));
+}
+
+// per-slot device arrays sized for `need` instance slots
+static void reserve_slots(chz_ctx *c, int need) {
+  if (need <= c->nslots) return;
+  int ns = std::max(16, c->nslots);
+  while (ns < need) ns *= 2;
+  c->d_rem.need(ns);
+  c->d_keepi.need(ns);
+  c->d_active.need(ns);
+  c->icnt.need((size_t)ns * IC_N);
+  c->d_lo.need(ns);
+  c->d_hi.need(ns);
+  c->nslots = ns;
+}
+
+// nodes of every slot in frontier buffer `b` of m nodes (nodes are contiguous per instance)
+__global__ void k_slot_bounds(const uint32_t *inst, long long n, long long *lo, long long *hi) {
+  for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (long long)gridDim.x * blockDim.x) {
+    const uint32_t s = inst[i];
+    if (i == 0 || inst[i - 1] != s) lo[s] = i;
+    if (i == n - 1 || inst[i + 1] != s) hi[s] = i + 1;
   }
+}
+__global__ void k_fill_ll(long long *p, long long n, long long v) {
+  for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (long long)gridDim.x * blockDim.x) p[i] = v;
+}
 
-  static void sync_counts(chz_ctx *c, unsigned long long *hc) {
+static std::vector<long long> slot_counts(chz_ctx *c, const uint32_t *inst, long long m) {
+  std::vector<long long> lo(c->nslots), hi(c->nslots), out(c->nslots);
+  hipLaunchKernelGGL(k_fill_ll, dim3(16), dim3(256), 0, c->stream, c->d_lo.p, (long long)c->nslots, 0ll);
+  hipLaunchKernelGGL(k_fill_ll, dim3(16), dim3(256), 0, c->stream, c->d_hi.p, (long long)c->nslots, 0ll);
+  if (m) hipLaunchKernelGGL(k_slot_bounds, dim3(grid_for(m / 64 + 1)), dim3(256), 0, c->stream, inst, m, c->d_lo.p, c->d_hi.p);
+  HIPCHK(hipMemcpyAsync(lo.data(), c->d_lo.p, c->nslots * 8, hipMemcpyDeviceToHost, c->stream));
+  HIPCHK(hipMemcpyAsync(hi.data(), c->d_hi.p, c->nslots * 8, hipMemcpyDeviceToHost, c->stream));
+  HIPCHK(hipStreamSynchronize(c->stream));
+  for (int s = 0; s < c->nslots; s++) out[s] = hi[s] - lo[s];
+  return out;
+}
+
+// the next frontier: `distinct` gathered children (already in buffer nxt) followed by the admitted roots
+template <int NQ> static void append_roots(chz_ctx *c, int nxt, long long distinct, uint32_t *tp, uint8_t *tg) {
+  constexpr int E = 1 << (2 * NQ);
+  const long long nr = (long long)c->pending.size();
+  if (!nr) return;
+  std::vector<int4> res(nr * E);
+  std::vector<int> pk(nr);
+  std::vector<uint8_t> none(nr, 255);
+  std::vector<uint32_t> slot(nr), np(nr, NO_PARENT);
+  for (long long i = 0; i < nr; i++) {
+    const Pending &q = c->pending[i];
+    for (int e = 0; e < E; e++) res[i * E + e] = q.U[e];
+    pk[i] = q.k;
+    slot[i] = (uint32_t)c->binst[q.inst].slot;
+  }
+  HIPCHK(hipMemcpyAsync(c->res[nxt].p + distinct * E, res.data(), nr * E * sizeof(int4), hipMemcpyHostToDevice, c->stream));
+  HIPCHK(hipMemcpyAsync(c->pk[nxt].p + distinct, pk.data(), nr * 4, hipMemcpyHostToDevice, c->stream));
+  HIPCHK(hipMemcpyAsync(c->g1[nxt].p + distinct, none.data(), nr, hipMemcpyHostToDevice, c->stream));
+  HIPCHK(hipMemcpyAsync(c->g2[nxt].p + distinct, none.data(), nr, hipMemcpyHostToDevice, c->stream));
+  HIPCHK(hipMemcpyAsync(c->finst[nxt].p + distinct, slot.data(), nr * 4, hipMemcpyHostToDevice, c->stream));
+  HIPCHK(hipMemcpyAsync(tp + distinct, np.data(), nr * 4, hipMemcpyHostToDevice, c->stream));
+  HIPCHK(hipMemcpyAsync(tg + distinct, none.data(), nr, hipMemcpyHostToDevice, c->stream));
+  HIPCHK(hipStreamSynchronize(c->stream));
+  for (long long i = 0; i < nr; i++) c->binst[c->pending[i].inst].root_step = c->gstep + 1;
+  c->pending.clear();
+}
+
+// the lexicographically first circuit of a FOUND child: gate of the child, then the tree walked up to the root
+static void walk_circuit(chz_ctx *c, unsigned long long o, int G, std::vector<int> &seq) {
+  seq.clear();
+  seq.push_back((int)(o % G));
+  uint32_t i = (uint32_t)(o / G);
+  for (int s = c->gstep; s >= 0; s--) {
+    uint32_t par;
+    uint8_t g;
+    HIPCHK(hipMemcpy(&par, c->tree_par[s] + i, 4, hipMemcpyDeviceToHost));
+    if (par == NO_PARENT) break;
+    HIPCHK(hipMemcpy(&g, c->tree_gate[s] + i, 1, hipMemcpyDeviceToHost));
+    seq.push_back(g);
+    i = par;
+  }
+  std::reverse(seq.begin(), seq.end());
+}
+
+// tree levels no live search can reach any more
+static void prune_tree(chz_ctx *c) {
+  int oldest = c->gstep;
+  for (const BInst &b : c->binst) if (b.live) oldest = std::min(oldest, b.root_step);
+  for (int s = 0; s < oldest && s < (int)c->tree_par.size(); s++) {
+    if (c->tree_par[s]) { (void)hipFree(c->tree_par[s]); c->tree_par[s] = nullptr; }
+    if (c->tree_gate[s]) { (void)hipFree(c->tree_gate[s]); c->tree_gate[s] = nullptr; }
+  }
+}
+
+// One level of every live search of the batch (src/cheese.cu:1249-1377 per search): expansion in chunks,
+// survivor compaction, sort, exact dedupe, next frontier; then the searches admitted since the last step join
+// the new frontier as roots.  Level lines and results go to the context's event queue.
+template <int NQ> static void batch_step(chz_ctx *c) {
+  constexpr int N = 1 << NQ, E = N * N, CPW = 64 / E;
+  const KSpec &ks = c->ks;
+  const int G = ks.G;
+  if (c->m == 0 && c->pending.empty()) return;
+  if (c->m == 0) {   // a frontier of roots only
+    const long long nr = (long long)c->pending.size();
+    const int nxt = c->cur ^ 1;
+    c->res[nxt].need(nr * E); c->pk[nxt].need(nr); c->g1[nxt].need(nr); c->g2[nxt].need(nr); c->finst[nxt].need(nr);
+    uint32_t *tp = nullptr;
+    uint8_t *tg = nullptr;
+    HIPCHK(hipMalloc((void **)&tp, nr * 4));
+    HIPCHK(hipMalloc((void **)&tg, nr));
+    c->tree_par.push_back(tp);
+    c->tree_gate.push_back(tg);
+    c->gstep = (int)c->tree_par.size() - 1;
+    c->gstep--;                 // append_roots marks roots with gstep + 1
+    append_roots<NQ>(c, nxt, 0, tp, tg);
+    c->gstep++;
+    c->cur = nxt;
+    c->m = nr;
+    const std::vector<long long> cnts = slot_counts(c, c->finst[c->cur].p, c->m);
+    for (BInst &b : c->binst) if (b.live) b.nodes = cnts[b.slot];
+  }
+  const double tl = now_s();
+  const int cur = c->cur;
+  const long long m = c->m;
+  // per-slot budgets
+  std::vector<int> rem(c->nslots, 0);
+  std::vector<uint8_t> keepi(c->nslots, 0), active(c->nslots, 0);
+  std::vector<unsigned long long> ic((size_t)c->nslots * IC_N, 0);
+  for (int s = 0; s < c->nslots; s++) ic[(size_t)s * IC_N + IC_FIRST] = ~0ull;
+  for (const BInst &b : c->binst)
+    if (b.live) { rem[b.slot] = b.depth - (b.t + 1); keepi[b.slot] = b.t + 1 < b.depth; }
+  HIPCHK(hipMemcpyAsync(c->d_rem.p, rem.data(), c->nslots * 4, hipMemcpyHostToDevice, c->stream));
+  HIPCHK(hipMemcpyAsync(c->d_keepi.p, keepi.data(), c->nslots, hipMemcpyHostToDevice, c->stream));
+  HIPCHK(hipMemcpyAsync(c->icnt.p, ic.data(), ic.size() * 8, hipMemcpyHostToDevice, c->stream));
+  const long long chunk = c->opts.chunk > 0 ? c->opts.chunk : (1ll << 20);
+  const long long maxcc = std::min<long long>(chunk, m) * G;
+  c->status.need(maxcc);
+  c->info.need(maxcc);
+  c->keys.need(maxcc);
+  c->bcount.need((maxcc + CT_TILE - 1) / CT_TILE + 1);
+  c->cnt.need(C_N);
+  HIPCHK(hipMemsetAsync(c->cnt.p, 0, C_N * 8, c->stream));
+  const uint32_t *pinst = c->finst[cur].p;
+  long long nsurv = 0;
+  for (long long off = 0; off < m; off += chunk) {
+    const long long cm = std::min(chunk, m - off), ncc = cm * G;
+    ExpandArgs ea{c->res[cur].p, c->pk[cur].p, c->g1[cur].p, c->g2[cur].p, off, ncc, 0, 0,
+                  c->status.p, c->keys.p, c->info.p, c->cnt.p, pinst, c->d_rem.p, c->d_keepi.p, c->icnt.p};
+    c->ev.begin(K_EXPAND, c->stream);
+    hipLaunchKernelGGL(k_level<NQ>, dim3(grid_for((cm + CPW - 1) / CPW)), dim3(256), 0, c->stream, ks, ea, cm);
+    c->ev.end(c->stream);
+    const long long nb = (ncc + CT_TILE - 1) / CT_TILE;
+    const FlagAlive fa{c->status.p, pinst, c->d_keepi.p, off, G};
+    c->ev.begin(K_COMPACT, c->stream);
+    hipLaunchKernelGGL(k_flag_count<FlagAlive>, dim3((unsigned)nb), dim3(CT_THREADS), 0, c->stream, fa, ncc, c->bcount.p);
+    hipLaunchKernelGGL(k_scan_one_block, dim3(1), dim3(1024), 0, c->stream, c->bcount.p, nb);
+    c->ev.end(c->stream);
+    unsigned long long tot = 0;
+    HIPCHK(hipMemcpyAsync(&tot, c->bcount.p + nb, 8, hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(hipStreamSynchronize(c->stream));
+    if (!tot) continue;
+    if ((size_t)(nsurv + tot) > c->surv_o.cap) {   // grow, keeping what earlier chunks wrote
+      DBuf<uint64_t> no, nk;
+      no.need(nsurv + tot);
+      nk.need(nsurv + tot);
+      if (nsurv) {
+        HIPCHK(hipMemcpyAsync(no.p, c->surv_o.p, nsurv * 8, hipMemcpyDeviceToDevice, c->stream));
+        HIPCHK(hipMemcpyAsync(nk.p, c->surv_key.p, nsurv * 8, hipMemcpyDeviceToDevice, c->stream));
+        HIPCHK(hipStreamSynchronize(c->stream));
+      }
+      c->surv_o.release();
+      c->surv_key.release();
+      c->surv_o = no;
+      c->surv_key = nk;
+    }
+    c->ev.begin(K_COMPACT, c->stream);
+    hipLaunchKernelGGL(k_scatter_alive, dim3((unsigned)nb), dim3(CT_THREADS), 0, c->stream, fa, ncc, c->bcount.p,
+                       c->keys.p, off * G, nsurv, c->surv_o.p, c->surv_key.p);
+    c->ev.end(c->stream);
+    c->k_work[K_COMPACT] += (double)ncc + (double)tot * 24.0;
+    nsurv += (long long)tot;
+  }
+  unsigned long long hc[C_N];
+  HIPCHK(hipMemcpyAsync(hc, c->cnt.p, C_N * 8, hipMemcpyDeviceToHost, c->stream));
+  HIPCHK(hipMemcpyAsync(ic.data(), c->icnt.p, ic.size() * 8, hipMemcpyDeviceToHost, c->stream));
+  HIPCHK(hipStreamSynchronize(c->stream));
+  c->ev.flush(c->k_ms, c->k_launch);
+  c->k_work[K_EXPAND] += (double)hc[C_NEEDT] * t_ops16(NQ);
+  // searches that end at this level with a circuit or an overflow; the rest continue if their budget allows
+  for (const BInst &b : c->binst)
+    if (b.live) {
+      const unsigned long long *x = &ic[(size_t)b.slot * IC_N];
+      active[b.slot] = !x[IC_FOUND] && !x[IC_OVF] && keepi[b.slot];
+    }
+  HIPCHK(hipMemcpyAsync(c->d_active.p, active.data(), c->nslots, hipMemcpyHostToDevice, c->stream));
+  long long distinct = 0;
+  const int nxt = cur ^ 1;
+  if (nsurv > 0) {
+    c->kept.need(nsurv);
+    c->kidx.need(nsurv);
+    if (c->opts.dedupe && nsurv > 1) {
+      for (int bb = 0; bb < 2; bb++) { c->sk[bb].need(nsurv); c->sv[bb].need(nsurv); }
+      c->runs.need(nsurv);
+      c->ev.begin(K_SORT, c->stream);
+      hipLaunchKernelGGL(k_init_sort, dim3(grid_for(nsurv / 64 + 1)), dim3(256), 0, c->stream, c->surv_key.p, nsurv,
+                         c->sk[0].p, c->sv[0].p);
+      sort_pairs(c, nsurv);
+      c->ev.end(c->stream);
+      c->k_work[K_SORT] += (double)nsurv * (16.0 + 4 * (4.0 + 16.0));
+      c->ev.begin(K_DEDUPE, c->stream);
+      HIPCHK(hipMemsetAsync(c->cnt.p + C_RUNS, 0, 8, c->stream));
+      hipLaunchKernelGGL(k_run_heads, dim3(grid_for(nsurv / 64 + 1)), dim3(256), 0, c->stream, c->sk[0].p,
+                         c->sv[0].p, nsurv, c->kept.p, c->runs.p, c->cnt.p);
+      const DedupeArgs da{c->sk[0].p, c->sv[0].p, nsurv, c->surv_o.p, c->surv_key.p, c->runs.p,
+                          c->res[cur].p, c->pk[cur].p, c->kept.p, c->cnt.p, pinst};
+      hipLaunchKernelGGL(k_dedupe_runs<NQ>, dim3(grid_for(nsurv / CPW + 1, 4096)), dim3(256), 0, c->stream, ks, da);
+      c->ev.end(c->stream);
+      c->k_work[K_DEDUPE] += (double)nsurv * 9.0;
+    } else {
+      HIPCHK(hipMemsetAsync(c->kept.p, 1, nsurv, c->stream));
+    }
+    const long long nb = (nsurv + CT_TILE - 1) / CT_TILE;
+    c->bcount.need(nb + 1);
+    const FlagKept fk{c->kept.p, c->surv_o.p, pinst, c->d_active.p, G};
+    c->ev.begin(K_COMPACT, c->stream);
+    hipLaunchKernelGGL(k_flag_count<FlagKept>, dim3((unsigned)nb), dim3(CT_THREADS), 0, c->stream, fk, nsurv, c->bcount.p);
+    hipLaunchKernelGGL(k_scan_one_block, dim3(1), dim3(1024), 0, c->stream, c->bcount.p, nb);
+    hipLaunchKernelGGL(k_scatter_idx<FlagKept>, dim3((unsigned)nb), dim3(CT_THREADS), 0, c->stream, fk, nsurv,
+                       c->bcount.p, c->kidx.p);
+    c->ev.end(c->stream);
+    unsigned long long tot = 0;
+    HIPCHK(hipMemcpyAsync(&tot, c->bcount.p + nb, 8, hipMemcpyDeviceToHost, c->stream));
     HIPCHK(hipMemcpyAsync(hc, c->cnt.p, C_N * 8, hipMemcpyDeviceToHost, c->stream));
     HIPCHK(hipStreamSynchronize(c->stream));
+    if (hc[C_REPOVF]) { set_err("a dedupe run holds more distinct classes than its buffer"); throw DeviceError(); }
+    distinct = (long long)tot;
+  }
+  // next frontier: kept children, then the searches admitted since the last step
+  const long long mn = distinct + (long long)c->pending.size();
+  std::vector<long long> cnts(c->nslots, 0);
+  if (mn) {
+    c->res[nxt].need((size_t)mn * E); c->pk[nxt].need(mn); c->g1[nxt].need(mn); c->g2[nxt].need(mn);
+    c->finst[nxt].need(mn);
+    uint32_t *tp = nullptr;
+    uint8_t *tg = nullptr;
+    HIPCHK(hipMalloc((void **)&tp, mn * 4));
+    HIPCHK(hipMalloc((void **)&tg, mn));
+    if (distinct) {
+      const GatherArgs ga{c->kidx.p, distinct, c->surv_o.p, c->res[cur].p, c->pk[cur].p, c->g1[cur].p,
+                          c->res[nxt].p, c->pk[nxt].p, c->g1[nxt].p, c->g2[nxt].p, tp, tg, pinst, c->finst[nxt].p};
+      c->ev.begin(K_GATHER, c->stream);
+      hipLaunchKernelGGL(k_gather<NQ>, dim3(grid_for((distinct + CPW - 1) / CPW)), dim3(256), 0, c->stream, ks, ga);
+      c->ev.end(c->stream);
+      c->k_work[K_GATHER] += (double)distinct * (E * 16.0 * 2.0 + 16.0);
+    }
+    append_roots<NQ>(c, nxt, distinct, tp, tg);
+    c->tree_par.push_back(tp);
+    c->tree_gate.push_back(tg);
+    cnts = slot_counts(c, c->finst[nxt].p, mn);
     c->ev.flush(c->k_ms, c->k_launch);
   }
-
-  static int run(chz_ctx *c, chz_level_cb cb, void *user, chz_result *res) {
-    const KSpec &ks = c->ks;
-    const int G = ks.G, depth = c->depth;
-    const double t0 = now_s();
-    long long m = 1;
-    int cur = 0;
-    {   // level 0: the target itself
-      std::vector<int4> U(E);
-      for (int i = 0; i < E; i++) {
-        const Zq &z = c->hs.target[i];
-        U[i] = make_int4((int)z.a, (int)z.b, (int)z.c, (int)z.d);
-      }
-      c->res[0].need(E); c->pk[0].need(1); c->g1[0].need(1); c->g2[0].need(1);
-      const int k0 = c->hs.tk;
-      const uint8_t none = 255;
-      HIPCHK(hipMemcpyAsync(c->res[0].p, U.data(), E * sizeof(int4), hipMemcpyHostToDevice, c->stream));
-      HIPCHK(hipMemcpyAsync(c->pk[0].p, &k0, sizeof(int), hipMemcpyHostToDevice, c->stream));
-      HIPCHK(hipMemcpyAsync(c->g1[0].p, &none, 1, hipMemcpyHostToDevice, c->stream));
-      HIPCHK(hipMemcpyAsync(c->g2[0].p, &none, 1, hipMemcpyHostToDevice, c->stream));
-      HIPCHK(hipStreamSynchronize(c->stream));
+  // events of this level, per search
+  const double now = now_s();
+  for (size_t bi = 0; bi < c->binst.size(); bi++) {
+    BInst &b = c->binst[bi];
+    if (!b.live || b.root_step > c->gstep) continue;      // admitted in this step: it has not expanded yet
+    const unsigned long long *x = &ic[(size_t)b.slot * IC_N];
+    chz_event evn{};
+    evn.tag = b.tag;
+    if (x[IC_OVF]) {
+      evn.kind = 1;
+      evn.result.rc = 4;
+      evn.result.seconds = now - b.t0;
+      set_err("residual coefficients exceed the storage type (int32)");
+    } else if (x[IC_FOUND]) {
+      std::vector<int> seq;
+      walk_circuit(c, x[IC_FIRST], G, seq);
+      evn.kind = 1;
+      evn.result.rc = 0;
+      evn.result.len = (int)seq.size();
+      evn.result.found = (long)x[IC_FOUND];
+      for (size_t j = 0; j < seq.size() && j < CHZ_DMAX; j++) evn.result.circuit[j] = seq[j];
+      evn.result.seconds = now - b.t0;
+    } else {
+      const long long dist = active[b.slot] ? cnts[b.slot] : 0;
+      chz_event lv{};
+      lv.tag = b.tag;
+      lv.kind = 0;
+      lv.level.t = b.t + 1;
+      lv.level.parents = (long)b.nodes;
+      lv.level.prefixes = (long)(b.nodes * G - (long long)x[IC_W2]);
+      lv.level.cut_w3 = (long)x[IC_W3];
+      lv.level.cut_lb = (long)x[IC_DEAD];
+      lv.level.survivors = (long)x[IC_ALIVE];
+      lv.level.distinct = (long)dist;
+      lv.level.proved_unsat = dist ? 0 : 1;
+      lv.level.seconds = now - tl;
+      c->events.push_back(lv);
+      b.t++;
+      b.nodes = dist;
+      if (dist) continue;
+      evn.kind = 1;
+      evn.result.rc = 1;
+      evn.result.seconds = now - b.t0;
     }
-    free_tree(c);
-    c->tree_par.push_back(nullptr);
-    c->tree_gate.push_back(nullptr);
-    const long long chunk = c->opts.chunk > 0 ? c->opts.chunk : (1ll << 20);
-    c->cnt.need(C_N);
-    unsigned long long hc[C_N];
-    for (int t = 1; t <= depth; t++) {
-      const double tl = now_s();
-      const bool keep = t < depth;
-      const long long nc = m * G;
-      const long long maxcc = std::min<long long>(chunk, m) * G;
-      c->status.need(maxcc);
-      c->info.need(maxcc);
-      c->bcount.need((maxcc + CT_TILE - 1) / CT_TILE + 1);
-      if (keep) c->keys.need(maxcc);
-      std::vector<unsigned long long> zero(C_N, 0);
-      zero[C_FIRST] = ~0ull;
-      HIPCHK(hipMemcpyAsync(c->cnt.p, zero.data(), C_N * 8, hipMemcpyHostToDevice, c->stream));
-      long long nsurv = 0;
-      bool found = false;
-      for (long long off = 0; off < m; off += chunk) {
-        const long long cm = std::min(chunk, m - off), ncc = cm * G;
-        const ExpandArgs ea{c->res[cur].p, c->pk[cur].p, c->g1[cur].p, c->g2[cur].p, off, ncc, t, keep ? 1 : 0,
-                            c->status.p, c->keys.p, c->info.p, c->cnt.p};
-        const long long nbc = (ncc + CT_TILE - 1) / CT_TILE;
-        c->ev.begin(K_EXPAND, c->stream);
-        hipLaunchKernelGGL(k_level<NQ>, dim3(grid_for((cm + CPW - 1) / CPW)), dim3(256), 0, c->stream, ks, ea, cm);
-        c->ev.end(c->stream);
-        sync_counts(c, hc);
-        if (hc[C_OVF]) { set_err("residual coefficients exceed the storage type (int32)"); return 4; }
-        if (hc[C_FOUND]) found = true;
-        if (found) continue;          // the level ends with a circuit: only FOUND and overflow still count
-        if (keep) {
-          const long long nb = nbc;
-          c->ev.begin(K_COMPACT, c->stream);
-          hipLaunchKernelGGL(k_flag_count<FlagAlive>, dim3((unsigned)nb), dim3(CT_THREADS), 0, c->stream,
-                             FlagAlive{c->status.p}, ncc, c->bcount.p);
-          hipLaunchKernelGGL(k_scan_one_block, dim3(1), dim3(1024), 0, c->stream, c->bcount.p, nb);
-          c->ev.end(c->stream);
-          unsigned long long tot = 0;
-          HIPCHK(hipMemcpyAsync(&tot, c->bcount.p + nb, 8, hipMemcpyDeviceToHost, c->stream));
-          HIPCHK(hipStreamSynchronize(c->stream));
-          if (tot) {
-            if ((size_t)(nsurv + tot) > c->surv_o.cap) {   // grow, keeping what earlier chunks wrote
-              DBuf<uint64_t> no, nk;
-              no.need(nsurv + tot);
-              nk.need(nsurv + tot);
-              if (nsurv) {
-                HIPCHK(hipMemcpyAsync(no.p, c->surv_o.p, nsurv * 8, hipMemcpyDeviceToDevice, c->stream));
-                HIPCHK(hipMemcpyAsync(nk.p, c->surv_key.p, nsurv * 8, hipMemcpyDeviceToDevice, c->stream));
-                HIPCHK(hipStreamSynchronize(c->stream));
-              }
-              c->surv_o.release();
-              c->surv_key.release();
-              c->surv_o = no;
-              c->surv_key = nk;
-            }
-            c->ev.begin(K_COMPACT, c->stream);
-            hipLaunchKernelGGL(k_scatter_alive, dim3((unsigned)nb), dim3(CT_THREADS), 0, c->stream,
-                               FlagAlive{c->status.p}, ncc, c->bcount.p, c->keys.p, off * G, nsurv, c->surv_o.p,
-                               c->surv_key.p);
-            c->ev.end(c->stream);
-            c->k_work[K_COMPACT] += (double)ncc + (double)tot * 24.0;
-            nsurv += (long long)tot;
-          }
-        }
-      }
-      sync_counts(c, hc);
-      c->k_work[K_EXPAND] += (double)hc[C_NEEDT] * t_ops16(NQ);
-      if (found) {   // the lexicographically first circuit: walk the tree up from the first FOUND child
-        const unsigned long long o = hc[C_FIRST];
-        std::vector<int> seq(t);
-        seq[t - 1] = (int)(o % G);
-        uint32_t i = (uint32_t)(o / G);
-        for (int l = t - 1; l >= 1; l--) {
-          uint8_t gg;
-          uint32_t pp;
-          HIPCHK(hipMemcpy(&gg, c->tree_gate[l] + i, 1, hipMemcpyDeviceToHost));
-          HIPCHK(hipMemcpy(&pp, c->tree_par[l] + i, 4, hipMemcpyDeviceToHost));
-          seq[l - 1] = gg;
-          i = pp;
-        }
-        res->rc = 0;
-        res->len = t;
-        res->found = (long)hc[C_FOUND];
-        for (int j = 0; j < t; j++) res->circuit[j] = seq[j];
-        res->seconds = now_s() - t0;
-        return 0;
-      }
-      long long distinct = 0;
-      if (keep && nsurv > 0) {
-        const int nxt = cur ^ 1;
-        c->kept.need(nsurv);
-        c->kidx.need(nsurv);
-        if (c->opts.dedupe && nsurv > 1) {
-          for (int b = 0; b < 2; b++) { c->sk[b].need(nsurv); c->sv[b].need(nsurv); }
-          c->runs.need(nsurv);
-          c->ev.begin(K_SORT, c->stream);
-          hipLaunchKernelGGL(k_init_sort, dim3(grid_for(nsurv / 64 + 1)), dim3(256), 0, c->stream, c->surv_key.p, nsurv,
-                             c->sk[0].p, c->sv[0].p);
-          sort_pairs(c, nsurv);
-          c->ev.end(c->stream);
-          c->k_work[K_SORT] += (double)nsurv * (16.0 + 4 * (4.0 + 16.0));
-          c->ev.begin(K_DEDUPE, c->stream);
-          HIPCHK(hipMemsetAsync(c->cnt.p + C_RUNS, 0, 8, c->stream));
-          hipLaunchKernelGGL(k_run_heads, dim3(grid_for(nsurv / 64 + 1)), dim3(256), 0, c->stream, c->sk[0].p,
-                             c->sv[0].p, nsurv, c->kept.p, c->runs.p, c->cnt.p);
-          const DedupeArgs da{c->sk[0].p, c->sv[0].p, nsurv, c->surv_o.p, c->surv_key.p, c->runs.p,
-                              c->res[cur].p, c->pk[cur].p, c->kept.p, c->cnt.p};
-          hipLaunchKernelGGL(k_dedupe_runs<NQ>, dim3(grid_for(nsurv / CPW + 1, 4096)), dim3(256), 0, c->stream, ks, da);
-          c->ev.end(c->stream);
-          c->k_work[K_DEDUPE] += (double)nsurv * 9.0;
-        } else {
-          HIPCHK(hipMemsetAsync(c->kept.p, 1, nsurv, c->stream));
-        }
-        const long long nb = (nsurv + CT_TILE - 1) / CT_TILE;
-        c->bcount.need(nb + 1);
-        c->ev.begin(K_COMPACT, c->stream);
-        hipLaunchKernelGGL(k_flag_count<FlagKept>, dim3((unsigned)nb), dim3(CT_THREADS), 0, c->stream,
-                           FlagKept{c->kept.p}, nsurv, c->bcount.p);
-        hipLaunchKernelGGL(k_scan_one_block, dim3(1), dim3(1024), 0, c->stream, c->bcount.p, nb);
-        hipLaunchKernelGGL(k_scatter_idx<FlagKept>, dim3((unsigned)nb), dim3(CT_THREADS), 0, c->stream,
-                           FlagKept{c->kept.p}, nsurv, c->bcount.p, c->kidx.p);
-        c->ev.end(c->stream);
-        unsigned long long tot = 0;
-        HIPCHK(hipMemcpyAsync(&tot, c->bcount.p + nb, 8, hipMemcpyDeviceToHost, c->stream));
-        sync_counts(c, hc);
-        if (hc[C_REPOVF]) { set_err("a dedupe run holds more distinct classes than its buffer"); return 4; }
-        distinct = (long long)tot;
-        if (distinct) {
-          c->res[nxt].need((size_t)distinct * E);
-          c->pk[nxt].need(distinct);
-          c->g1[nxt].need(distinct);
-          c->g2[nxt].need(distinct);
-          uint32_t *tp = nullptr;
-          uint8_t *tg = nullptr;
-          HIPCHK(hipMalloc((void **)&tp, distinct * 4));
-          HIPCHK(hipMalloc((void **)&tg, distinct));
-          c->tree_par.push_back(tp);
-          c->tree_gate.push_back(tg);
-          const GatherArgs ga{c->kidx.p, distinct, c->surv_o.p, c->res[cur].p, c->pk[cur].p, c->g1[cur].p,
-                              c->res[nxt].p, c->pk[nxt].p, c->g1[nxt].p, c->g2[nxt].p, tp, tg};
-          c->ev.begin(K_GATHER, c->stream);
-          hipLaunchKernelGGL(k_gather<NQ>, dim3(grid_for((distinct + CPW - 1) / CPW)), dim3(256), 0, c->stream, ks, ga);
-          c->ev.end(c->stream);
-          c->k_work[K_GATHER] += (double)distinct * (E * 16.0 * 2.0 + 16.0);
-          HIPCHK(hipStreamSynchronize(c->stream));
-          c->ev.flush(c->k_ms, c->k_launch);
-        }
-        cur = nxt;
-      }
-      chz_level lv{};
-      lv.t = t;
-      lv.parents = (long)m;
-      lv.prefixes = (long)(nc - (long long)hc[C_W2]);
-      lv.cut_w3 = (long)hc[C_W3];
-      lv.cut_lb = (long)hc[C_DEAD];
-      lv.survivors = (long)hc[C_ALIVE];
-      lv.distinct = (long)distinct;
-      lv.proved_unsat = distinct ? 0 : 1;
-      lv.seconds = now_s() - tl;
-      if (cb) cb(&lv, user);
-      if (!distinct) { res->rc = 1; res->seconds = now_s() - t0; return 1; }
-      m = distinct;
-    }
-    res->rc = 1;   // depth 0 with a target that is not the goal: nothing to expand
-    res->seconds = now_s() - t0;
-    return 1;
+    c->events.push_back(evn);
+    b.live = false;
   }
-};
+  // roots admitted in this step
+  for (BInst &b : c->binst)
+    if (b.live && b.root_step == c->gstep + 1) b.nodes = cnts[b.slot];
+  c->cur = nxt;
+  c->m = mn;
+  c->gstep++;
+  prune_tree(c);
+}
 
 static void upload_spec(chz_ctx *c) {
   const HostSpec &s = c->hs;
@@ -685,6 +855,60 @@ static void upload_spec(chz_ctx *c) {
   k.symperm = c->d_symperm; k.symconj = c->d_symconj;
 }
 
+// a new search of the batch: the depth-0 cases end at once (src/cheese.cu:1794-1801), the rest wait to join
+// the frontier as roots at the next step
+static int batch_add_parsed(chz_ctx *c, const HostSpec &hs, int depth, int tag) {
+  const int N = hs.N(), E = N * N;
+  chz_event ev{};
+  ev.tag = tag;
+  ev.kind = 1;
+  for (int i = 0; i < E; i++) {
+    const Zq &z = hs.target[i];
+    if (z.a != (int32_t)z.a || z.b != (int32_t)z.b || z.c != (int32_t)z.c || z.d != (int32_t)z.d) {
+      set_err("target coefficients exceed int32");
+      ev.result.rc = 4;
+      c->events.push_back(ev);
+      return 0;
+    }
+  }
+  HostMat U;
+  U.e = hs.target;
+  U.k = hs.tk;
+  if (host_equal(U, c->hs.goal, c->hs.gk)) { ev.result.rc = 0; c->events.push_back(ev); return 0; }
+  if (depth <= 0) { ev.result.rc = 1; c->events.push_back(ev); return 0; }
+  // a free instance slot: not held by a live search
+  std::vector<bool> used(c->nslots + c->binst.size() + 1, false);
+  for (const BInst &b : c->binst) if (b.live) used[b.slot] = true;
+  int slot = 0;
+  while (used[slot]) slot++;
+  try {
+    HIPCHK(hipSetDevice(c->device));
+    reserve_slots(c, slot + 1);
+  } catch (DeviceError &) {
+    return 5;
+  }
+  int bi = -1;
+  for (size_t i = 0; i < c->binst.size(); i++) if (!c->binst[i].live) { bi = (int)i; break; }
+  if (bi < 0) { c->binst.emplace_back(); bi = (int)c->binst.size() - 1; }
+  BInst &b = c->binst[bi];
+  b = BInst{};
+  b.tag = tag;
+  b.depth = depth;
+  b.slot = slot;
+  b.live = true;
+  b.t0 = now_s();
+  Pending q;
+  q.inst = bi;
+  q.k = hs.tk;
+  q.U.resize(E);
+  for (int i = 0; i < E; i++) {
+    const Zq &z = hs.target[i];
+    q.U[i] = make_int4((int)z.a, (int)z.b, (int)z.c, (int)z.d);
+  }
+  c->pending.push_back(std::move(q));
+  return 0;
+}
+
 }  // namespace chz
 
 // ------------------------------------------------------------------------------------------------------
@@ -725,35 +949,67 @@ int chz_run(chz_ctx *c, chz_level_cb cb, void *user, chz_result *out) {
   chz_result tmp{};
   chz_result *res = out ? out : &tmp;
   std::memset(res, 0, sizeof *res);
-  const int N = c->hs.N(), E = N * N;
-  // depth 0: the target itself (src/cheese.cu:1794-1801)
-  for (int i = 0; i < E; i++) {
-    const chz::Zq &z = c->hs.target[i];
-    if (z.a != (int32_t)z.a || z.b != (int32_t)z.b || z.c != (int32_t)z.c || z.d != (int32_t)z.d) {
-      chz::set_err("target coefficients exceed int32");
-      return res->rc = 4;
-    }
-  }
-  {
-    chz::HostMat U;
-    U.e = c->hs.target;
-    U.k = c->hs.tk;
-    if (chz::host_equal(U, c->hs.goal, c->hs.gk)) { res->rc = 0; res->len = 0; res->found = 0; return 0; }
-  }
+  if (chz_batch_live(c)) { chz::set_err("the context's batch holds searches"); return res->rc = 3; }
   for (int f = 0; f < chz::K_NFAM; f++) { c->k_launch[f] = 0; c->k_ms[f] = 0; c->k_work[f] = 0; }
+  c->events.clear();
+  const int rc0 = chz::batch_add_parsed(c, c->hs, c->depth, 0);
+  if (rc0) return res->rc = rc0;
+  while (true) {
+    chz_event e;
+    bool done = false;
+    while (!c->events.empty()) {
+      e = c->events.front();
+      c->events.pop_front();
+      if (e.kind == 0) { if (cb) cb(&e.level, user); }
+      else { *res = e.result; done = true; }
+    }
+    if (done) return res->rc;
+    if (const int rc = chz_batch_step(c)) return res->rc = rc;
+  }
+}
+
+int chz_batch_add(chz_ctx *c, const char *spec_text, size_t len, int depth, int tag) {
+  chz::HostSpec hs;
+  std::string err;
+  if (const int prc = chz::parse_spec(std::string(spec_text, len), hs, err)) { chz::set_err(err); return prc; }
+  if (hs.truncated) { chz::set_err("malformed spec"); return 2; }
+  if (hs.states) { chz::set_err("state targets are not supported by this build"); return 3; }
+  if ((int)hs.target.size() != hs.N() * hs.N()) { chz::set_err("spec has no target"); return 2; }
+  if (!chz::same_gate_set(hs, c->hs)) { chz::set_err("the spec's gate set differs from the batch's"); return 3; }
+  const int d = depth >= 0 ? depth : hs.depth;
+  if (d > CHZ_DMAX) { chz::set_err("depth > 48"); return 3; }
+  return chz::batch_add_parsed(c, hs, d, tag);
+}
+
+int chz_batch_step(chz_ctx *c) {
   try {
     HIPCHK(hipSetDevice(c->device));
-    int rc = 3;
     switch (c->hs.n) {
-      case 1: rc = chz::Level<1>::run(c, cb, user, res); break;
-      case 2: rc = chz::Level<2>::run(c, cb, user, res); break;
-      case 3: rc = chz::Level<3>::run(c, cb, user, res); break;
+      case 1: chz::batch_step<1>(c); break;
+      case 2: chz::batch_step<2>(c); break;
+      case 3: chz::batch_step<3>(c); break;
     }
-    res->rc = rc;
-    return rc;
+    return 0;
   } catch (chz::DeviceError &) {
-    return res->rc = 5;
+    return 5;
   }
+}
+
+int chz_batch_events(chz_ctx *c, chz_event *out, int max) {
+  int k = 0;
+  while (k < max && !c->events.empty()) { out[k++] = c->events.front(); c->events.pop_front(); }
+  return k;
+}
+
+int chz_batch_live(const chz_ctx *c) {
+  int n = 0;
+  for (const chz::BInst &b : c->binst) n += b.live;
+  return n;
+}
+
+long long chz_batch_nodes(const chz_ctx *c) {
+  long long n = c->m;
+  return n + (long long)c->pending.size();
 }
 
 int chz_num_gates(const chz_ctx *c) { return c->hs.G(); }
@@ -784,16 +1040,16 @@ int chz_kernel_stats(const chz_ctx *c, chz_kstat *out, int n) {
   return j;
 }
 
-int chz_verify(const chz_ctx *c, const int *circuit, int len) {
+static int verify_hs(const chz::HostSpec &hs, const int *circuit, int len) {
   // product of the forward gate matrices, gate 1 first, compared exactly with the target (verify.py:20-33)
-  const int n = c->hs.n, N = 1 << n;
+  const int n = hs.n, N = 1 << n;
   chz::HostMat P;
   P.e.assign(N * N, chz::Zq{0, 0, 0, 0});
   for (int i = 0; i < N; i++) P.e[i * N + i] = chz::Zq{1, 0, 0, 0};
   for (int j = 0; j < len; j++) {
-    if (circuit[j] < 0 || circuit[j] >= c->hs.G()) return -1;
+    if (circuit[j] < 0 || circuit[j] >= hs.G()) return -1;
     // P <- g P via (P^dag g^dag)^dag: apply g on the left with its forward matrix
-    const chz::HostGate &g = c->hs.gates[circuit[j]];
+    const chz::HostGate &g = hs.gates[circuit[j]];
     const int L = 1 << g.nq;
     int mask = 0;
     for (int q = 0; q < g.nq; q++) mask |= 1 << g.q[q];
@@ -827,7 +1083,20 @@ int chz_verify(const chz_ctx *c, const int *circuit, int len) {
       P.k--;
     }
   }
-  return chz::host_equal(P, c->hs.target, c->hs.tk) ? 1 : 0;
+  return chz::host_equal(P, hs.target, hs.tk) ? 1 : 0;
+}
+
+int chz_verify(const chz_ctx *c, const int *circuit, int len) { return verify_hs(c->hs, circuit, len); }
+
+int chz_verify_spec(const char *spec_text, size_t len, const int *circuit, int n) {
+  chz::HostSpec hs;
+  std::string err;
+  if (chz::parse_spec(std::string(spec_text, len), hs, err) || hs.truncated ||
+      (int)hs.target.size() != hs.N() * hs.N()) {
+    chz::set_err(err.empty() ? "malformed spec" : err);
+    return -2;
+  }
+  return verify_hs(hs, circuit, n);
 }
 
 void chz_close(chz_ctx *c) {
@@ -839,6 +1108,8 @@ void chz_close(chz_ctx *c) {
     c->res[b].release(); c->pk[b].release(); c->g1[b].release(); c->g2[b].release();
     c->sk[b].release(); c->sv[b].release();
   }
+  c->finst[0].release(); c->finst[1].release(); c->d_rem.release(); c->d_keepi.release(); c->d_active.release();
+  c->icnt.release(); c->scan_tmp.release(); c->d_lo.release(); c->d_hi.release();
   c->status.release(); c->kept.release(); c->keys.release(); c->surv_o.release(); c->surv_key.release();
   c->info.release(); c->cnt.release(); c->bcount.release(); c->hist.release(); c->runs.release(); c->kidx.release();
   void *ptrs[] = {c->d_gates, c->d_bad2, c->d_symperm, c->d_symconj, c->d_bad3, c->d_goal, c->d_cx};

@@ -201,4 +201,24 @@ bool host_equal(const HostMat &S, const std::vector<Zq> &M, int k) {
   return true;
 }
 
+static bool zq_vec_eq(const std::vector<Zq> &a, const std::vector<Zq> &b) {
+  if (a.size() != b.size()) return false;
+  for (size_t i = 0; i < a.size(); i++)
+    if (a[i].a != b[i].a || a[i].b != b[i].b || a[i].c != b[i].c || a[i].d != b[i].d) return false;
+  return true;
+}
+
+bool same_gate_set(const HostSpec &a, const HostSpec &b) {
+  if (a.n != b.n || a.states != b.states || a.standard != b.standard || a.lite != b.lite || a.tdiv != b.tdiv ||
+      a.gk != b.gk || a.G() != b.G() || a.bad2 != b.bad2 || a.bad3 != b.bad3 || a.cutmask != b.cutmask ||
+      a.cutsize != b.cutsize || a.rad != b.rad || a.cxtable != b.cxtable || !zq_vec_eq(a.goal, b.goal))
+    return false;
+  for (int g = 0; g < a.G(); g++) {
+    const HostGate &x = a.gates[g], &y = b.gates[g];
+    if (x.nq != y.nq || x.gk != y.gk || x.ik != y.ik || !zq_vec_eq(x.fwd, y.fwd) || !zq_vec_eq(x.inv, y.inv)) return false;
+    for (int j = 0; j < x.nq; j++) if (x.q[j] != y.q[j]) return false;
+  }
+  return true;
+}
+
 }  // namespace chz

@@ -51,4 +51,9 @@ struct HostMat { std::vector<Zq> e; int k = 0; bool ovf = false; };
 void host_apply_inverse(HostMat &S, const HostGate &g, int n);   // S <- S g^-1, normalised
 bool host_equal(const HostMat &S, const std::vector<Zq> &M, int k);
 
+// two specs share everything the search kernels read except the target and the depth: qubit count, mode,
+// candidates and their matrices, windows, cuts, cx table, bound flags and goal (searches of such specs can run
+// as one batch)
+bool same_gate_set(const HostSpec &a, const HostSpec &b);
+
 }  // namespace chz

@@ -84,9 +84,32 @@ int chz_kernel_stats(const chz_ctx *ctx, chz_kstat *out, int n);
 
 /* Exact check of a gate list against the spec's target (verify.py:20-33 restated): 1 equal, 0 not, <0 error. */
 int chz_verify(const chz_ctx *ctx, const int *circuit, int len);
+/* the same check against the target of a .spec text, on the host only (no device work): 1, 0, <0 error */
+int chz_verify_spec(const char *spec_text, size_t len, const int *circuit, int n);
 
 /* last error message of the calling thread (stderr text of the reference) */
 const char *chz_last_error(void);
+
+/* ---- batches (no reference counterpart: the reference runs one search per process) ----
+ * A context also carries a batch: any number of searches whose specs share its gate set (everything but the
+ * target and the depth, e.g. every spec written for the same qubit count and gate set) advance level by level
+ * in the same kernel launches, their frontier nodes tagged by search.  Each search's level lines and result
+ * are those it gets alone (src/cheese.cu:1249-1377 per search).  chz_run drives a batch of one. */
+typedef struct chz_event {
+  int tag;             /* the caller's tag of the search */
+  int kind;            /* 0: a level line, 1: the search's result (it has finished) */
+  chz_level level;
+  chz_result result;
+} chz_event;
+/* queues a search; it joins at the next step.  depth < 0: the spec's depth.  0, 2 (bad spec), 3 (other gate
+ * set, states mode, depth > 48).  A target that is the goal, or depth 0, finishes at once (a result event). */
+int chz_batch_add(chz_ctx *ctx, const char *spec_text, size_t len, int depth, int tag);
+/* one level of every search in the batch; events are queued.  0, or 5 (device error) */
+int chz_batch_step(chz_ctx *ctx);
+/* pops up to max queued events in order; returns how many */
+int chz_batch_events(chz_ctx *ctx, chz_event *out, int max);
+int chz_batch_live(const chz_ctx *ctx);         /* searches not finished */
+long long chz_batch_nodes(const chz_ctx *ctx);  /* frontier nodes held on the device, queued roots included */
 
 /* ---------------- Clifford-group table (src/clifford.cu) ----------------
  * The whole n-qubit Clifford group (n <= 3, up to global phase) tabulated by breadth-first search over the gate

@@ -131,6 +131,56 @@ def tcount_batch(cliff, count=256):
     return out
 
 
+def serving(cliff, n_req=400, seed=1):
+    """C5: a seeded mixed stream (50% gate-count 2/3-qubit circuits of 4..13 gates at budget = length, 35% T-count
+    with T-count <= k in 2..7, 15% Clifford re-synthesis of 5..40-letter words).  Capacity: the whole stream
+    offered at once, plain (one request at a time) and served (continuous batching).  Latency: Poisson arrivals
+    at 1.5x the plain capacity, the same trace for both modes.  Served answers must equal plain answers."""
+    from cheese_amd import serve
+    from cheese_amd import workloads as W
+    t = time.perf_counter()
+    reqs = serve.mixed_requests(n_req, seed, cliff)
+    gen = time.perf_counter() - t
+    spec2 = W.spec_from_skeleton(W.skeleton(2), W.unitary(2, []), 1)
+    spec3 = W.spec_from_skeleton(W.skeleton(3), W.unitary(3, []), 1)
+    with open(os.path.join(ROOT, "tests", "golden", "tspec", "toffoli.tspec")) as f:
+        tspec = f.read()
+    out = {"requests": n_req, "mix": {"gates": sum(r["kind"] == serve.GATES for r in reqs),
+                                      "tcount": sum(r["kind"] == serve.TCOUNT for r in reqs),
+                                      "clifford": sum(r["kind"] == serve.CLIFFORD for r in reqs)},
+           "generation_s_host": round(gen, 2)}
+    res = {}
+    for mode in ("plain", "served"):
+        srv = serve.Server(spec2, spec3, tspec, plain=(mode == "plain"))
+        try:
+            ans, wall = serve.replay(srv, reqs, [0.0] * n_req)
+        finally:
+            srv.close()
+        res[mode] = ans
+        out[mode + "_capacity"] = serve.summarise(ans, wall)
+        out[mode + "_finish_by_kind_s"] = serve.finish_by_kind(reqs, ans)
+        if mode == "plain":
+            out["plain_service_by_kind"] = serve.service_breakdown(reqs, ans)
+    for i in range(n_req):
+        a, b = res["plain"][i], res["served"][i]
+        if any(a[f] != b[f] for f in ("rc", "len", "out", "found", "split")):
+            print(f"bench.py: served answer {i} differs from the plain answer", file=sys.stderr)
+            sys.exit(1)
+    rate = 1.5 * out["plain_capacity"]["solves_per_s"]
+    arrivals = serve.poisson_arrivals(n_req, rate, seed)
+    for mode in ("plain", "served"):
+        srv = serve.Server(spec2, spec3, tspec, plain=(mode == "plain"))
+        try:
+            ans, wall = serve.replay(srv, reqs, arrivals)
+        finally:
+            srv.close()
+        out[mode + "_poisson"] = serve.summarise(ans, wall)
+    out["poisson_rate_per_s"] = round(rate, 2)
+    out["capacity_ratio"] = round(out["served_capacity"]["solves_per_s"] / out["plain_capacity"]["solves_per_s"], 2)
+    out["answers_equal"] = True
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -145,6 +195,8 @@ def main():
                     help="also build the 3-qubit Clifford table and answer 1e5 queries")
     ap.add_argument("--tcount", action=argparse.BooleanOptionalAction, default=True,
                     help="also build the 3-qubit T-count table and search 256 batched targets")
+    ap.add_argument("--serve", type=int, default=400, metavar="N",
+                    help="also serve a mixed stream of N requests, plain and batched (0: skip)")
     a = ap.parse_args()
     if a.gpus != 1:
         print("bench.py: this build runs on one GPU; multi-GPU sharding is a design note only", file=sys.stderr)
@@ -200,7 +252,7 @@ def main():
     }
     if a.random:
         line["random32"] = random_circuits()
-    if a.clifford or a.tcount:
+    if a.clifford or a.tcount or a.serve:
         from cheese_amd import clifford
         t = time.perf_counter()
         cliff = clifford.CliffordTable(3)
@@ -209,6 +261,8 @@ def main():
             line["clifford"] = clifford_table(cliff, wall)
         if a.tcount:
             line["tcount256"] = tcount_batch(cliff)
+        if a.serve:
+            line["serving"] = serving(cliff, a.serve)
         cliff.close()
     if not a.no_cpu:
         line["cpu_baseline"] = cpu_baseline(a.cpu_sample)

@@ -186,6 +186,57 @@ int chz_tc_dedupe_counts(const chz_tc *t, double *out);
 void chz_tc_free(chz_tc *t);
 const char *chz_tc_last_error(void);
 
+
+/* ---------------- request server (no reference counterpart) ----------------
+ * Gate-count, T-count and Clifford re-synthesis requests submitted from any thread and answered by worker
+ * threads.  Served mode batches continuously: each gate-count batch (2 and 3 qubits) advances a level at a
+ * time and admits queued searches between levels while its frontier-memory estimate stays in budget; queued
+ * T-count targets are searched together; queued Clifford words are read back together; the workers' kernels
+ * overlap on the device.  Plain mode answers one request at a time in arrival order on the same engines.
+ * Every answer is the one the request gets alone. */
+enum { CHZ_REQ_GATES = 0, CHZ_REQ_TCOUNT = 1, CHZ_REQ_CLIFFORD = 2 };
+
+typedef struct chz_server chz_server;
+
+typedef struct chz_server_opts {
+  int device;
+  int plain;             /* 1: one request at a time, in arrival order */
+  int tcount_K;          /* T-count table depth (<= 0: 4) */
+  int max_batch;         /* most searches admitted to one batch (<= 0: no limit) */
+  double mem_budget_gb;  /* admission budget of a gate-count batch's frontier (<= 0: 200) */
+} chz_server_opts;
+
+typedef struct chz_request {
+  int kind;              /* CHZ_REQ_* */
+  uint64_t id;
+  const char *text;      /* .spec (gate count) or .tspec (T-count); copied at submission */
+  size_t len;
+  int budget;            /* gate count: depth (< 0: the spec's); T-count: maximum T-count (< 0: 16) */
+  const uint8_t *word;   /* Clifford: letters over the 3-qubit gate set, applied in order; copied */
+  int wlen;
+} chz_request;
+
+typedef struct chz_response {
+  uint64_t id;
+  int kind, rc;          /* rc as the matching entry point's */
+  int len;               /* gates of the circuit, T-count, or letters of the optimal word */
+  int out[2 * CHZ_DMAX]; /* candidate indices / (p, inv) pairs / letters */
+  long found;            /* gate count: children equal to the goal at the last level */
+  int split[2];          /* T-count: rotations peeled, table level */
+  double t_submit, t_done; /* seconds on the server's clock */
+} chz_response;
+
+/* spec2 / spec3: any gate-count spec of the 2- and 3-qubit gate sets served (NULL: not served); tspec3: any
+ * 3-qubit .tspec (its rotation tables; NULL: no T-count).  Builds the T-count and Clifford tables. */
+int chz_server_open(const char *spec2, size_t len2, const char *spec3, size_t len3, const char *tspec3, size_t lent,
+                    const chz_server_opts *opts, chz_server **out);
+int chz_server_submit(chz_server *s, const chz_request *r);
+int chz_server_poll(chz_server *s, chz_response *out, int max);   /* finished answers, up to max */
+long long chz_server_outstanding(const chz_server *s);            /* submitted and not yet answered */
+long long chz_server_peak_bytes(const chz_server *s);             /* largest frontier-memory estimate seen */
+double chz_server_clock(const chz_server *s);                      /* seconds since the server opened */
+void chz_server_close(chz_server *s);                              /* finishes outstanding work first */
+
 #ifdef __cplusplus
 }
 #endif

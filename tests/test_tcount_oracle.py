@@ -64,3 +64,45 @@ def test_reference_decompositions_verify(name):
     assert T.verify(n, U, res["rotations"])
     for drop in range(len(res["rotations"])):
         assert not T.verify(n, U, res["rotations"][:drop] + res["rotations"][drop + 1:])
+
+
+def test_residue_exactness_bound():
+    """the HIP T-count path identifies channel entries (x + y sqrt2) / sqrt2^e by residues mod p = 2^31 - 1,
+    where 2^16 is a square root of 2 and 2^15 its inverse.  For an orthogonal matrix (and hence its orthogonal
+    Galois conjugate) |x| <= sqrt2^e and |y| <= sqrt2^(e-1), so x + 2^16 y lies strictly inside (-p/2, p/2) for
+    every e <= 28, where the residue fixes the integer, and |x| < 2^15 then fixes (x, y).  At e = 29 the
+    bound fails, which is why the host refuses such runs."""
+    import math
+    p = 2 ** 31 - 1
+    assert pow(2 ** 16, 2, p) == 2 and (2 ** 15 * 2 ** 16) % p == 1
+
+    def span(e):
+        return math.floor(2 ** (e / 2)) + 2 ** 16 * math.floor(2 ** ((e - 1) / 2))
+    assert all(span(e) < p / 2 for e in range(1, 29))
+    assert span(29) > p / 2
+    seen = {}
+    for x in range(-300, 301, 7):
+        for y in range(-300, 301, 11):
+            r = (x + 2 ** 16 * y) % p
+            assert seen.setdefault(r, (x, y)) == (x, y)
+
+
+@pytest.mark.parametrize("name", NAMES2 + NAMES3)
+def test_reference_channels_satisfy_the_entry_bound(name):
+    """every entry of the reference's channel matrices obeys |x + y sqrt2| <= sqrt2^e and |x - y sqrt2| <=
+    sqrt2^e, and the matrix is exactly orthogonal"""
+    toks = open(os.path.join(GOLDEN, "tspec", name + ".tspec")).read().split()
+    n = int(toks[toks.index("n") + 1])
+    P = 4 ** n
+    i = toks.index("channel")
+    e = int(toks[i + 1])
+    vals = list(map(int, toks[i + 2:i + 2 + 2 * P * P]))
+    X, Y = vals[0::2], vals[1::2]
+    s2 = 2 ** 0.5
+    for x, y in zip(X, Y):
+        assert abs(x + y * s2) <= s2 ** e + 1e-9 and abs(x - y * s2) <= s2 ** e + 1e-9
+    for b in range(P):
+        for c in range(P):
+            A = sum(X[a * P + b] * X[a * P + c] + 2 * Y[a * P + b] * Y[a * P + c] for a in range(P))
+            B = sum(X[a * P + b] * Y[a * P + c] + Y[a * P + b] * X[a * P + c] for a in range(P))
+            assert B == 0 and A == (2 ** e if b == c else 0)

@@ -227,6 +227,11 @@ def main():
                 s[f] += v[f]
     total = sum(times)
     tb = kstats.get("expand", {"ms": 0.0, "work": 0.0, "launches": 0})
+    traffic = None        # HBM bytes per k_level launch, from the committed rocprofv3 FETCH/WRITE summary
+    tpath = os.path.join(ROOT, "profiles", "k_level-toffoli-d12.traffic.json")
+    if os.path.exists(tpath):
+        with open(tpath) as f:
+            traffic = round(json.load(f)["k_level_hbm_bytes_per_launch"])
     achieved = tb["work"] / (tb["ms"] / 1e3) / 1e12 if tb["ms"] else 0.0
     line = {
         "metric": METRIC,
@@ -244,7 +249,7 @@ def main():
         "config": {"workload": "Toffoli gate-count proof, budget 12 (3 qubits, 27 candidates, 12 symmetries)",
                    "prefixes_per_step": prefixes, "parallelism": "1 GPU"},
         "roofline": {"bound": "valu", "achieved": round(achieved, 3), "peak": VALU16_PEAK_TOPS, "unit": "Top/s",
-                     "frac": round(achieved / VALU16_PEAK_TOPS, 4), "traffic": None,
+                     "frac": round(achieved / VALU16_PEAK_TOPS, 4), "traffic": traffic,
                      "kernel": "k_level", "kernel_ms_per_step": round(tb["ms"] / a.steps, 3),
                      "launches_per_step": tb["launches"] / a.steps,
                      "work": "16-bit integer ops of the T-bound transforms only (cheap bounds not counted)"},

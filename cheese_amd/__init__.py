@@ -139,6 +139,52 @@ def search(spec_text: str, depth: int | None = None, dedupe: bool = True, sym: b
         L.chz_close(ctx)
 
 
+class Searcher:
+    """one context reused for repeated searches of its spec (chz_open once, chz_run per search): device buffers,
+    the stream and the tree pool persist between runs, as in a long-running caller"""
+
+    def __init__(self, spec_text: str, depth: int | None = None, dedupe: bool = True, sym: bool = True,
+                 chunk: int = 0, device: int = 0):
+        L = lib()
+        o = Opts(-1 if depth is None else depth, int(dedupe), int(sym), chunk, 0, device)
+        self._ctx = ctypes.c_void_p()
+        raw = spec_text.encode()
+        rc = L.chz_open(raw, len(raw), ctypes.byref(o), ctypes.byref(self._ctx))
+        if rc:
+            raise SearchError(rc, L.chz_last_error().decode())
+
+    def run(self, stats: bool = False):
+        L = lib()
+        levels = []
+
+        def on_level(p, _user):
+            v = p.contents
+            levels.append({f: getattr(v, f) for f, _ in Level._fields_})
+
+        cb = LEVEL_CB(on_level)
+        res = Result()
+        rc = L.chz_run(self._ctx, cb, None, ctypes.byref(res))
+        out = {"rc": rc, "levels": levels, "found": res.found, "seconds": res.seconds,
+               "circuit_idx": [res.circuit[i] for i in range(res.len)] if rc == 0 else None}
+        if stats:
+            ks = (KStat * 16)()
+            nk = L.chz_kernel_stats(self._ctx, ks, 16)
+            out["kernel_stats"] = {ks[i].name.decode(): {"launches": ks[i].launches, "ms": ks[i].ms, "work": ks[i].work}
+                                   for i in range(nk)}
+        return out
+
+    def close(self):
+        if getattr(self, "_ctx", None):
+            lib().chz_close(self._ctx)
+            self._ctx = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
 def result_line(out: dict) -> str:
     """the reference's result line (src/cheese.cu:1658-1662, :1800)"""
     if out["rc"] != 0:

@@ -8,6 +8,7 @@
 #include <cstdio>
 #include <cstring>
 #include <deque>
+#include <map>
 #include <memory>
 #include <string>
 #include <vector>
@@ -22,7 +23,7 @@ namespace chz {
 
 // ------------------------------------------------------------------------------------------------------
 // counters of one level (device): indices into a u64 array
-enum { C_W2 = 0, C_W3, C_DEAD, C_FOUND, C_ALIVE, C_OVF, C_NEEDT, C_FIRST, C_RUNS, C_REPOVF, C_KEPT, C_EVAL, C_N };
+enum { C_W2 = 0, C_W3, C_DEAD, C_FOUND, C_ALIVE, C_OVF, C_NEEDT, C_FIRST, C_RUNS, C_REPOVF, C_KEPT, C_EVAL, C_SURV, C_N };
 // per-instance counters of one level (a batch carries several searches that share the spec's gate set)
 enum { IC_W2 = 0, IC_W3, IC_DEAD, IC_FOUND, IC_ALIVE, IC_OVF, IC_FIRST, IC_N };
 
@@ -41,6 +42,7 @@ struct ExpandArgs {
   const int *rem;           // per slot: gate budget left after this level's gate (depth - t)
   const uint8_t *keepi;     // per slot: this level's survivors become parents (t < depth)
   unsigned long long *icnt; // per slot: IC_N counters
+  int gsplit;               // a parent's candidates are split among this many lane groups (small levels)
 };
 
 // One level's children, fused (src/cheese.cu:662-720, :959-1021): a lane group per parent holds the parent's
@@ -95,9 +97,13 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) void k
     }
     n_w2 = n_w3 = n_dead = n_found = n_ovf = n_alive = 0;
   };
-  for (long long base = wave * CPW; base < np; base += nwaves * CPW) {
-    const long long pl = base + slot;
-    if (pl >= np) continue;
+  const long long nitems = np * a.gsplit;
+  const int cs = (G + a.gsplit - 1) / a.gsplit;
+  for (long long base = wave * CPW; base < nitems; base += nwaves * CPW) {
+    const long long item = base + slot;
+    if (item >= nitems) continue;
+    const long long pl = item / a.gsplit;
+    const int gbeg = (int)(item % a.gsplit) * cs, gend = gbeg + cs < G ? gbeg + cs : G;
     const long long pi = a.off + pl;
     const int inst = a.pinst ? (int)a.pinst[pi] : 0;
     if (inst != cur) { flush(); cur = inst; }
@@ -109,7 +115,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) void k
     __builtin_amdgcn_wave_barrier();
     P[le] = S;
     __builtin_amdgcn_wave_barrier();
-    for (int g = 0; g < G; g++) {
+    for (int g = gbeg; g < gend; g++) {
       const long long cc = pl * G + g;
       uint8_t stt = 255;
       if (g1 != 255 && sp.bad2[g1 * G + g]) stt = ST_W2;
@@ -251,11 +257,12 @@ struct FlagNeed {       // children that need the T bound, in child order
 
 // survivors of a chunk -> level survivor list (child index in the level, class key)
 __global__ __launch_bounds__(CT_THREADS) void k_scatter_alive(FlagAlive f, long long n, const unsigned long long *boff,
-                                                              const uint64_t *keys, long long child0, long long out0,
-                                                              uint64_t *surv_o, uint64_t *surv_key) {
+                                                              const uint64_t *keys, long long child0,
+                                                              const unsigned long long *out0, uint64_t *surv_o,
+                                                              uint64_t *surv_key) {
   __shared__ int ws[CT_THREADS / 64];
   const long long t0 = (long long)blockIdx.x * CT_TILE;
-  long long pos = out0 + (long long)boff[blockIdx.x];
+  long long pos = (long long)*out0 + (long long)boff[blockIdx.x];
   for (int i = 0; i < CT_ITEMS; i++) {
     const long long x = t0 + (long long)i * CT_THREADS + threadIdx.x;
     const bool fl = x < n && f(x);
@@ -264,6 +271,11 @@ __global__ __launch_bounds__(CT_THREADS) void k_scatter_alive(FlagAlive f, long 
     if (fl) { surv_o[pos + ex] = (uint64_t)(child0 + x); surv_key[pos + ex] = keys[x]; }
     pos += tot;
   }
+}
+
+// the level's survivor count grows by a chunk's total (the scan's last entry)
+__global__ void k_add_total(unsigned long long *acc, const unsigned long long *tot) {
+  if (threadIdx.x == 0 && blockIdx.x == 0) *acc += *tot;
 }
 
 // ------------------------------------------------------------------------------------------------------
@@ -450,6 +462,9 @@ struct chz_ctx {
   chz::DBuf<uint8_t> d_keepi, d_active;
   chz::DBuf<unsigned long long> icnt, scan_tmp;
   chz::DBuf<long long> d_lo, d_hi;
+  long long tune_waves = 4096;     // waves a level should offer before its parents' candidates are split
+  std::map<void *, size_t> tree_blocks;   // every tree block allocated, with its size
+  std::vector<void *> tree_free;          // blocks no live search reaches, ready for reuse
   // measurement
   chz::EventLog ev;
   long k_launch[chz::K_NFAM] = {};
@@ -459,10 +474,30 @@ struct chz_ctx {
 namespace chz {
 
 static void free_tree(chz_ctx *c) {
-  for (auto p : c->tree_par) if (p) (void)hipFree(p);
-  for (auto p : c->tree_gate) if (p) (void)hipFree(p);
+  for (auto &kv : c->tree_blocks) (void)hipFree(kv.first);
+  c->tree_blocks.clear();
+  c->tree_free.clear();
   c->tree_par.clear();
   c->tree_gate.clear();
+}
+
+// one tree level of n nodes (parents u32, then gates u8) from the block pool
+static void tree_alloc(chz_ctx *c, long long n, uint32_t *&tp, uint8_t *&tg) {
+  const size_t need = (size_t)std::max<long long>(n, 1) * 5;
+  void *blk = nullptr;
+  for (size_t i = 0; i < c->tree_free.size(); i++)
+    if (c->tree_blocks[c->tree_free[i]] >= need) {
+      blk = c->tree_free[i];
+      c->tree_free.erase(c->tree_free.begin() + i);
+      break;
+    }
+  if (!blk) {
+    const size_t sz = need + need / 4 + 256;
+    HIPCHK(hipMalloc(&blk, sz));
+    c->tree_blocks[blk] = sz;
+  }
+  tp = (uint32_t *)blk;
+  tg = (uint8_t *)(tp + std::max<long long>(n, 1));
 }
 
 // 16-bit integer operations of one T-bound evaluation as implemented: for each of the 4^n - 1 flip pairs,
@@ -571,8 +606,8 @@ static void prune_tree(chz_ctx *c) {
   int oldest = c->gstep;
   for (const BInst &b : c->binst) if (b.live) oldest = std::min(oldest, b.root_step);
   for (int s = 0; s < oldest && s < (int)c->tree_par.size(); s++) {
-    if (c->tree_par[s]) { (void)hipFree(c->tree_par[s]); c->tree_par[s] = nullptr; }
-    if (c->tree_gate[s]) { (void)hipFree(c->tree_gate[s]); c->tree_gate[s] = nullptr; }
+    if (c->tree_par[s]) { c->tree_free.push_back(c->tree_par[s]); c->tree_par[s] = nullptr; }
+    c->tree_gate[s] = nullptr;
   }
 }
 
@@ -590,8 +625,7 @@ template <int NQ> static void batch_step(chz_ctx *c) {
     c->res[nxt].need(nr * E); c->pk[nxt].need(nr); c->g1[nxt].need(nr); c->g2[nxt].need(nr); c->finst[nxt].need(nr);
     uint32_t *tp = nullptr;
     uint8_t *tg = nullptr;
-    HIPCHK(hipMalloc((void **)&tp, nr * 4));
-    HIPCHK(hipMalloc((void **)&tg, nr));
+    tree_alloc(c, nr, tp, tg);
     c->tree_par.push_back(tp);
     c->tree_gate.push_back(tg);
     c->gstep = (int)c->tree_par.size() - 1;
@@ -625,50 +659,36 @@ template <int NQ> static void batch_step(chz_ctx *c) {
   c->cnt.need(C_N);
   HIPCHK(hipMemsetAsync(c->cnt.p, 0, C_N * 8, c->stream));
   const uint32_t *pinst = c->finst[cur].p;
-  long long nsurv = 0;
+  // survivors of every chunk append at the device-side count C_SURV: the buffers hold every child of the level
+  c->surv_o.need((size_t)m * G);
+  c->surv_key.need((size_t)m * G);
   for (long long off = 0; off < m; off += chunk) {
     const long long cm = std::min(chunk, m - off), ncc = cm * G;
+    // few parents: split each parent's candidates over several lane groups so the level fills the GPU
+    const int gsplit = (int)std::max<long long>(1, std::min<long long>(G, (c->tune_waves * CPW + cm - 1) / cm));
     ExpandArgs ea{c->res[cur].p, c->pk[cur].p, c->g1[cur].p, c->g2[cur].p, off, ncc, 0, 0,
-                  c->status.p, c->keys.p, c->info.p, c->cnt.p, pinst, c->d_rem.p, c->d_keepi.p, c->icnt.p};
+                  c->status.p, c->keys.p, c->info.p, c->cnt.p, pinst, c->d_rem.p, c->d_keepi.p, c->icnt.p, gsplit};
     c->ev.begin(K_EXPAND, c->stream);
-    hipLaunchKernelGGL(k_level<NQ>, dim3(grid_for((cm + CPW - 1) / CPW)), dim3(256), 0, c->stream, ks, ea, cm);
+    hipLaunchKernelGGL(k_level<NQ>, dim3(grid_for((cm * gsplit + CPW - 1) / CPW)), dim3(256), 0, c->stream, ks, ea, cm);
     c->ev.end(c->stream);
     const long long nb = (ncc + CT_TILE - 1) / CT_TILE;
     const FlagAlive fa{c->status.p, pinst, c->d_keepi.p, off, G};
     c->ev.begin(K_COMPACT, c->stream);
     hipLaunchKernelGGL(k_flag_count<FlagAlive>, dim3((unsigned)nb), dim3(CT_THREADS), 0, c->stream, fa, ncc, c->bcount.p);
     hipLaunchKernelGGL(k_scan_one_block, dim3(1), dim3(1024), 0, c->stream, c->bcount.p, nb);
-    c->ev.end(c->stream);
-    unsigned long long tot = 0;
-    HIPCHK(hipMemcpyAsync(&tot, c->bcount.p + nb, 8, hipMemcpyDeviceToHost, c->stream));
-    HIPCHK(hipStreamSynchronize(c->stream));
-    if (!tot) continue;
-    if ((size_t)(nsurv + tot) > c->surv_o.cap) {   // grow, keeping what earlier chunks wrote
-      DBuf<uint64_t> no, nk;
-      no.need(nsurv + tot);
-      nk.need(nsurv + tot);
-      if (nsurv) {
-        HIPCHK(hipMemcpyAsync(no.p, c->surv_o.p, nsurv * 8, hipMemcpyDeviceToDevice, c->stream));
-        HIPCHK(hipMemcpyAsync(nk.p, c->surv_key.p, nsurv * 8, hipMemcpyDeviceToDevice, c->stream));
-        HIPCHK(hipStreamSynchronize(c->stream));
-      }
-      c->surv_o.release();
-      c->surv_key.release();
-      c->surv_o = no;
-      c->surv_key = nk;
-    }
-    c->ev.begin(K_COMPACT, c->stream);
     hipLaunchKernelGGL(k_scatter_alive, dim3((unsigned)nb), dim3(CT_THREADS), 0, c->stream, fa, ncc, c->bcount.p,
-                       c->keys.p, off * G, nsurv, c->surv_o.p, c->surv_key.p);
+                       c->keys.p, off * G, c->cnt.p + C_SURV, c->surv_o.p, c->surv_key.p);
+    hipLaunchKernelGGL(k_add_total, dim3(1), dim3(64), 0, c->stream, c->cnt.p + C_SURV, c->bcount.p + nb);
     c->ev.end(c->stream);
-    c->k_work[K_COMPACT] += (double)ncc + (double)tot * 24.0;
-    nsurv += (long long)tot;
+    c->k_work[K_COMPACT] += (double)ncc;
   }
   unsigned long long hc[C_N];
   HIPCHK(hipMemcpyAsync(hc, c->cnt.p, C_N * 8, hipMemcpyDeviceToHost, c->stream));
   HIPCHK(hipMemcpyAsync(ic.data(), c->icnt.p, ic.size() * 8, hipMemcpyDeviceToHost, c->stream));
   HIPCHK(hipStreamSynchronize(c->stream));
   c->ev.flush(c->k_ms, c->k_launch);
+  const long long nsurv = (long long)hc[C_SURV];
+  c->k_work[K_COMPACT] += (double)nsurv * 24.0;
   c->k_work[K_EXPAND] += (double)hc[C_NEEDT] * t_ops16(NQ);
   // searches that end at this level with a circuit or an overflow; the rest continue if their budget allows
   for (const BInst &b : c->binst)
@@ -727,8 +747,7 @@ template <int NQ> static void batch_step(chz_ctx *c) {
     c->finst[nxt].need(mn);
     uint32_t *tp = nullptr;
     uint8_t *tg = nullptr;
-    HIPCHK(hipMalloc((void **)&tp, mn * 4));
-    HIPCHK(hipMalloc((void **)&tg, mn));
+    tree_alloc(c, mn, tp, tg);
     if (distinct) {
       const GatherArgs ga{c->kidx.p, distinct, c->surv_o.p, c->res[cur].p, c->pk[cur].p, c->g1[cur].p,
                           c->res[nxt].p, c->pk[nxt].p, c->g1[nxt].p, c->g2[nxt].p, tp, tg, pinst, c->finst[nxt].p};
@@ -737,10 +756,18 @@ template <int NQ> static void batch_step(chz_ctx *c) {
       c->ev.end(c->stream);
       c->k_work[K_GATHER] += (double)distinct * (E * 16.0 * 2.0 + 16.0);
     }
+    const bool single = c->pending.empty() && chz_batch_live(c) == 1;
     append_roots<NQ>(c, nxt, distinct, tp, tg);
     c->tree_par.push_back(tp);
     c->tree_gate.push_back(tg);
-    cnts = slot_counts(c, c->finst[nxt].p, mn);
+    if (single) {   // one search: its nodes are the whole frontier (its kernel times are read next step)
+      for (const BInst &b : c->binst) if (b.live) cnts[b.slot] = mn;
+    } else {
+      cnts = slot_counts(c, c->finst[nxt].p, mn);
+      c->ev.flush(c->k_ms, c->k_launch);
+    }
+  } else {
+    HIPCHK(hipStreamSynchronize(c->stream));
     c->ev.flush(c->k_ms, c->k_launch);
   }
   // events of this level, per search

@@ -99,3 +99,17 @@ def test_batch_rejects_other_gate_sets():
     assert b.add(read_spec("cs-d4"), 0) == 3          # two-qubit spec in a three-qubit batch
     assert b.add("cheese-spec 1\nbogus\n", 1) == 2
     b.close()
+
+
+def test_reused_context_repeats_its_log():
+    """a context reused for repeated searches (buffers, tree pool and instance slots persist) gives the golden
+    log every time"""
+    ref = [l for l in strip_seconds(read_log("toffoli-d9")) if "prefixes" in l]
+    s = cheese_amd.Searcher(read_spec("toffoli-d9"))
+    try:
+        for _ in range(3):
+            out = s.run()
+            assert out["rc"] == 1
+            assert strip_seconds(_levels_text(out["levels"])) == ref
+    finally:
+        s.close()

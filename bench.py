@@ -195,6 +195,8 @@ def main():
                     help="also build the 3-qubit Clifford table and answer 1e5 queries")
     ap.add_argument("--tcount", action=argparse.BooleanOptionalAction, default=True,
                     help="also build the 3-qubit T-count table and search 256 batched targets")
+    ap.add_argument("--tune", action=argparse.BooleanOptionalAction, default=True,
+                    help="autotune the expansion schedule on the workload before the timed proofs")
     ap.add_argument("--serve", type=int, default=400, metavar="N",
                     help="also serve a mixed stream of N requests, plain and batched (0: skip)")
     a = ap.parse_args()
@@ -212,6 +214,7 @@ def main():
     # one context for every proof (spec parsed and uploaded once, device buffers kept), as a long-running caller
     # holds it; the timed region is the searches themselves
     searcher = cheese_amd.Searcher(spec)
+    tuned = searcher.tune(2) if a.tune else None
     for _ in range(a.warmup):
         searcher.run()
     times, prefixes, kstats = [], 0, {}
@@ -257,6 +260,7 @@ def main():
                      "launches_per_step": tb["launches"] / a.steps,
                      "work": "16-bit integer ops of the T-bound transforms only (cheap bounds not counted)"},
         "kernels_ms_per_step": {k: round(v["ms"] / a.steps, 3) for k, v in kstats.items()},
+        "autotune": {b: [v["waves_per_eu"], v["grid_cap"], v["split_waves"]] for b, v in tuned.items()} if tuned else None,
     }
     if a.random:
         line["random32"] = random_circuits()

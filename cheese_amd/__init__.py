@@ -61,6 +61,7 @@ SIGNATURES = {
     "chz_verify": (ctypes.c_int, [ctypes.c_void_p, ctypes.POINTER(ctypes.c_int), ctypes.c_int]),
     "chz_last_error": (ctypes.c_char_p, []),
     "chz_verify_spec": (ctypes.c_int, [ctypes.c_char_p, ctypes.c_size_t, ctypes.POINTER(ctypes.c_int), ctypes.c_int]),
+    "chz_tune": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int, ctypes.c_char_p, ctypes.c_size_t]),
     "chz_batch_add": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_char_p, ctypes.c_size_t, ctypes.c_int, ctypes.c_int]),
     "chz_batch_step": (ctypes.c_int, [ctypes.c_void_p]),
     "chz_batch_events": (ctypes.c_int, [ctypes.c_void_p, ctypes.POINTER(Event), ctypes.c_int]),
@@ -152,6 +153,14 @@ class Searcher:
         rc = L.chz_open(raw, len(raw), ctypes.byref(o), ctypes.byref(self._ctx))
         if rc:
             raise SearchError(rc, L.chz_last_error().decode())
+
+    def tune(self, reps: int = 2) -> dict:
+        """chz_tune: the fastest expansion schedule per level-size bucket, kept by this context"""
+        buf = ctypes.create_string_buffer(1 << 14)
+        rc = lib().chz_tune(self._ctx, reps, buf, len(buf))
+        if rc:
+            raise SearchError(rc, lib().chz_last_error().decode())
+        return json.loads(buf.value.decode())
 
     def run(self, stats: bool = False):
         L = lib()

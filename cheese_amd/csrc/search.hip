@@ -52,9 +52,9 @@ struct ExpandArgs {
 struct GateHdr { int nq, L, mask, ik, q0, q1, q2, kind; };   // kind: 0 general, 1 unit entries, 2 unit permutation
 constexpr int MAXG = 64;
 
-template <int NQ>
-__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) void k_level(KSpec sp, ExpandArgs a,
-                                                                                       long long np) {
+template <int NQ, int WPE>
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(WPE))) void k_level(KSpec sp, ExpandArgs a,
+                                                                                         long long np) {
   constexpr int N = 1 << NQ, E = N * N, GS = E, CPW = 64 / GS;
   __shared__ TStage stage[256];
   __shared__ int4 spar[256];                   // each group's parent residual
@@ -462,7 +462,13 @@ struct chz_ctx {
   chz::DBuf<uint8_t> d_keepi, d_active;
   chz::DBuf<unsigned long long> icnt, scan_tmp;
   chz::DBuf<long long> d_lo, d_hi;
-  long long tune_waves = 4096;     // waves a level should offer before its parents' candidates are split
+  // expansion schedule per level-size bucket (log2 of the chunk's parents): occupancy variant, grid cap in
+  // blocks, candidate-split threshold in waves; chosen by chz_tune, defaults otherwise
+  struct Sched { int var = 1, cap = 8192, tw = 4096; };
+  Sched sched[32];
+  bool tuning = false;
+  Sched force;
+  std::vector<std::pair<int, std::pair<hipEvent_t, hipEvent_t>>> tune_ev;   // (bucket, events) per launch
   std::map<void *, size_t> tree_blocks;   // every tree block allocated, with its size
   std::vector<void *> tree_free;          // blocks no live search reaches, ready for reuse
   // measurement
@@ -664,13 +670,31 @@ template <int NQ> static void batch_step(chz_ctx *c) {
   c->surv_key.need((size_t)m * G);
   for (long long off = 0; off < m; off += chunk) {
     const long long cm = std::min(chunk, m - off), ncc = cm * G;
-    // few parents: split each parent's candidates over several lane groups so the level fills the GPU
-    const int gsplit = (int)std::max<long long>(1, std::min<long long>(G, (c->tune_waves * CPW + cm - 1) / cm));
+    // the schedule of this chunk's size bucket: few parents split each parent's candidates over several lane
+    // groups so the level fills the GPU
+    const int bucket = std::min(31, 63 - __builtin_clzll((unsigned long long)cm));
+    const chz_ctx::Sched sc = c->tuning ? c->force : c->sched[bucket];
+    const int gsplit = (int)std::max<long long>(1, std::min<long long>(G, ((long long)sc.tw * CPW + cm - 1) / cm));
     ExpandArgs ea{c->res[cur].p, c->pk[cur].p, c->g1[cur].p, c->g2[cur].p, off, ncc, 0, 0,
                   c->status.p, c->keys.p, c->info.p, c->cnt.p, pinst, c->d_rem.p, c->d_keepi.p, c->icnt.p, gsplit};
+    const dim3 grid(grid_for((cm * gsplit + CPW - 1) / CPW, sc.cap));
+    hipEvent_t te0 = nullptr, te1 = nullptr;
+    if (c->tuning) {
+      te0 = c->ev.take();
+      te1 = c->ev.take();
+      HIPCHK(hipEventRecord(te0, c->stream));
+    }
     c->ev.begin(K_EXPAND, c->stream);
-    hipLaunchKernelGGL(k_level<NQ>, dim3(grid_for((cm * gsplit + CPW - 1) / CPW)), dim3(256), 0, c->stream, ks, ea, cm);
+    switch (sc.var) {
+      case 0: hipLaunchKernelGGL((k_level<NQ, 2>), grid, dim3(256), 0, c->stream, ks, ea, cm); break;
+      case 2: hipLaunchKernelGGL((k_level<NQ, 4>), grid, dim3(256), 0, c->stream, ks, ea, cm); break;
+      default: hipLaunchKernelGGL((k_level<NQ, 3>), grid, dim3(256), 0, c->stream, ks, ea, cm); break;
+    }
     c->ev.end(c->stream);
+    if (c->tuning) {
+      HIPCHK(hipEventRecord(te1, c->stream));
+      c->tune_ev.push_back({bucket, {te0, te1}});
+    }
     const long long nb = (ncc + CT_TILE - 1) / CT_TILE;
     const FlagAlive fa{c->status.p, pinst, c->d_keepi.p, off, G};
     c->ev.begin(K_COMPACT, c->stream);
@@ -993,6 +1017,61 @@ int chz_run(chz_ctx *c, chz_level_cb cb, void *user, chz_result *out) {
     if (done) return res->rc;
     if (const int rc = chz_batch_step(c)) return res->rc = rc;
   }
+}
+
+// the expansion schedule per level-size bucket, chosen by timing every candidate schedule on this context's own
+// search (reps runs each); the fastest mean launch time per bucket wins.  Per-level results do not depend on
+// the schedule.  out (may be NULL) receives the chosen table as JSON text.
+int chz_tune(chz_ctx *c, int reps, char *out, size_t cap) {
+  if (chz_batch_live(c)) { chz::set_err("the context's batch holds searches"); return 3; }
+  const int vars[3] = {0, 1, 2}, caps[3] = {4096, 8192, 16384}, tws[3] = {2048, 4096, 8192};
+  std::vector<chz_ctx::Sched> cands;
+  for (int v : vars) for (int cp : caps) for (int tw : tws) { chz_ctx::Sched s; s.var = v; s.cap = cp; s.tw = tw; cands.push_back(s); }
+  std::vector<std::vector<double>> sum(cands.size(), std::vector<double>(32, 0.0));
+  std::vector<std::vector<int>> num(cands.size(), std::vector<int>(32, 0));
+  try {
+    HIPCHK(hipSetDevice(c->device));
+    for (size_t k = 0; k < cands.size(); k++) {
+      c->tuning = true;
+      c->force = cands[k];
+      for (int r = 0; r < std::max(1, reps); r++) {
+        chz_result res{};
+        const int rc = chz_run(c, nullptr, nullptr, &res);
+        if (rc >= 2) { c->tuning = false; return rc; }
+        HIPCHK(hipStreamSynchronize(c->stream));
+        for (auto &te : c->tune_ev) {
+          float ms = 0;
+          HIPCHK(hipEventElapsedTime(&ms, te.second.first, te.second.second));
+          sum[k][te.first] += ms;
+          num[k][te.first]++;
+          c->ev.pool.push_back(te.second.first);
+          c->ev.pool.push_back(te.second.second);
+        }
+        c->tune_ev.clear();
+      }
+    }
+  } catch (chz::DeviceError &) {
+    c->tuning = false;
+    return 5;
+  }
+  c->tuning = false;
+  std::string js = "{";
+  bool first = true;
+  for (int b = 0; b < 32; b++) {
+    int best = -1;
+    for (size_t k = 0; k < cands.size(); k++)
+      if (num[k][b] && (best < 0 || sum[k][b] / num[k][b] < sum[best][b] / num[best][b])) best = (int)k;
+    if (best < 0) continue;
+    c->sched[b] = cands[best];
+    char buf[128];
+    std::snprintf(buf, sizeof buf, "%s\"%d\": {\"waves_per_eu\": %d, \"grid_cap\": %d, \"split_waves\": %d, \"ms\": %.4f}",
+                  first ? "" : ", ", b, cands[best].var + 2, cands[best].cap, cands[best].tw, sum[best][b] / num[best][b]);
+    js += buf;
+    first = false;
+  }
+  js += "}";
+  if (out && cap) { std::strncpy(out, js.c_str(), cap - 1); out[cap - 1] = 0; }
+  return 0;
 }
 
 int chz_batch_add(chz_ctx *c, const char *spec_text, size_t len, int depth, int tag) {

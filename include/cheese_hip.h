@@ -82,6 +82,12 @@ typedef struct chz_kstat {
 } chz_kstat;
 int chz_kernel_stats(const chz_ctx *ctx, chz_kstat *out, int n);
 
+/* Autotuner (no reference counterpart): times every candidate expansion schedule (occupancy variant x grid cap x
+ * candidate-split threshold) on this context's own search, reps runs each, and keeps the fastest per level-size
+ * bucket (log2 of the parents in a launch); later runs on the context use that table.  Results do not depend on
+ * the schedule.  out (may be NULL) receives the table as JSON.  0, 3 (batch busy), 4/5 as chz_run. */
+int chz_tune(chz_ctx *ctx, int reps, char *out, size_t cap);
+
 /* Exact check of a gate list against the spec's target (verify.py:20-33 restated): 1 equal, 0 not, <0 error. */
 int chz_verify(const chz_ctx *ctx, const int *circuit, int len);
 /* the same check against the target of a .spec text, on the host only (no device work): 1, 0, <0 error */

@@ -113,3 +113,16 @@ def test_reused_context_repeats_its_log():
             assert strip_seconds(_levels_text(out["levels"])) == ref
     finally:
         s.close()
+
+
+def test_autotuned_schedule_keeps_the_log():
+    """chz_tune tries every expansion schedule on the context's own search; the chosen table changes timing only"""
+    ref = [l for l in strip_seconds(read_log("toffoli-d9")) if "prefixes" in l]
+    s = cheese_amd.Searcher(read_spec("toffoli-d9"))
+    try:
+        table = s.tune(1)
+        assert table and all(v["waves_per_eu"] in (2, 3, 4) for v in table.values())
+        out = s.run()
+        assert strip_seconds(_levels_text(out["levels"])) == ref
+    finally:
+        s.close()

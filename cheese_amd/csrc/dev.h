@@ -14,6 +14,8 @@ struct DevGate {                               // S <- S g^-1 data of one candid
   int nq, L, mask, ik;
   int q[3];
   int unit;                                    // every nonzero inverse entry is +-w^p
+  int cliff;                                   // Clifford: children keep the parent's T bound
+  int cutloc;                                  // bit i: the gate acts on one side of cut i only
   int4 inv[64];                                // L x L inverse local matrix, row m, column lc
   int8_t ph[64];                               // unit gates: w-power p of inv[m][lc], -1 for zero
 };

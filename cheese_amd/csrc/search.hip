@@ -36,7 +36,8 @@ struct ExpandArgs {
   int t, keep;
   uint8_t *status;          // [ncc]
   uint64_t *keys;           // [ncc] class keys of kept children
-  uint32_t *info;           // [ncc] packed cheap bounds of children that need the T bound
+  uint32_t *info;           // [ncc] bound words of kept children
+  const uint32_t *pb;       // parents' bound words
   unsigned long long *cnt;  // global counters (C_EVAL, C_NEEDT)
   const uint32_t *pinst;    // instance slot of every parent of the level (null: one instance, slot 0)
   const int *rem;           // per slot: gate budget left after this level's gate (depth - t)
@@ -49,7 +50,8 @@ struct ExpandArgs {
 // residual in LDS and walks all G candidates: windows, child residual, overflow, goal, cheap bounds, T bound
 // (mod 2^16, one lane per flip pair, when n + k <= 16; else mod 2^32 / 2^64, one lane per entry) and the
 // class key of a kept child.  Writes the status byte (and key) of every child of the chunk.
-struct GateHdr { int nq, L, mask, ik, q0, q1, q2, kind; };   // kind: 0 general, 1 unit entries, 2 unit permutation
+// kind: 0 general, 1 unit entries, 2 unit permutation; cliff, loc: DevGate::cliff, DevGate::cutloc
+struct GateHdr { int nq, L, mask, ik, q0, q1, q2, kind, cliff, loc; };
 constexpr int MAXG = 64;
 
 template <int NQ, int WPE>
@@ -72,7 +74,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(WPE))) void
         if (nz != 1) kind = 1;
       }
     }
-    shdr[g] = GateHdr{d.nq, d.L, d.mask, d.ik, d.q[0], d.q[1], d.q[2], kind};
+    shdr[g] = GateHdr{d.nq, d.L, d.mask, d.ik, d.q[0], d.q[1], d.q[2], kind, d.cliff, d.cutloc};
   }
   __syncthreads();
   const int lane = threadIdx.x & 63, le = lane & (GS - 1), slot = lane / GS;
@@ -111,6 +113,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(WPE))) void
     const bool keep = a.keepi[inst] != 0;
     const int4 S = a.res[pi * E + le];
     const int pk = a.pk[pi], g1 = a.g1[pi], g2 = a.g2[pi];
+    const uint32_t pbw = a.pb[pi];
     const bool small = gall<GS>(zmag(S) < (1u << 28));
     __builtin_amdgcn_wave_barrier();
     P[le] = S;
@@ -124,6 +127,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(WPE))) void
         if ((sp.bad3[tri >> 5] >> (tri & 31)) & 1) stt = ST_W3;
       }
       uint64_t key = 0;
+      uint32_t bw = BW_TUNK;
       if (stt == 255) {
         // child residual entry (r, c) = sum_m P[r][x_m] ginv[m][lc] (src/cheese.cu:146-174)
         const GateHdr h = shdr[g];
@@ -180,11 +184,16 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(WPE))) void
                 const int hs = h_struct<NQ>(s);
                 hh = k > hs ? k : hs;
                 supp = support<NQ>(s, le);
-                c2 = cx_bound<NQ>(s, le, sp);
+                uint32_t needs = (pbw & ~BW_CUTS) >> 8;
+                c2 = cx_bound<NQ>(s, le, sp, (pbw & BW_CUTS) ? (uint32_t)h.loc : 0u, needs);
+                bw = (needs << 8 & ~BW_CUTS) | BW_CUTS | BW_TUNK;
               }
-              if (combine_bound(hh, 0, c2, supp) > rem) alive = false;
-              else {
-                int T;
+              // a Clifford gate keeps the parent's T bound; the bound combination is monotone in T, so a
+              // known T settles the child at once
+              const bool tknown = h.cliff && (pbw & BW_TUNK) != BW_TUNK;
+              int T = tknown ? (int)(pbw & BW_TUNK) : 0;
+              if (combine_bound(hh, (T + sp.tdiv - 1) / sp.tdiv, c2, supp) > rem) alive = false;
+              else if (!tknown) {
                 n_t += le == 0;
                 if (NQ + k <= 16) {
                   __builtin_amdgcn_wave_barrier();
@@ -195,9 +204,9 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(WPE))) void
                 } else {
                   T = t_bound_any<NQ>(s, k, le);
                 }
-                T = (T + sp.tdiv - 1) / sp.tdiv;
-                alive = combine_bound(hh, T, c2, supp) <= rem;
+                alive = combine_bound(hh, (T + sp.tdiv - 1) / sp.tdiv, c2, supp) <= rem;
               }
+              if (alive && T < (int)BW_TUNK) bw = (bw & ~BW_TUNK) | (uint32_t)T;
             }
             if (alive && keep) key = class_key<NQ>(s, k, le, sp) ^ ((uint64_t)inst * 0x9e3779b97f4a7c15ull);
             stt = alive ? ST_ALIVE : ST_DEAD;
@@ -211,7 +220,10 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(WPE))) void
           case ST_W3: n_w3++; break;
           case ST_DEAD: n_dead++; break;
           case ST_OVF: n_ovf++; break;
-          case ST_ALIVE: n_alive++; if (keep) a.keys[cc] = key; break;
+          case ST_ALIVE:
+            n_alive++;
+            if (keep) { a.keys[cc] = key; a.info[cc] = bw; }
+            break;
           case ST_FOUND:
             n_found++;
             atomicMin(&a.icnt[(size_t)inst * IC_N + IC_FIRST], (unsigned long long)(a.off * G + cc));
@@ -259,7 +271,8 @@ struct FlagNeed {       // children that need the T bound, in child order
 __global__ __launch_bounds__(CT_THREADS) void k_scatter_alive(FlagAlive f, long long n, const unsigned long long *boff,
                                                               const uint64_t *keys, long long child0,
                                                               const unsigned long long *out0, uint64_t *surv_o,
-                                                              uint64_t *surv_key) {
+                                                              uint64_t *surv_key, const uint32_t *info,
+                                                              uint32_t *surv_bw) {
   __shared__ int ws[CT_THREADS / 64];
   const long long t0 = (long long)blockIdx.x * CT_TILE;
   long long pos = (long long)*out0 + (long long)boff[blockIdx.x];
@@ -268,7 +281,11 @@ __global__ __launch_bounds__(CT_THREADS) void k_scatter_alive(FlagAlive f, long 
     const bool fl = x < n && f(x);
     int ex;
     const int tot = block_flag_scan(fl, ex, ws);
-    if (fl) { surv_o[pos + ex] = (uint64_t)(child0 + x); surv_key[pos + ex] = keys[x]; }
+    if (fl) {
+      surv_o[pos + ex] = (uint64_t)(child0 + x);
+      surv_key[pos + ex] = keys[x];
+      surv_bw[pos + ex] = info[x];
+    }
     pos += tot;
   }
 }
@@ -364,6 +381,8 @@ struct GatherArgs {
   uint8_t *tree_gate;
   const uint32_t *pinst;          // parents' instance slots (null: one instance)
   uint32_t *inst_next;
+  const uint32_t *surv_bw;        // bound words of the survivors
+  uint32_t *pb_next;
 };
 
 // next frontier: tree node, window state and materialised residual of every kept child
@@ -390,6 +409,7 @@ __global__ __launch_bounds__(256) void k_gather(KSpec sp, GatherArgs a) {
       a.tree_par[p] = (uint32_t)pi;
       a.tree_gate[p] = (uint8_t)g;
       a.inst_next[p] = a.pinst ? a.pinst[pi] : 0u;
+      a.pb_next[p] = a.surv_bw[a.kidx[p]];
     }
   }
 }
@@ -443,12 +463,13 @@ struct chz_ctx {
   chz::DBuf<int4> res[2];
   chz::DBuf<int> pk[2];
   chz::DBuf<uint8_t> g1[2], g2[2];
+  chz::DBuf<uint32_t> pb[2];        // bound words (T bound, cut needs) of the frontier's nodes
   std::vector<uint32_t *> tree_par;
   std::vector<uint8_t *> tree_gate;
   // per-level work arrays
   chz::DBuf<uint8_t> status, kept;
   chz::DBuf<uint64_t> keys, surv_o, surv_key;
-  chz::DBuf<uint32_t> info;
+  chz::DBuf<uint32_t> info, surv_bw;
   chz::DBuf<unsigned long long> cnt, bcount, hist;
   chz::DBuf<uint32_t> sk[2], sv[2], runs, kidx;
   // batch state: searches, frontier instance slots, per-slot device arrays, queued events
@@ -571,7 +592,7 @@ template <int NQ> static void append_roots(chz_ctx *c, int nxt, long long distin
   std::vector<int4> res(nr * E);
   std::vector<int> pk(nr);
   std::vector<uint8_t> none(nr, 255);
-  std::vector<uint32_t> slot(nr), np(nr, NO_PARENT);
+  std::vector<uint32_t> slot(nr), np(nr, NO_PARENT), bw(nr, BW_TUNK);
   for (long long i = 0; i < nr; i++) {
     const Pending &q = c->pending[i];
     for (int e = 0; e < E; e++) res[i * E + e] = q.U[e];
@@ -583,6 +604,7 @@ template <int NQ> static void append_roots(chz_ctx *c, int nxt, long long distin
   HIPCHK(hipMemcpyAsync(c->g1[nxt].p + distinct, none.data(), nr, hipMemcpyHostToDevice, c->stream));
   HIPCHK(hipMemcpyAsync(c->g2[nxt].p + distinct, none.data(), nr, hipMemcpyHostToDevice, c->stream));
   HIPCHK(hipMemcpyAsync(c->finst[nxt].p + distinct, slot.data(), nr * 4, hipMemcpyHostToDevice, c->stream));
+  HIPCHK(hipMemcpyAsync(c->pb[nxt].p + distinct, bw.data(), nr * 4, hipMemcpyHostToDevice, c->stream));
   HIPCHK(hipMemcpyAsync(tp + distinct, np.data(), nr * 4, hipMemcpyHostToDevice, c->stream));
   HIPCHK(hipMemcpyAsync(tg + distinct, none.data(), nr, hipMemcpyHostToDevice, c->stream));
   HIPCHK(hipStreamSynchronize(c->stream));
@@ -629,6 +651,7 @@ template <int NQ> static void batch_step(chz_ctx *c) {
     const long long nr = (long long)c->pending.size();
     const int nxt = c->cur ^ 1;
     c->res[nxt].need(nr * E); c->pk[nxt].need(nr); c->g1[nxt].need(nr); c->g2[nxt].need(nr); c->finst[nxt].need(nr);
+    c->pb[nxt].need(nr);
     uint32_t *tp = nullptr;
     uint8_t *tg = nullptr;
     tree_alloc(c, nr, tp, tg);
@@ -668,6 +691,7 @@ template <int NQ> static void batch_step(chz_ctx *c) {
   // survivors of every chunk append at the device-side count C_SURV: the buffers hold every child of the level
   c->surv_o.need((size_t)m * G);
   c->surv_key.need((size_t)m * G);
+  c->surv_bw.need((size_t)m * G);
   for (long long off = 0; off < m; off += chunk) {
     const long long cm = std::min(chunk, m - off), ncc = cm * G;
     // the schedule of this chunk's size bucket: few parents split each parent's candidates over several lane
@@ -676,7 +700,8 @@ template <int NQ> static void batch_step(chz_ctx *c) {
     const chz_ctx::Sched sc = c->tuning ? c->force : c->sched[bucket];
     const int gsplit = (int)std::max<long long>(1, std::min<long long>(G, ((long long)sc.tw * CPW + cm - 1) / cm));
     ExpandArgs ea{c->res[cur].p, c->pk[cur].p, c->g1[cur].p, c->g2[cur].p, off, ncc, 0, 0,
-                  c->status.p, c->keys.p, c->info.p, c->cnt.p, pinst, c->d_rem.p, c->d_keepi.p, c->icnt.p, gsplit};
+                  c->status.p, c->keys.p, c->info.p, c->pb[cur].p, c->cnt.p, pinst, c->d_rem.p, c->d_keepi.p,
+                  c->icnt.p, gsplit};
     const dim3 grid(grid_for((cm * gsplit + CPW - 1) / CPW, sc.cap));
     hipEvent_t te0 = nullptr, te1 = nullptr;
     if (c->tuning) {
@@ -701,7 +726,7 @@ template <int NQ> static void batch_step(chz_ctx *c) {
     hipLaunchKernelGGL(k_flag_count<FlagAlive>, dim3((unsigned)nb), dim3(CT_THREADS), 0, c->stream, fa, ncc, c->bcount.p);
     hipLaunchKernelGGL(k_scan_one_block, dim3(1), dim3(1024), 0, c->stream, c->bcount.p, nb);
     hipLaunchKernelGGL(k_scatter_alive, dim3((unsigned)nb), dim3(CT_THREADS), 0, c->stream, fa, ncc, c->bcount.p,
-                       c->keys.p, off * G, c->cnt.p + C_SURV, c->surv_o.p, c->surv_key.p);
+                       c->keys.p, off * G, c->cnt.p + C_SURV, c->surv_o.p, c->surv_key.p, c->info.p, c->surv_bw.p);
     hipLaunchKernelGGL(k_add_total, dim3(1), dim3(64), 0, c->stream, c->cnt.p + C_SURV, c->bcount.p + nb);
     c->ev.end(c->stream);
     c->k_work[K_COMPACT] += (double)ncc;
@@ -768,13 +793,15 @@ template <int NQ> static void batch_step(chz_ctx *c) {
   std::vector<long long> cnts(c->nslots, 0);
   if (mn) {
     c->res[nxt].need((size_t)mn * E); c->pk[nxt].need(mn); c->g1[nxt].need(mn); c->g2[nxt].need(mn);
+    c->pb[nxt].need(mn);
     c->finst[nxt].need(mn);
     uint32_t *tp = nullptr;
     uint8_t *tg = nullptr;
     tree_alloc(c, mn, tp, tg);
     if (distinct) {
       const GatherArgs ga{c->kidx.p, distinct, c->surv_o.p, c->res[cur].p, c->pk[cur].p, c->g1[cur].p,
-                          c->res[nxt].p, c->pk[nxt].p, c->g1[nxt].p, c->g2[nxt].p, tp, tg, pinst, c->finst[nxt].p};
+                          c->res[nxt].p, c->pk[nxt].p, c->g1[nxt].p, c->g2[nxt].p, tp, tg, pinst, c->finst[nxt].p,
+                          c->surv_bw.p, c->pb[nxt].p};
       c->ev.begin(K_GATHER, c->stream);
       hipLaunchKernelGGL(k_gather<NQ>, dim3(grid_for((distinct + CPW - 1) / CPW)), dim3(256), 0, c->stream, ks, ga);
       c->ev.end(c->stream);
@@ -875,6 +902,15 @@ static void upload_spec(chz_ctx *c) {
       if (nz != 1 || (cf[idx] != 1 && cf[idx] != -1)) { d.unit = 0; d.ph[i] = -1; continue; }
       d.ph[i] = (int8_t)(idx + (cf[idx] < 0 ? 4 : 0));
     }
+    // a Clifford gate's Pauli transfer matrix is a signed permutation, so a child's T bound (the smallest
+    // sqrt2-valuation over the transfer matrix of the residual) is its parent's; a gate on one side of a cut
+    // leaves that cut's operator-Schmidt rank unchanged.  Packed cut needs take 3 bits each in the node's
+    // bound word, so at most BW_MAXCUTS cuts are carried.
+    d.cliff = gate_is_clifford(h) ? 1 : 0;
+    d.cutloc = 0;
+    if ((int)s.cutmask.size() <= BW_MAXCUTS)
+      for (size_t i = 0; i < s.cutmask.size(); i++)
+        if ((d.mask & s.cutmask[i]) == 0 || (d.mask & ~s.cutmask[i]) == 0) d.cutloc |= 1 << i;
   }
   std::vector<int4> goal(N * N);
   for (int i = 0; i < N * N; i++) goal[i] = make_int4((int)s.goal[i].a, (int)s.goal[i].b, (int)s.goal[i].c, (int)s.goal[i].d);
@@ -1212,11 +1248,13 @@ void chz_close(chz_ctx *c) {
   chz::free_tree(c);
   for (int b = 0; b < 2; b++) {
     c->res[b].release(); c->pk[b].release(); c->g1[b].release(); c->g2[b].release();
+    c->pb[b].release();
     c->sk[b].release(); c->sv[b].release();
   }
   c->finst[0].release(); c->finst[1].release(); c->d_rem.release(); c->d_keepi.release(); c->d_active.release();
   c->icnt.release(); c->scan_tmp.release(); c->d_lo.release(); c->d_hi.release();
   c->status.release(); c->kept.release(); c->keys.release(); c->surv_o.release(); c->surv_key.release();
+  c->surv_bw.release();
   c->info.release(); c->cnt.release(); c->bcount.release(); c->hist.release(); c->runs.release(); c->kidx.release();
   void *ptrs[] = {c->d_gates, c->d_bad2, c->d_symperm, c->d_symconj, c->d_bad3, c->d_goal, c->d_cx};
   for (void *p : ptrs) if (p) (void)hipFree(p);

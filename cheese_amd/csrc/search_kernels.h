@@ -156,23 +156,41 @@ template <int NQ> __device__ __forceinline__ int cut_rank(uint32_t fp, int le, i
   return rank;
 }
 
-// CX/CZ lower bound from the cut ranks and the ILP table (src/cheese.cu:585-597)
-template <int NQ> __device__ __forceinline__ int cx_bound(int4 s, int le, const KSpec &sp) {
-  const uint32_t fp = fp_of(s, sp.wm);
+// Bound word carried by every frontier node: bits 0-7 its raw T bound (BW_TUNK: not known), bits 8 + 3 i the
+// need of cut i (ceil log2 of its clipped rank, at most 2 for n <= 3), bit 31 set when the cut needs are known.
+// Roots carry BW_TUNK.  A child copies what its gate cannot change and computes the rest.
+constexpr uint32_t BW_TUNK = 0xffu, BW_CUTS = 1u << 31;
+constexpr int BW_MAXCUTS = 7;
+
+// CX/CZ lower bound from the cut ranks and the ILP table (src/cheese.cu:585-597).  Cuts in `keepc` take the
+// need packed in `needs` (the parent's: the gate acts on one side of them, so the rank is unchanged); the
+// others are ranked here.  On return `needs` holds this matrix's packed needs.
+template <int NQ> __device__ __forceinline__ int cx_bound(int4 s, int le, const KSpec &sp, uint32_t keepc,
+                                                          uint32_t &needs) {
+  uint32_t fp = 0;
+  bool have_fp = false;
   int idx = 0, mult = 1, mx = 0;
+  uint32_t out = 0;
   for (int i = 0; i < sp.ncuts; i++) {
     const int rad = sp.rad[i], sa = sp.cutsize[i];
-    const int R = 1 << (2 * sa);
-    int stop = R;
-    if (rad >= 2 && rad - 2 < 30 && (1 << (rad - 2)) + 1 < stop) stop = (1 << (rad - 2)) + 1;
-    const int rk = cut_rank<NQ>(fp, le, sp.cutmask[i], sa, stop);
-    int need = 0;
-    while ((1 << need) < rk) need++;
-    if (need > rad - 1) need = rad - 1;
+    int need;
+    if ((keepc >> i) & 1) need = (int)((needs >> (3 * i)) & 7u);
+    else {
+      if (!have_fp) { fp = fp_of(s, sp.wm); have_fp = true; }
+      const int R = 1 << (2 * sa);
+      int stop = R;
+      if (rad >= 2 && rad - 2 < 30 && (1 << (rad - 2)) + 1 < stop) stop = (1 << (rad - 2)) + 1;
+      const int rk = cut_rank<NQ>(fp, le, sp.cutmask[i], sa, stop);
+      need = 0;
+      while ((1 << need) < rk) need++;
+      if (need > rad - 1) need = rad - 1;
+    }
+    if (i < BW_MAXCUTS) out |= (uint32_t)need << (3 * i);
     idx += need * mult;
     mult *= rad;
     if (need > mx) mx = need;
   }
+  needs = out;
   return sp.have_table ? sp.cxtable[idx] : mx;
 }
 

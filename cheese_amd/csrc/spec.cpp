@@ -221,4 +221,49 @@ bool same_gate_set(const HostSpec &a, const HostSpec &b) {
   return true;
 }
 
+// A gate is Clifford when it maps every Pauli operator to a multiple of one Pauli operator under conjugation:
+// for each Pauli B = X^x Z^z, C = g B g^dagger has a nonzero overlap Tr(A^dagger C) with exactly one Pauli A
+// (the Paulis are an orthogonal basis).  Exact in Z[w]; the scale sqrt2^(2 gk) of g g^dagger cancels.
+static Zq zq_conj(const Zq &x) { return {x.a, -x.d, -x.c, -x.b}; }
+bool gate_is_clifford(const HostGate &g) {
+  const int L = 1 << g.nq;
+  const std::vector<Zq> &M = g.fwd;
+  if ((int)M.size() != L * L) return false;
+  std::vector<Zq> MB(L * L), C(L * L);
+  for (int bx = 0; bx < L; bx++)
+    for (int bz = 0; bz < L; bz++) {
+      // (M B)[r][c] = M[r][c ^ bx] (-1)^(bz . c): B |c> = (-1)^(bz . c) |c ^ bx>
+      for (int r = 0; r < L; r++)
+        for (int c = 0; c < L; c++) {
+          Zq v = M[r * L + (c ^ bx)];
+          if (__builtin_popcount(bz & c) & 1) v = {-v.a, -v.b, -v.c, -v.d};
+          MB[r * L + c] = v;
+        }
+      // C = (M B) M^dagger
+      for (int r = 0; r < L; r++)
+        for (int c = 0; c < L; c++) {
+          Zq acc{0, 0, 0, 0};
+          for (int j = 0; j < L; j++) {
+            const Zq t = zq_mul(MB[r * L + j], zq_conj(M[c * L + j]));
+            acc = {acc.a + t.a, acc.b + t.b, acc.c + t.c, acc.d + t.d};
+          }
+          C[r * L + c] = acc;
+        }
+      int hits = 0;
+      for (int ax = 0; ax < L; ax++)
+        for (int az = 0; az < L; az++) {
+          // Tr(A^dagger C) = sum_c <c ^ ax| C |c> (-1)^(az . c)
+          Zq tr{0, 0, 0, 0};
+          for (int c = 0; c < L; c++) {
+            Zq v = C[(c ^ ax) * L + c];
+            if (__builtin_popcount(az & c) & 1) v = {-v.a, -v.b, -v.c, -v.d};
+            tr = {tr.a + v.a, tr.b + v.b, tr.c + v.c, tr.d + v.d};
+          }
+          hits += !zq_zero(tr);
+        }
+      if (hits != 1) return false;
+    }
+  return true;
+}
+
 }  // namespace chz

@@ -56,4 +56,7 @@ bool host_equal(const HostMat &S, const std::vector<Zq> &M, int k);
 // as one batch)
 bool same_gate_set(const HostSpec &a, const HostSpec &b);
 
+// exact: the gate maps every Pauli operator to a multiple of a Pauli operator under conjugation
+bool gate_is_clifford(const HostGate &g);
+
 }  // namespace chz

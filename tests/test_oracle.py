@@ -166,3 +166,36 @@ def test_random_circuits_are_seeded():
             circ = workloads.random_circuit(m["seed"], 3, 10, 13)
             assert [[g, list(q)] for g, q in circ] == m["circuit"]
             assert m["depth"] == len(circ)
+
+
+def test_invariances_the_expansion_carries(oracle_lib):
+    """A child keeps its parent's T bound when its gate is Clifford (the Pauli transfer matrix of a Clifford gate
+    is a signed permutation), and keeps the F_p rank of every cut its gate does not straddle (a local invertible
+    factor leaves an operator-Schmidt rank unchanged).  T and T-dagger children do change the T bound."""
+    L = oracle_lib
+    spec = read_spec("toffoli-d9")
+    names = [l.split()[0] for l in spec.splitlines() if l.startswith("L_")]
+    qubits = [[int(q) for q in nm.split("_")[2:]] for nm in names]
+    clifford = [nm.split("_")[1] not in ("t", "tdg") for nm in names]
+    G = len(names)
+    cuts = [1, 2, 4]
+    rng = random.Random(11)
+    changed = 0
+    for _ in range(25):
+        seq = [rng.randrange(G) for _ in range(rng.randrange(0, 8))]
+        n, k, buf = _residual(L, spec, seq)
+        t0 = L.orc_bound(n, k, buf, 0, 0)
+        r0 = [L.orc_bound(n, k, buf, 4, A) for A in cuts]
+        for g in range(G):
+            n1, k1, b1 = _residual(L, spec, seq + [g])
+            t1 = L.orc_bound(n1, k1, b1, 0, 0)
+            if clifford[g]:
+                assert t1 == t0, (seq, names[g])
+            else:
+                assert abs(t1 - t0) <= 1
+                changed += t1 != t0
+            mask = sum(1 << q for q in qubits[g])
+            for A, r in zip(cuts, r0):
+                if (mask & A) == 0 or (mask & ~A) == 0:
+                    assert L.orc_bound(n1, k1, b1, 4, A) == r, (seq, names[g], A)
+    assert changed > 0

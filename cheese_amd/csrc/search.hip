@@ -58,11 +58,21 @@ template <int NQ, int WPE>
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(WPE))) void k_level(KSpec sp, ExpandArgs a,
                                                                                          long long np) {
   constexpr int N = 1 << NQ, E = N * N, GS = E, CPW = 64 / GS;
-  __shared__ TStage stage[256];
+  __shared__ uint2 stage_x[TS_STRIDE];
+  __shared__ uint32_t stage_p[6 * TS_STRIDE];
   __shared__ int4 spar[256];                   // each group's parent residual
   __shared__ int8_t sph[MAXG * 64];            // unit phase table of every candidate
   __shared__ GateHdr shdr[MAXG];
+  __shared__ uint8_t spos[NQ == 3 ? 16 * 64 : 1];   // n = 3: position of entry le under symmetry y (y * 64 + le)
   for (int i = threadIdx.x; i < sp.G * 64; i += blockDim.x) sph[i] = sp.gates[i >> 6].ph[i & 63];
+  uint32_t cjm = 0;
+  if constexpr (NQ == 3) {
+    for (int i = threadIdx.x; i < 16 * 64; i += blockDim.x) {
+      const int y = i >> 6, e = i & 63;
+      spos[i] = y < sp.nsym ? (uint8_t)(sp.symperm[y * 8 + (e >> 3)] * 8 + sp.symperm[y * 8 + (e & 7)]) : 0;
+    }
+    for (int y = 0; y < sp.nsym && y < 16; y++) cjm |= (uint32_t)(sp.symconj[y] != 0) << y;
+  }
   for (int g = threadIdx.x; g < sp.G; g += blockDim.x) {
     const DevGate &d = sp.gates[g];
     int kind = 0;
@@ -79,7 +89,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(WPE))) void
   __syncthreads();
   const int lane = threadIdx.x & 63, le = lane & (GS - 1), slot = lane / GS;
   const int r = le >> NQ, c = le & (N - 1);
-  TStage *st = stage + (threadIdx.x & ~(GS - 1));
+  const TStage st{stage_x + (threadIdx.x & ~(GS - 1)), stage_p + (threadIdx.x & ~(GS - 1))};
   int4 *P = spar + (threadIdx.x & ~(GS - 1));
   const long long wave = ((long long)blockIdx.x * blockDim.x + threadIdx.x) >> 6;
   const long long nwaves = ((long long)gridDim.x * blockDim.x) >> 6;
@@ -181,34 +191,47 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(WPE))) void
               int hh, supp = 0, c2 = 0;
               if (sp.lite) hh = k;
               else {
-                const int hs = h_struct<NQ>(s);
+                const uint64_t nzb = gballot<GS>(!zzero(s));
+                const int hs = h_struct<NQ>(nzb);
                 hh = k > hs ? k : hs;
-                supp = support<NQ>(s, le);
-                uint32_t needs = (pbw & ~BW_CUTS) >> 8;
-                c2 = cx_bound<NQ>(s, le, sp, (pbw & BW_CUTS) ? (uint32_t)h.loc : 0u, needs);
-                bw = (needs << 8 & ~BW_CUTS) | BW_CUTS | BW_TUNK;
+                supp = support<NQ>(s, le, nzb);
               }
-              // a Clifford gate keeps the parent's T bound; the bound combination is monotone in T, so a
-              // known T settles the child at once
+              // a Clifford gate keeps the parent's T bound.  The bound combination is monotone in T and in the
+              // cut bound, so it is tried with what is known before the cut ranks and the T bound are computed.
               const bool tknown = h.cliff && (pbw & BW_TUNK) != BW_TUNK;
               int T = tknown ? (int)(pbw & BW_TUNK) : 0;
-              if (combine_bound(hh, (T + sp.tdiv - 1) / sp.tdiv, c2, supp) > rem) alive = false;
-              else if (!tknown) {
-                n_t += le == 0;
-                if (NQ + k <= 16) {
-                  __builtin_amdgcn_wave_barrier();
-                  t_stage<NQ>(st, s, le);
-                  __builtin_amdgcn_wave_barrier();
-                  T = t_bound16<NQ>(st, s, k, le);
-                  __builtin_amdgcn_wave_barrier();
-                } else {
-                  T = t_bound_any<NQ>(s, k, le);
+              if (combine_bound(hh, (T + sp.tdiv - 1) / sp.tdiv, 0, supp) > rem) alive = false;
+              else {
+                if (!sp.lite) {
+                  uint32_t needs = (pbw & ~BW_CUTS) >> 8;
+                  c2 = cx_bound<NQ>(s, le, sp, (pbw & BW_CUTS) ? (uint32_t)h.loc : 0u, needs);
+                  bw = (needs << 8 & ~BW_CUTS) | BW_CUTS | BW_TUNK;
                 }
-                alive = combine_bound(hh, (T + sp.tdiv - 1) / sp.tdiv, c2, supp) <= rem;
+                if (combine_bound(hh, (T + sp.tdiv - 1) / sp.tdiv, c2, supp) > rem) alive = false;
+                else if (!tknown) {
+                  n_t += le == 0;
+                  if (NQ + k <= 16) {
+                    __builtin_amdgcn_wave_barrier();
+                    t_stage<NQ>(st, s, le);
+                    __builtin_amdgcn_wave_barrier();
+                    T = t_bound16<NQ>(st, s, k, le);
+                    __builtin_amdgcn_wave_barrier();
+                  } else {
+                    T = t_bound_any<NQ>(s, k, le);
+                  }
+                  alive = combine_bound(hh, (T + sp.tdiv - 1) / sp.tdiv, c2, supp) <= rem;
+                }
               }
               if (alive && T < (int)BW_TUNK) bw = (bw & ~BW_TUNK) | (uint32_t)T;
             }
-            if (alive && keep) key = class_key<NQ>(s, k, le, sp) ^ ((uint64_t)inst * 0x9e3779b97f4a7c15ull);
+            if (alive && keep) {
+              if constexpr (NQ == 3) {
+                key = sp.nsym <= 16 ? class_key3(s, k, lane, spos, cjm, sp.nsym) : class_key<NQ>(s, k, le, sp);
+              } else {
+                key = class_key<NQ>(s, k, le, sp);
+              }
+              key ^= (uint64_t)inst * 0x9e3779b97f4a7c15ull;
+            }
             stt = alive ? ST_ALIVE : ST_DEAD;
           }
         }

@@ -76,10 +76,10 @@ __device__ __forceinline__ bool make_child(const int4 *__restrict__ par, int pk,
   return true;
 }
 
-// 2 if the matrix is monomial with a column permutation that is not affine over GF(2) (src/cheese.cu:321-338)
-template <int NQ> __device__ __forceinline__ int h_struct(int4 s) {
-  constexpr int N = 1 << NQ, GS = N * N;
-  const uint64_t b = gballot<GS>(!zzero(s));
+// 2 if the matrix is monomial with a column permutation that is not affine over GF(2) (src/cheese.cu:321-338);
+// b = group ballot of the nonzero entries
+template <int NQ> __device__ __forceinline__ int h_struct(uint64_t b) {
+  constexpr int N = 1 << NQ;
   uint64_t col0 = 0;
 #pragma unroll
   for (int r = 0; r < N; r++) col0 |= 1ull << (r * N);
@@ -100,18 +100,27 @@ template <int NQ> __device__ __forceinline__ int h_struct(int4 s) {
   return 0;
 }
 
-// number of qubits the matrix acts on non-trivially (src/cheese.cu:305-319)
-template <int NQ> __device__ __forceinline__ int support(int4 s, int le) {
+// entries (r, c) with bit q of r and c different, as a group-local lane mask
+template <int NQ> __host__ __device__ constexpr uint64_t offdiag_mask(int q) {
+  constexpr int N = 1 << NQ, GS = N * N;
+  uint64_t m = 0;
+  for (int le = 0; le < GS; le++)
+    if ((((le / N) ^ (le % N)) >> q) & 1) m |= 1ull << le;
+  return m;
+}
+
+// number of qubits the matrix acts on non-trivially (src/cheese.cu:305-319); b = group ballot of the nonzero
+// entries.  A nonzero entry off the qubit's diagonal blocks settles that qubit without comparing blocks.
+template <int NQ> __device__ __forceinline__ int support(int4 s, int le, uint64_t b) {
   constexpr int N = 1 << NQ, GS = N * N;
   const int r = le >> NQ, c = le & (N - 1);
   int cnt = 0;
 #pragma unroll
   for (int q = 0; q < NQ; q++) {
+    if (b & offdiag_mask<NQ>(q)) { cnt++; continue; }
     const int bit = 1 << q;
     const int4 o = gxor<GS>(s, bit * N + bit);
-    bool viol = false;
-    if ((r ^ c) & bit) viol = !zzero(s);
-    else if (!((r | c) & bit)) viol = !zeq(s, o);
+    const bool viol = !((r | c) & bit) && !zeq(s, o);
     cnt += gany<GS>(viol);
   }
   return cnt;
@@ -234,25 +243,25 @@ typedef unsigned short u16x2 __attribute__((ext_vector_type(2)));
 __device__ __forceinline__ u16x2 pk(int lo, int hi) { u16x2 r; r.x = (unsigned short)lo; r.y = (unsigned short)hi; return r; }
 __device__ __forceinline__ uint32_t bits(u16x2 v) { return (uint32_t)v.x | ((uint32_t)v.y << 16); }
 
-// Per-entry operands of the packed transform, staged in LDS by the lane owning entry e:
-// X[e] = (a, b | c, d) of S[e] mod 2^16, and the six packed multipliers of conj(S[e]) with which
-// S[x] conj(S[e]) = (Qa, Qb | Qc, Qd) = xa P1 + xb P2 + xc P3 + xd P4 | xa P5 + xb P6 + xc P1 + xd P2.
+// Per-entry operands of the packed transform, staged in LDS by the lane owning entry e, one array per field
+// (a group's entries are read at a stride of 8 bytes, free of bank conflicts):
+// x[e] = (a, b | c, d) of S[e] mod 2^16, and p[i * TS_STRIDE + e] the six packed multipliers of conj(S[e]) with
+// which S[x] conj(S[e]) = (Qa, Qb | Qc, Qd) = xa P1 + xb P2 + xc P3 + xd P4 | xa P5 + xb P6 + xc P1 + xd P2.
+constexpr int TS_STRIDE = 256;      // entries per field: one per thread of a block
 struct TStage {
-  uint2 x;           // (a | b << 16, c | d << 16)
-  uint32_t p[6];
+  uint2 *x;
+  uint32_t *p;
 };
 
-template <int NQ> __device__ __forceinline__ void t_stage(TStage *st, int4 s, int le) {
+template <int NQ> __device__ __forceinline__ void t_stage(TStage st, int4 s, int le) {
   const int a = s.x, b = s.y, c = s.z, d = s.w;
-  TStage t;
-  t.x = make_uint2(bits(pk(a, b)), bits(pk(c, d)));
-  t.p[0] = bits(pk(a, -d));
-  t.p[1] = bits(pk(b, a));
-  t.p[2] = bits(pk(c, b));
-  t.p[3] = bits(pk(d, c));
-  t.p[4] = bits(pk(-c, -b));
-  t.p[5] = bits(pk(-d, -c));
-  st[le] = t;
+  st.x[le] = make_uint2(bits(pk(a, b)), bits(pk(c, d)));
+  st.p[0 * TS_STRIDE + le] = bits(pk(a, -d));
+  st.p[1 * TS_STRIDE + le] = bits(pk(b, a));
+  st.p[2 * TS_STRIDE + le] = bits(pk(c, b));
+  st.p[3 * TS_STRIDE + le] = bits(pk(d, c));
+  st.p[4 * TS_STRIDE + le] = bits(pk(-c, -b));
+  st.p[5 * TS_STRIDE + le] = bits(pk(-d, -c));
 }
 
 __device__ __forceinline__ u16x2 as16(uint32_t v) { u16x2 r; r.x = (unsigned short)v; r.y = (unsigned short)(v >> 16); return r; }
@@ -267,7 +276,7 @@ __device__ __forceinline__ int ctz16(uint32_t v) { return v & 0xffffu ? __ffs((i
 // x + y sqrt2 has valuation min(2 v2(x), 2 v2(y) + 1), so the smallest valuation over all outputs is read
 // from three OR accumulators.  f = 0 (Q_0 = |S|^2, real) is transformed by the whole group, one lane per entry.
 // Lane f of the group handles flip pair f (lane 0 repeats f = 1; OR-accumulation is idempotent).
-template <int NQ> __device__ __forceinline__ int t_bound16(const TStage *st, int4 s, int k, int le) {
+template <int NQ> __device__ __forceinline__ int t_bound16(TStage st, int4 s, int k, int le) {
   constexpr int E = 1 << (2 * NQ), H = E / 2, GS = E;
   int f = le == 0 ? 1 : le;
   // keep the per-lane LDS address arithmetic inside the child loop: hoisted, its 64 addresses would
@@ -279,12 +288,12 @@ template <int NQ> __device__ __forceinline__ int t_bound16(const TStage *st, int
 #pragma unroll
   for (int i = 0; i < H; i++) {
     const int e = ((i & ~lowm) << 1) | (i & lowm);     // i with a 0 inserted at bit j
-    const uint2 xv = st[e ^ f].x;
-    const TStage &z = st[e];
+    const uint2 xv = st.x[e ^ f];
     const u16x2 x01 = as16(xv.x), x23 = as16(xv.y);
     const u16x2 xa = x01.xx, xb = x01.yy, xc = x23.xx, xd = x23.yy;
-    const u16x2 p1 = as16(z.p[0]), p2 = as16(z.p[1]), p3 = as16(z.p[2]), p4 = as16(z.p[3]), p5 = as16(z.p[4]),
-                p6 = as16(z.p[5]);
+    const u16x2 p1 = as16(st.p[0 * TS_STRIDE + e]), p2 = as16(st.p[1 * TS_STRIDE + e]),
+                p3 = as16(st.p[2 * TS_STRIDE + e]), p4 = as16(st.p[3 * TS_STRIDE + e]),
+                p5 = as16(st.p[4 * TS_STRIDE + e]), p6 = as16(st.p[5 * TS_STRIDE + e]);
     qab[i] = xa * p1 + xb * p2 + xc * p3 + xd * p4;
     qcd[i] = xa * p5 + xb * p6 + xc * p1 + xd * p2;
   }
@@ -351,6 +360,83 @@ template <int NQ> __device__ __forceinline__ uint64_t class_key(int4 s, int k, i
     best = key < best ? key : best;
   }
   return best;
+}
+
+// ---- wave64 exchanges without the LDS crossbar (gfx950): permlane32/16 swaps and DPP row controls ----
+__device__ __forceinline__ uint64_t u64_of(uint32_t lo, uint32_t hi) { return (uint64_t)lo | ((uint64_t)hi << 32); }
+
+// lanes 0-31 receive a + a[lane ^ 32], lanes 32-63 receive b + b[lane ^ 32]: v_permlane32_swap exchanges the
+// upper half of its first operand with the lower half of its second
+__device__ __forceinline__ uint64_t rs32(uint64_t a, uint64_t b) {
+  const auto lo = __builtin_amdgcn_permlane32_swap((uint32_t)a, (uint32_t)b, false, false);
+  const auto hi = __builtin_amdgcn_permlane32_swap((uint32_t)(a >> 32), (uint32_t)(b >> 32), false, false);
+  return u64_of(lo[0], hi[0]) + u64_of(lo[1], hi[1]);
+}
+// lanes with bit 4 clear receive a + a[lane ^ 16], the others b + b[lane ^ 16]: v_permlane16_swap exchanges
+// the odd 16-lane rows of its first operand with the even rows of its second
+__device__ __forceinline__ uint64_t rs16(uint64_t a, uint64_t b) {
+  const auto lo = __builtin_amdgcn_permlane16_swap((uint32_t)a, (uint32_t)b, false, false);
+  const auto hi = __builtin_amdgcn_permlane16_swap((uint32_t)(a >> 32), (uint32_t)(b >> 32), false, false);
+  return u64_of(lo[0], hi[0]) + u64_of(lo[1], hi[1]);
+}
+__device__ __forceinline__ uint64_t partner32(uint64_t x, int lane) {       // x[lane ^ 32]
+  const auto lo = __builtin_amdgcn_permlane32_swap((uint32_t)x, (uint32_t)x, false, false);
+  const auto hi = __builtin_amdgcn_permlane32_swap((uint32_t)(x >> 32), (uint32_t)(x >> 32), false, false);
+  return (lane & 32) ? u64_of(lo[0], hi[0]) : u64_of(lo[1], hi[1]);
+}
+__device__ __forceinline__ uint64_t partner16(uint64_t x, int lane) {       // x[lane ^ 16]
+  const auto lo = __builtin_amdgcn_permlane16_swap((uint32_t)x, (uint32_t)x, false, false);
+  const auto hi = __builtin_amdgcn_permlane16_swap((uint32_t)(x >> 32), (uint32_t)(x >> 32), false, false);
+  return (lane & 16) ? u64_of(lo[0], hi[0]) : u64_of(lo[1], hi[1]);
+}
+// DPP controls: row mirror (lane ^ 15 within a 16-lane row), half-row mirror (lane ^ 7), quad permutations
+constexpr int DPP_MIRROR = 0x140, DPP_HALF_MIRROR = 0x141, DPP_QXOR1 = 0xB1, DPP_QXOR2 = 0x4E;
+template <int CTRL> __device__ __forceinline__ uint64_t dpp64(uint64_t v) {
+  const uint32_t lo = (uint32_t)__builtin_amdgcn_update_dpp(0, (int)(uint32_t)v, CTRL, 0xf, 0xf, false);
+  const uint32_t hi = (uint32_t)__builtin_amdgcn_update_dpp(0, (int)(uint32_t)(v >> 32), CTRL, 0xf, 0xf, false);
+  return u64_of(lo, hi);
+}
+
+// class_key for n = 3 (one matrix per wave64, whole wave active), at most 16 symmetries: the 16 per-lane
+// hashes are reduce-scattered over the wave (lane ^ 32, ^ 16 by permlane swaps; ^ 15, ^ 7 by DPP mirrors; then
+// sums over each quad), so lane L ends with the wave sum of slot (L >> 2) & 15; one mix per lane and a min over
+// lane bits 2-5 give the same key as class_key.  spos[y * 64 + le]: position of entry le under symmetry y;
+// cjm bit y: symmetry y conjugates.
+__device__ __forceinline__ uint64_t class_key3(int4 s, int k, int lane, const uint8_t *spos, uint32_t cjm, int nsym) {
+  const uint64_t hv = entry_hash(s), hc = entry_hash(zconj(s));
+  auto slot = [&](int y) -> uint64_t {
+    if (y >= nsym) return 0;
+    const uint64_t pos = (uint64_t)spos[y * 64 + lane] + 1;
+    return mix64((((cjm >> y) & 1) ? hc : hv) ^ (pos * 0x9e3779b97f4a7c15ull));
+  };
+  uint64_t h[8];      // slots y and y + 8 are combined as soon as they are hashed: 8 live values, not 16
+#pragma unroll
+  for (int j = 0; j < 8; j++) h[j] = rs32(slot(j), slot(j + 8));
+#pragma unroll
+  for (int j = 0; j < 4; j++) h[j] = rs16(h[j], h[j + 4]);
+  const bool b3 = lane & 8, b2 = lane & 4;
+#pragma unroll
+  for (int j = 0; j < 2; j++) {
+    const uint64_t send = b3 ? h[j] : h[j + 2], keep = b3 ? h[j + 2] : h[j];
+    h[j] = keep + dpp64<DPP_MIRROR>(send);
+  }
+  uint64_t x;
+  {
+    const uint64_t send = b2 ? h[0] : h[1], keep = b2 ? h[1] : h[0];
+    x = keep + dpp64<DPP_HALF_MIRROR>(send);
+  }
+  x += dpp64<DPP_QXOR1>(x);
+  x += dpp64<DPP_QXOR2>(x);
+  const int y = (lane >> 2) & 15;
+  uint64_t key = y < nsym ? mix64(x ^ ((uint64_t)(k + 1) * 0xc2b2ae3d27d4eb4full)) : ~0ull;
+  uint64_t o = dpp64<DPP_HALF_MIRROR>(key);
+  key = o < key ? o : key;
+  o = dpp64<DPP_MIRROR>(key);
+  key = o < key ? o : key;
+  o = partner16(key, lane);
+  key = o < key ? o : key;
+  o = partner32(key, lane);
+  return o < key ? o : key;
 }
 
 // exact: B = sigma(A) for some group element (src/cheese.cu:242-255)

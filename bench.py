@@ -18,8 +18,8 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 METRIC = "prefixes/s & time-to-proof, Toffoli d=12; served solves/s + p50/p99 vs plain"
-# 256 CUs x 4 SIMD16 x 2.4 GHz x 2 (packed 16-bit halves): peak 16-bit integer VALU operations per second
-VALU16_PEAK_TOPS = 256 * 4 * 16 * 2 * 2.4e9 / 1e12
+# 256 CUs x 4 SIMD16 x 2.4 GHz: peak 32-bit integer VALU lane operations per second
+VALU32_PEAK_TOPS = 256 * 4 * 16 * 2.4e9 / 1e12
 
 
 def _strip(lines):
@@ -217,7 +217,7 @@ def main():
     tuned = searcher.tune(2) if a.tune else None
     for _ in range(a.warmup):
         searcher.run()
-    times, prefixes, kstats = [], 0, {}
+    times, prefixes, kstats, stages = [], 0, {}, {}
     for _ in range(a.steps):
         t = time.perf_counter()
         out = searcher.run(stats=True)
@@ -227,17 +227,21 @@ def main():
             print("bench.py: per-level log differs from the reference golden log", file=sys.stderr)
             sys.exit(1)
         prefixes = sum(l["prefixes"] for l in out["levels"])
+        for k, v in out["expand_stages"].items():
+            stages[k] = stages.get(k, 0) + v
         for k, v in out["kernel_stats"].items():
             s = kstats.setdefault(k, {"launches": 0, "ms": 0.0, "work": 0.0})
             for f in s:
                 s[f] += v[f]
     total = sum(times)
     tb = kstats.get("expand", {"ms": 0.0, "work": 0.0, "launches": 0})
-    traffic = None        # HBM bytes per k_level launch, from the committed rocprofv3 FETCH/WRITE summary
+    traffic = valu_busy = None   # from the committed rocprofv3 summaries (FETCH/WRITE_SIZE; SQ_INSTS_VALU)
     tpath = os.path.join(ROOT, "profiles", "k_level-toffoli-d12.traffic.json")
     if os.path.exists(tpath):
         with open(tpath) as f:
-            traffic = round(json.load(f)["k_level_hbm_bytes_per_launch"])
+            prof = json.load(f)
+        traffic = round(prof["k_level_hbm_bytes_per_launch"])
+        valu_busy = prof.get("k_level_valu_busy")
     achieved = tb["work"] / (tb["ms"] / 1e3) / 1e12 if tb["ms"] else 0.0
     line = {
         "metric": METRIC,
@@ -254,11 +258,14 @@ def main():
         "data": "reference target tests/qasm/toffoli.qasm, exact; log checked against the reference each step",
         "config": {"workload": "Toffoli gate-count proof, budget 12 (3 qubits, 27 candidates, 12 symmetries)",
                    "prefixes_per_step": prefixes, "parallelism": "1 GPU"},
-        "roofline": {"bound": "valu", "achieved": round(achieved, 3), "peak": VALU16_PEAK_TOPS, "unit": "Top/s",
-                     "frac": round(achieved / VALU16_PEAK_TOPS, 4), "traffic": traffic,
+        "roofline": {"bound": "valu", "achieved": round(achieved, 3), "peak": VALU32_PEAK_TOPS, "unit": "Top/s",
+                     "frac": round(achieved / VALU32_PEAK_TOPS, 4), "traffic": traffic,
                      "kernel": "k_level", "kernel_ms_per_step": round(tb["ms"] / a.steps, 3),
                      "launches_per_step": tb["launches"] / a.steps,
-                     "work": "16-bit integer ops of the T-bound transforms only (cheap bounds not counted)"},
+                     "work": "algorithmic 32-bit integer lane ops: stage counts x per-stage ops (residual, "
+                             "structural bounds, cut ranks, T transform at two 16-bit ops per lane op, class keys)",
+                     "stages_per_step": {k: v // a.steps for k, v in stages.items()},
+                     "valu_busy_profiled": valu_busy},
         "kernels_ms_per_step": {k: round(v["ms"] / a.steps, 3) for k, v in kstats.items()},
         "autotune": {b: [v["waves_per_eu"], v["grid_cap"], v["split_waves"]] for b, v in tuned.items()} if tuned else None,
     }

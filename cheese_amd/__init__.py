@@ -58,6 +58,7 @@ SIGNATURES = {
     "chz_close": (None, [ctypes.c_void_p]),
     "chz_profile": (ctypes.c_int, [ctypes.c_void_p, ctypes.POINTER(ctypes.c_char_p), ctypes.POINTER(ctypes.c_double), ctypes.c_int]),
     "chz_kernel_stats": (ctypes.c_int, [ctypes.c_void_p, ctypes.POINTER(KStat), ctypes.c_int]),
+    "chz_expand_stages": (ctypes.c_int, [ctypes.c_void_p, ctypes.POINTER(ctypes.c_double)]),
     "chz_verify": (ctypes.c_int, [ctypes.c_void_p, ctypes.POINTER(ctypes.c_int), ctypes.c_int]),
     "chz_last_error": (ctypes.c_char_p, []),
     "chz_verify_spec": (ctypes.c_int, [ctypes.c_char_p, ctypes.c_size_t, ctypes.POINTER(ctypes.c_int), ctypes.c_int]),
@@ -140,6 +141,10 @@ def search(spec_text: str, depth: int | None = None, dedupe: bool = True, sym: b
         L.chz_close(ctx)
 
 
+# expansion stage counts (chz_expand_stages)
+STAGES = ("residuals", "structural", "cut_ranks", "t_bounds", "keys")
+
+
 class Searcher:
     """one context reused for repeated searches of its spec (chz_open once, chz_run per search): device buffers,
     the stream and the tree pool persist between runs, as in a long-running caller"""
@@ -180,6 +185,9 @@ class Searcher:
             nk = L.chz_kernel_stats(self._ctx, ks, 16)
             out["kernel_stats"] = {ks[i].name.decode(): {"launches": ks[i].launches, "ms": ks[i].ms, "work": ks[i].work}
                                    for i in range(nk)}
+            st = (ctypes.c_double * 5)()
+            L.chz_expand_stages(self._ctx, st)
+            out["expand_stages"] = dict(zip(STAGES, [int(v) for v in st]))
         return out
 
     def close(self):

@@ -23,7 +23,10 @@ namespace chz {
 
 // ------------------------------------------------------------------------------------------------------
 // counters of one level (device): indices into a u64 array
-enum { C_W2 = 0, C_W3, C_DEAD, C_FOUND, C_ALIVE, C_OVF, C_NEEDT, C_FIRST, C_RUNS, C_REPOVF, C_KEPT, C_EVAL, C_SURV, C_N };
+// C_EVAL children whose residual is formed, C_CHEAP through the structural bounds, C_CUTS cut ranks computed,
+// C_NEEDT T-bound evaluations, C_KEYS class keys: the stage counts of the expansion's work model
+enum { C_W2 = 0, C_W3, C_DEAD, C_FOUND, C_ALIVE, C_OVF, C_NEEDT, C_FIRST, C_RUNS, C_REPOVF, C_KEPT, C_EVAL, C_SURV,
+       C_CHEAP, C_CUTS, C_KEYS, C_N };
 // per-instance counters of one level (a batch carries several searches that share the spec's gate set)
 enum { IC_W2 = 0, IC_W3, IC_DEAD, IC_FOUND, IC_ALIVE, IC_OVF, IC_FIRST, IC_N };
 
@@ -38,7 +41,7 @@ struct ExpandArgs {
   uint64_t *keys;           // [ncc] class keys of kept children
   uint32_t *info;           // [ncc] bound words of kept children
   const uint32_t *pb;       // parents' bound words
-  unsigned long long *cnt;  // global counters (C_EVAL, C_NEEDT)
+  unsigned long long *cnt;  // global counters (stage counts C_EVAL, C_CHEAP, C_CUTS, C_NEEDT, C_KEYS)
   const uint32_t *pinst;    // instance slot of every parent of the level (null: one instance, slot 0)
   const int *rem;           // per slot: gate budget left after this level's gate (depth - t)
   const uint8_t *keepi;     // per slot: this level's survivors become parents (t < depth)
@@ -62,9 +65,32 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(WPE))) void
   __shared__ uint32_t stage_p[6 * TS_STRIDE];
   __shared__ int4 spar[256];                   // each group's parent residual
   __shared__ int8_t sph[MAXG * 64];            // unit phase table of every candidate
+  // per candidate and entry (g * 64 + e), the child entry's terms for unit gates with at most two per column:
+  // bits 0-2 source column x0, 3-5 power p0 of w, 6-8 x1, 9-11 p1, bit 12 two terms; bit 15: use the table loop
+  __shared__ uint16_t sdesc[MAXG * 64];
   __shared__ GateHdr shdr[MAXG];
   __shared__ uint8_t spos[NQ == 3 ? 16 * 64 : 1];   // n = 3: position of entry le under symmetry y (y * 64 + le)
   for (int i = threadIdx.x; i < sp.G * 64; i += blockDim.x) sph[i] = sp.gates[i >> 6].ph[i & 63];
+  for (int i = threadIdx.x; i < sp.G * GS; i += blockDim.x) {
+    const int g = i / GS, e = i % GS, ec = e & (N - 1);
+    const DevGate &d = sp.gates[g];
+    int lc = (ec >> d.q[0]) & 1;
+    if (d.nq > 1) lc |= ((ec >> d.q[1]) & 1) << 1;
+    if (d.nq > 2) lc |= ((ec >> d.q[2]) & 1) << 2;
+    const int cb = ec & ~d.mask;
+    uint32_t desc = 0;
+    int nt = 0;
+    for (int m = 0; d.unit && m < d.L; m++) {
+      const int p = d.ph[m * d.L + lc];
+      if (p < 0) continue;
+      int x = cb | ((m & 1) << d.q[0]);
+      if (d.nq > 1) x |= ((m >> 1) & 1) << d.q[1];
+      if (d.nq > 2) x |= ((m >> 2) & 1) << d.q[2];
+      if (nt < 2) desc |= (uint32_t)(x | (p << 3)) << (6 * nt);
+      nt++;
+    }
+    sdesc[g * 64 + e] = (uint16_t)(!d.unit || nt == 0 || nt > 2 ? 0x8000u : nt == 2 ? desc | 0x1000u : desc);
+  }
   uint32_t cjm = 0;
   if constexpr (NQ == 3) {
     for (int i = threadIdx.x; i < 16 * 64; i += blockDim.x) {
@@ -95,17 +121,19 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(WPE))) void
   const long long nwaves = ((long long)gridDim.x * blockDim.x) >> 6;
   const int G = sp.G;
   // per-instance tallies live in each group's leader lane and are flushed when the group's instance changes
-  unsigned long long n_w2 = 0, n_w3 = 0, n_dead = 0, n_found = 0, n_ovf = 0, n_alive = 0, n_eval = 0, n_t = 0;
+  // (32-bit: one wave's share of a launch is far below 2^32 children; fewer registers live across the T bound)
+  uint32_t n_w2 = 0, n_w3 = 0, n_dead = 0, n_found = 0, n_ovf = 0, n_alive = 0, n_eval = 0, n_t = 0;
+  uint32_t n_cheap = 0, n_cuts = 0, n_keys = 0;
   int cur = -1;
   auto flush = [&]() {
     if (le == 0 && cur >= 0) {
       unsigned long long *ic = a.icnt + (size_t)cur * IC_N;
-      if (n_w2) atomicAdd(&ic[IC_W2], n_w2);
-      if (n_w3) atomicAdd(&ic[IC_W3], n_w3);
-      if (n_dead) atomicAdd(&ic[IC_DEAD], n_dead);
-      if (n_found) atomicAdd(&ic[IC_FOUND], n_found);
-      if (n_ovf) atomicAdd(&ic[IC_OVF], n_ovf);
-      if (n_alive) atomicAdd(&ic[IC_ALIVE], n_alive);
+      if (n_w2) atomicAdd(&ic[IC_W2], (unsigned long long)n_w2);
+      if (n_w3) atomicAdd(&ic[IC_W3], (unsigned long long)n_w3);
+      if (n_dead) atomicAdd(&ic[IC_DEAD], (unsigned long long)n_dead);
+      if (n_found) atomicAdd(&ic[IC_FOUND], (unsigned long long)n_found);
+      if (n_ovf) atomicAdd(&ic[IC_OVF], (unsigned long long)n_ovf);
+      if (n_alive) atomicAdd(&ic[IC_ALIVE], (unsigned long long)n_alive);
     }
     n_w2 = n_w3 = n_dead = n_found = n_ovf = n_alive = 0;
   };
@@ -128,14 +156,27 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(WPE))) void
     __builtin_amdgcn_wave_barrier();
     P[le] = S;
     __builtin_amdgcn_wave_barrier();
+    // window cuts of every candidate of this parent, one candidate per lane, as two bit masks: the candidate
+    // loop then tests bits instead of waiting on a table load per child
+    uint64_t w2m = 0, w3m = 0;
+    for (int b0 = 0; b0 < G; b0 += GS) {
+      const int g = b0 + le;
+      bool c2 = false, c3 = false;
+      if (g < G && g1 != 255) {
+        c2 = sp.bad2[g1 * G + g] != 0;
+        if (g2 != 255) {
+          const uint32_t tri = ((uint32_t)g2 * G + g1) * G + g;
+          c3 = ((sp.bad3[tri >> 5] >> (tri & 31)) & 1) != 0;
+        }
+      }
+      w2m |= gballot<GS>(c2) << b0;
+      w3m |= gballot<GS>(c3) << b0;
+    }
     for (int g = gbeg; g < gend; g++) {
       const long long cc = pl * G + g;
       uint8_t stt = 255;
-      if (g1 != 255 && sp.bad2[g1 * G + g]) stt = ST_W2;
-      else if (g2 != 255) {
-        const uint32_t tri = ((uint32_t)g2 * G + g1) * G + g;
-        if ((sp.bad3[tri >> 5] >> (tri & 31)) & 1) stt = ST_W3;
-      }
+      if ((w2m >> g) & 1) stt = ST_W2;
+      else if ((w3m >> g) & 1) stt = ST_W3;
       uint64_t key = 0;
       uint32_t bw = BW_TUNK;
       if (stt == 255) {
@@ -148,7 +189,14 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(WPE))) void
         const int8_t *ph = sph + g * 64;
         int4 s;
         bool of = false;
-        if (h.kind && small) {
+        const uint32_t desc = sdesc[g * 64 + le];
+        if (h.kind && small && !(desc & 0x8000u)) {
+          s = zrot(P[r * N + (desc & 7)], (desc >> 3) & 7);
+          if (desc & 0x1000u) {
+            const int4 t = zrot(P[r * N + ((desc >> 6) & 7)], (desc >> 9) & 7);
+            s.x += t.x; s.y += t.y; s.z += t.z; s.w += t.w;
+          }
+        } else if (h.kind && small) {
           int4 acc = make_int4(0, 0, 0, 0);
           for (int m = 0; m < h.L; m++) {
             const int p = ph[m * h.L + lc];
@@ -191,6 +239,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(WPE))) void
               int hh, supp = 0, c2 = 0;
               if (sp.lite) hh = k;
               else {
+                n_cheap += le == 0;
                 const uint64_t nzb = gballot<GS>(!zzero(s));
                 const int hs = h_struct<NQ>(nzb);
                 hh = k > hs ? k : hs;
@@ -204,7 +253,9 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(WPE))) void
               else {
                 if (!sp.lite) {
                   uint32_t needs = (pbw & ~BW_CUTS) >> 8;
-                  c2 = cx_bound<NQ>(s, le, sp, (pbw & BW_CUTS) ? (uint32_t)h.loc : 0u, needs);
+                  const uint32_t keepc = (pbw & BW_CUTS) ? (uint32_t)h.loc : 0u;
+                  n_cuts += le == 0 ? sp.ncuts - __popc(keepc & ((1u << sp.ncuts) - 1)) : 0;
+                  c2 = cx_bound<NQ>(s, le, sp, keepc, needs);
                   bw = (needs << 8 & ~BW_CUTS) | BW_CUTS | BW_TUNK;
                 }
                 if (combine_bound(hh, (T + sp.tdiv - 1) / sp.tdiv, c2, supp) > rem) alive = false;
@@ -225,6 +276,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(WPE))) void
               if (alive && T < (int)BW_TUNK) bw = (bw & ~BW_TUNK) | (uint32_t)T;
             }
             if (alive && keep) {
+              n_keys += le == 0;
               if constexpr (NQ == 3) {
                 key = sp.nsym <= 16 ? class_key3(s, k, lane, spos, cjm, sp.nsym) : class_key<NQ>(s, k, le, sp);
               } else {
@@ -259,9 +311,15 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(WPE))) void
   flush();
   n_eval = gsum<64>(n_eval);
   n_t = gsum<64>(n_t);
+  n_cheap = gsum<64>(n_cheap);
+  n_cuts = gsum<64>(n_cuts);
+  n_keys = gsum<64>(n_keys);
   if (lane == 0) {
-    if (n_eval) atomicAdd(&a.cnt[C_EVAL], n_eval);
-    if (n_t) atomicAdd(&a.cnt[C_NEEDT], n_t);
+    if (n_eval) atomicAdd(&a.cnt[C_EVAL], (unsigned long long)n_eval);
+    if (n_t) atomicAdd(&a.cnt[C_NEEDT], (unsigned long long)n_t);
+    if (n_cheap) atomicAdd(&a.cnt[C_CHEAP], (unsigned long long)n_cheap);
+    if (n_cuts) atomicAdd(&a.cnt[C_CUTS], (unsigned long long)n_cuts);
+    if (n_keys) atomicAdd(&a.cnt[C_KEYS], (unsigned long long)n_keys);
   }
 }
 
@@ -519,6 +577,8 @@ struct chz_ctx {
   chz::EventLog ev;
   long k_launch[chz::K_NFAM] = {};
   double k_ms[chz::K_NFAM] = {}, k_work[chz::K_NFAM] = {};
+  double k_stage[5] = {};       // expansion stage counts: residuals, structural bounds, cut ranks, T bounds, keys
+  double avg_terms = 1;         // nonzero terms per inverse column, averaged over the candidates
 };
 
 namespace chz {
@@ -556,6 +616,23 @@ static void tree_alloc(chz_ctx *c, long long n, uint32_t *&tp, uint8_t *&tg) {
 static double t_ops16(int nq) {
   const double E = (double)(1 << (2 * nq)), H = E / 2;
   return (E - 1) * (H * 28 + (2 * nq - 1) * H * 4 + H * 6) + E * 12 + 2 * nq * E * 2;
+}
+
+// Algorithmic integer operations of one level's expansion, by stage, in 32-bit lane operations (two 16-bit
+// operations of the packed T transform count as one):
+//   residual: per entry, 8 per nonzero term of the gate's inverse column (4 coefficient rotations, 4 adds), plus
+//             6 for the sqrt2-divisibility test and 4 for the goal comparison;
+//   structural bounds: per entry 4 for the nonzero test, 5 per qubit for the support comparison;
+//   cut rank: per entry 30 for the F_p image (4 Montgomery products and reductions) and 14 per elimination
+//             step (2 Montgomery products, 1 modular subtraction), stop = 3 steps;
+//   T bound: t_ops16 / 2;
+//   class key: per entry 20 per entry hash (2), 22 per symmetry (mix and 64-bit add).
+static double expand_ops(const chz_ctx *c, int nq, const unsigned long long *hc) {
+  const double E = (double)(1 << (2 * nq));
+  const double child = E * (8.0 * c->avg_terms + 10.0), cheap = E * (4.0 + 5.0 * nq), cut = E * (30.0 + 3 * 14.0);
+  const double key = E * (2 * 20.0 + 22.0 * c->ks.nsym);
+  return (double)hc[C_EVAL] * child + (double)hc[C_CHEAP] * cheap + (double)hc[C_CUTS] * cut +
+         (double)hc[C_NEEDT] * t_ops16(nq) / 2 + (double)hc[C_KEYS] * key;
 }
 
 
@@ -761,7 +838,9 @@ template <int NQ> static void batch_step(chz_ctx *c) {
   c->ev.flush(c->k_ms, c->k_launch);
   const long long nsurv = (long long)hc[C_SURV];
   c->k_work[K_COMPACT] += (double)nsurv * 24.0;
-  c->k_work[K_EXPAND] += (double)hc[C_NEEDT] * t_ops16(NQ);
+  c->k_work[K_EXPAND] += expand_ops(c, NQ, hc);
+  static const int stage_ctr[5] = {C_EVAL, C_CHEAP, C_CUTS, C_NEEDT, C_KEYS};
+  for (int i = 0; i < 5; i++) c->k_stage[i] += (double)hc[stage_ctr[i]];
   // searches that end at this level with a circuit or an overflow; the rest continue if their budget allows
   for (const BInst &b : c->binst)
     if (b.live) {
@@ -904,6 +983,7 @@ static void upload_spec(chz_ctx *c) {
   const HostSpec &s = c->hs;
   const int N = s.N(), G = s.G();
   std::vector<DevGate> dg(G);
+  double terms = 0, columns = 0;
   for (int g = 0; g < G; g++) {
     const HostGate &h = s.gates[g];
     DevGate &d = dg[g];
@@ -930,11 +1010,14 @@ static void upload_spec(chz_ctx *c) {
     // leaves that cut's operator-Schmidt rank unchanged.  Packed cut needs take 3 bits each in the node's
     // bound word, so at most BW_MAXCUTS cuts are carried.
     d.cliff = gate_is_clifford(h) ? 1 : 0;
+    for (const Zq &z : h.inv) terms += (z.a | z.b | z.c | z.d) != 0;
+    columns += d.L;
     d.cutloc = 0;
     if ((int)s.cutmask.size() <= BW_MAXCUTS)
       for (size_t i = 0; i < s.cutmask.size(); i++)
         if ((d.mask & s.cutmask[i]) == 0 || (d.mask & ~s.cutmask[i]) == 0) d.cutloc |= 1 << i;
   }
+  c->avg_terms = columns > 0 ? terms / columns : 1.0;
   std::vector<int4> goal(N * N);
   for (int i = 0; i < N * N; i++) goal[i] = make_int4((int)s.goal[i].a, (int)s.goal[i].b, (int)s.goal[i].c, (int)s.goal[i].d);
   std::vector<int> cx = s.cxtable.empty() ? std::vector<int>{0} : s.cxtable;
@@ -1061,6 +1144,7 @@ int chz_run(chz_ctx *c, chz_level_cb cb, void *user, chz_result *out) {
   std::memset(res, 0, sizeof *res);
   if (chz_batch_live(c)) { chz::set_err("the context's batch holds searches"); return res->rc = 3; }
   for (int f = 0; f < chz::K_NFAM; f++) { c->k_launch[f] = 0; c->k_ms[f] = 0; c->k_work[f] = 0; }
+  for (double &v : c->k_stage) v = 0;
   c->events.clear();
   const int rc0 = chz::batch_add_parsed(c, c->hs, c->depth, 0);
   if (rc0) return res->rc = rc0;
@@ -1192,6 +1276,11 @@ int chz_profile(const chz_ctx *c, const char **names, double *seconds, int n) {
     if (seconds) seconds[j] = c->k_ms[f] / 1000.0;
   }
   return j;
+}
+
+int chz_expand_stages(const chz_ctx *c, double *out) {
+  for (int i = 0; i < 5; i++) out[i] = c->k_stage[i];
+  return 5;
 }
 
 int chz_kernel_stats(const chz_ctx *c, chz_kstat *out, int n) {

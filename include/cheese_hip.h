@@ -82,6 +82,11 @@ typedef struct chz_kstat {
 } chz_kstat;
 int chz_kernel_stats(const chz_ctx *ctx, chz_kstat *out, int n);
 
+/* Stage counts of the last run's expansion (no reference counterpart; for measurement), the factors of its work
+ * model: out[0] residuals formed, [1] children through the structural bounds, [2] cut ranks computed, [3] T-bound
+ * evaluations, [4] class keys.  Returns 5. */
+int chz_expand_stages(const chz_ctx *ctx, double *out);
+
 /* Autotuner (no reference counterpart): times every candidate expansion schedule (occupancy variant x grid cap x
  * candidate-split threshold) on this context's own search, reps runs each, and keeps the fastest per level-size
  * bucket (log2 of the parents in a launch); later runs on the context use that table.  Results do not depend on

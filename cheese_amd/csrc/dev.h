@@ -14,14 +14,19 @@ struct DevGate {                               // S <- S g^-1 data of one candid
   int nq, L, mask, ik;
   int q[3];
   int unit;                                    // every nonzero inverse entry is +-w^p
-  int cliff;                                   // Clifford: children keep the parent's T bound
+  int tdel;                                    // sqrt2-denominator exponent of the gate's Pauli transfer
+                                               // matrix: a child's T bound is within tdel of its parent's
+                                               // (0: Clifford, equal); 255: not bounded
   int cutloc;                                  // bit i: the gate acts on one side of cut i only
+  int cdel;                                    // 3 bits per cut: a child's cut need is at least its parent's
+                                               // minus this (ceil log2 of the gate's Schmidt rank); 7: none
   int4 inv[64];                                // L x L inverse local matrix, row m, column lc
   int8_t ph[64];                               // unit gates: w-power p of inv[m][lc], -1 for zero
 };
 
 struct KSpec {                                 // device view of a spec (kernel argument)
   int n, G, depth, standard, lite, tdiv, gk, nsym, ncuts, have_table;
+  int cx_mono;                                 // cx table non-decreasing in every cut need
   int cutmask[16], cutsize[16], rad[16];
   uint32_t wm[4];                              // Montgomery constants w^i * 2^32 mod p, i = 0..3
   const DevGate *gates;

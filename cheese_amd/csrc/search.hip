@@ -53,8 +53,12 @@ struct ExpandArgs {
 // residual in LDS and walks all G candidates: windows, child residual, overflow, goal, cheap bounds, T bound
 // (mod 2^16, one lane per flip pair, when n + k <= 16; else mod 2^32 / 2^64, one lane per entry) and the
 // class key of a kept child.  Writes the status byte (and key) of every child of the chunk.
-// kind: 0 general, 1 unit entries, 2 unit permutation; cliff, loc: DevGate::cliff, DevGate::cutloc
-struct GateHdr { int nq, L, mask, ik, q0, q1, q2, kind, cliff, loc; };
+// kind: 0 general, 1 unit entries, 2 unit permutation; tdel, loc, cdel: DevGate::tdel, ::cutloc, ::cdel
+struct GateHdr { int nq, L, mask, ik, q0, q1, q2, kind, tdel, loc, cdel; };
+template <int NQ> __device__ __forceinline__ GateHdr wuni(const GateHdr &h) {
+  return GateHdr{wuni<NQ>(h.nq), wuni<NQ>(h.L), wuni<NQ>(h.mask), wuni<NQ>(h.ik), wuni<NQ>(h.q0), wuni<NQ>(h.q1),
+                 wuni<NQ>(h.q2), wuni<NQ>(h.kind), wuni<NQ>(h.tdel), wuni<NQ>(h.loc), wuni<NQ>(h.cdel)};
+}
 constexpr int MAXG = 64;
 
 template <int NQ, int WPE>
@@ -110,7 +114,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(WPE))) void
         if (nz != 1) kind = 1;
       }
     }
-    shdr[g] = GateHdr{d.nq, d.L, d.mask, d.ik, d.q[0], d.q[1], d.q[2], kind, d.cliff, d.cutloc};
+    shdr[g] = GateHdr{d.nq, d.L, d.mask, d.ik, d.q[0], d.q[1], d.q[2], kind, d.tdel, d.cutloc, d.cdel};
   }
   __syncthreads();
   const int lane = threadIdx.x & 63, le = lane & (GS - 1), slot = lane / GS;
@@ -145,13 +149,13 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(WPE))) void
     const long long pl = item / a.gsplit;
     const int gbeg = (int)(item % a.gsplit) * cs, gend = gbeg + cs < G ? gbeg + cs : G;
     const long long pi = a.off + pl;
-    const int inst = a.pinst ? (int)a.pinst[pi] : 0;
+    const int inst = wuni<NQ>(a.pinst ? (int)a.pinst[pi] : 0);
     if (inst != cur) { flush(); cur = inst; }
-    const int rem = a.rem[inst];
-    const bool keep = a.keepi[inst] != 0;
+    const int rem = wuni<NQ>(a.rem[inst]);
+    const bool keep = wuni<NQ>((int)a.keepi[inst]) != 0;
     const int4 S = a.res[pi * E + le];
-    const int pk = a.pk[pi], g1 = a.g1[pi], g2 = a.g2[pi];
-    const uint32_t pbw = a.pb[pi];
+    const int pk = wuni<NQ>(a.pk[pi]), g1 = wuni<NQ>((int)a.g1[pi]), g2 = wuni<NQ>((int)a.g2[pi]);
+    const uint32_t pbw = (uint32_t)wuni<NQ>((int)a.pb[pi]);
     const bool small = gall<GS>(zmag(S) < (1u << 28));
     __builtin_amdgcn_wave_barrier();
     P[le] = S;
@@ -181,7 +185,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(WPE))) void
       uint32_t bw = BW_TUNK;
       if (stt == 255) {
         // child residual entry (r, c) = sum_m P[r][x_m] ginv[m][lc] (src/cheese.cu:146-174)
-        const GateHdr h = shdr[g];
+        const GateHdr h = wuni<NQ>(shdr[g]);
         int lc = (c >> h.q0) & 1;
         if (h.nq > 1) lc |= ((c >> h.q1) & 1) << 1;
         if (h.nq > 2) lc |= ((c >> h.q2) & 1) << 2;
@@ -245,21 +249,31 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(WPE))) void
                 hh = k > hs ? k : hs;
                 supp = support<NQ>(s, le, nzb);
               }
-              // a Clifford gate keeps the parent's T bound.  The bound combination is monotone in T and in the
-              // cut bound, so it is tried with what is known before the cut ranks and the T bound are computed.
-              const bool tknown = h.cliff && (pbw & BW_TUNK) != BW_TUNK;
-              int T = tknown ? (int)(pbw & BW_TUNK) : 0;
-              if (combine_bound(hh, (T + sp.tdiv - 1) / sp.tdiv, 0, supp) > rem) alive = false;
+              // Bracket the child's bounds from its parent's: the T bound moves by at most the gate's transfer-
+              // matrix exponent h.tdel (Clifford gates keep it), each cut need drops by at most h.cdel.  The
+              // bound combination is monotone in T and in the cut bound, so the brackets settle most children
+              // before the cut ranks and the T transform are computed.
+              const bool tpar = (pbw & BW_TUNK) != BW_TUNK && h.tdel != 255;
+              const int Tp = (int)(pbw & BW_TUNK);
+              const bool tknown = tpar && h.tdel == 0;
+              int T = tpar ? (Tp > h.tdel ? Tp - h.tdel : 0) : 0;   // exact if tknown, else a lower bound
+              bool texact = tknown;
+              const int c2lo = !sp.lite && sp.cx_mono && (pbw & BW_CUTS) ? cx_lower(sp, (pbw & ~BW_CUTS) >> 8, (uint32_t)h.cdel) : 0;
+              if (combine_bound(hh, (T + sp.tdiv - 1) / sp.tdiv, c2lo, supp) > rem) alive = false;
               else {
                 if (!sp.lite) {
                   uint32_t needs = (pbw & ~BW_CUTS) >> 8;
                   const uint32_t keepc = (pbw & BW_CUTS) ? (uint32_t)h.loc : 0u;
                   n_cuts += le == 0 ? sp.ncuts - __popc(keepc & ((1u << sp.ncuts) - 1)) : 0;
                   c2 = cx_bound<NQ>(s, le, sp, keepc, needs);
-                  bw = (needs << 8 & ~BW_CUTS) | BW_CUTS | BW_TUNK;
+                  bw = (needs << 8 & ~BW_CUTS) | (sp.ncuts <= BW_MAXCUTS ? BW_CUTS : 0u) | BW_TUNK;
                 }
                 if (combine_bound(hh, (T + sp.tdiv - 1) / sp.tdiv, c2, supp) > rem) alive = false;
-                else if (!tknown) {
+                else if (!texact && tpar && rem <= 1 &&
+                         combine_bound(hh, (Tp + h.tdel + sp.tdiv - 1) / sp.tdiv, c2, supp) <= rem) {
+                  // alive whatever the exact T; its children (budget 0) need no inherited T bound
+                  alive = true;
+                } else if (!texact) {
                   n_t += le == 0;
                   if (NQ + k <= 16) {
                     __builtin_amdgcn_wave_barrier();
@@ -270,10 +284,11 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(WPE))) void
                   } else {
                     T = t_bound_any<NQ>(s, k, le);
                   }
+                  texact = true;
                   alive = combine_bound(hh, (T + sp.tdiv - 1) / sp.tdiv, c2, supp) <= rem;
                 }
               }
-              if (alive && T < (int)BW_TUNK) bw = (bw & ~BW_TUNK) | (uint32_t)T;
+              if (alive && texact && T < (int)BW_TUNK) bw = (bw & ~BW_TUNK) | (uint32_t)T;
             }
             if (alive && keep) {
               n_keys += le == 0;
@@ -1009,13 +1024,20 @@ static void upload_spec(chz_ctx *c) {
     // sqrt2-valuation over the transfer matrix of the residual) is its parent's; a gate on one side of a cut
     // leaves that cut's operator-Schmidt rank unchanged.  Packed cut needs take 3 bits each in the node's
     // bound word, so at most BW_MAXCUTS cuts are carried.
-    d.cliff = gate_is_clifford(h) ? 1 : 0;
+    const int sde = gate_ptm_sde(h);
+    d.tdel = sde >= 0 && sde < 255 ? sde : 255;
     for (const Zq &z : h.inv) terms += (z.a | z.b | z.c | z.d) != 0;
     columns += d.L;
     d.cutloc = 0;
+    d.cdel = 0;
     if ((int)s.cutmask.size() <= BW_MAXCUTS)
-      for (size_t i = 0; i < s.cutmask.size(); i++)
+      for (size_t i = 0; i < s.cutmask.size(); i++) {
         if ((d.mask & s.cutmask[i]) == 0 || (d.mask & ~s.cutmask[i]) == 0) d.cutloc |= 1 << i;
+        const int sr = gate_cut_rank(h, s.cutmask[i]);
+        int del = 0;
+        while (del < 7 && (1 << del) < sr) del++;
+        d.cdel |= (sr < 1 ? 7 : del) << (3 * i);
+      }
   }
   c->avg_terms = columns > 0 ? terms / columns : 1.0;
   std::vector<int4> goal(N * N);
@@ -1037,6 +1059,22 @@ static void upload_spec(chz_ctx *c) {
   k.n = s.n; k.G = G; k.depth = c->depth; k.standard = s.standard; k.lite = s.lite; k.tdiv = s.tdiv;
   k.gk = s.gk; k.nsym = c->grp.size(); k.ncuts = (int)s.cutmask.size(); k.have_table = !s.cxtable.empty();
   for (int i = 0; i < k.ncuts; i++) { k.cutmask[i] = s.cutmask[i]; k.cutsize[i] = s.cutsize[i]; k.rad[i] = s.rad[i]; }
+  // a cut bound from lowered needs is a lower bound only if the table never decreases when a need grows
+  k.cx_mono = 1;
+  if (k.have_table) {
+    size_t total = 1;
+    for (int i = 0; i < k.ncuts; i++) total *= (size_t)s.rad[i];
+    if (total != s.cxtable.size()) k.cx_mono = 0;
+    for (size_t idx = 0; k.cx_mono && idx < total; idx++) {
+      size_t rest = idx, mult = 1;
+      for (int i = 0; i < k.ncuts; i++) {
+        const size_t nd = rest % (size_t)s.rad[i];
+        rest /= (size_t)s.rad[i];
+        if (nd + 1 < (size_t)s.rad[i] && s.cxtable[idx + mult] < s.cxtable[idx]) k.cx_mono = 0;
+        mult *= (size_t)s.rad[i];
+      }
+    }
+  }
   // Montgomery images of w^i: w = 3^((p-1)/8) mod p, times 2^32
   auto mulm = [](uint64_t a, uint64_t b) { return (a * b) % P_MOD; };
   uint64_t w = 1, b = 3, e = (P_MOD - 1) / 8;

@@ -5,6 +5,14 @@
 
 namespace chz {
 
+// For n = 3 a lane group is the whole wave, so values shared by a group are wave-uniform: reading them through
+// readfirstlane keeps them in scalar registers (fewer vector registers live across the T bound).  Smaller groups
+// share a wave with other groups and keep such values per lane.
+template <int NQ> __device__ __forceinline__ int wuni(int v) {
+  if constexpr (NQ == 3) return __builtin_amdgcn_readfirstlane(v);
+  else return v;
+}
+
 enum : uint8_t { ST_DEAD = 0, ST_ALIVE = 1, ST_FOUND = 2, ST_W2 = 3, ST_W3 = 4, ST_NEEDT = 5, ST_OVF = 6 };
 constexpr uint32_t INFO_NOBOUND = 1u << 31;    // spec without bounds: every child is kept
 
@@ -200,6 +208,21 @@ template <int NQ> __device__ __forceinline__ int cx_bound(int4 s, int le, const 
     if (need > mx) mx = need;
   }
   needs = out;
+  return sp.have_table ? sp.cxtable[idx] : mx;
+}
+
+// Lower bound on a child's cut bound from its parent's packed needs: each need drops by at most the gate's
+// per-cut delta (operator-Schmidt ranks multiply), and the table is non-decreasing in every need.
+__device__ __forceinline__ int cx_lower(const KSpec &sp, uint32_t pneeds, uint32_t cdel) {
+  int idx = 0, mult = 1, mx = 0;
+  for (int i = 0; i < sp.ncuts; i++) {
+    const int dl = (int)((cdel >> (3 * i)) & 7u);
+    int nd = (int)((pneeds >> (3 * i)) & 7u) - dl;
+    nd = dl == 7 || nd < 0 ? 0 : nd;
+    idx += nd * mult;
+    mult *= sp.rad[i];
+    if (nd > mx) mx = nd;
+  }
   return sp.have_table ? sp.cxtable[idx] : mx;
 }
 

@@ -221,15 +221,23 @@ bool same_gate_set(const HostSpec &a, const HostSpec &b) {
   return true;
 }
 
-// A gate is Clifford when it maps every Pauli operator to a multiple of one Pauli operator under conjugation:
-// for each Pauli B = X^x Z^z, C = g B g^dagger has a nonzero overlap Tr(A^dagger C) with exactly one Pauli A
-// (the Paulis are an orthogonal basis).  Exact in Z[w]; the scale sqrt2^(2 gk) of g g^dagger cancels.
+// Pauli transfer matrix of a gate, R_ab = Tr(P_a g P_b g^dagger) / L with g = M / sqrt2^gk: each entry is
+// t_ab / sqrt2^(2 (nq + gk)) with t_ab = Tr(P_a M P_b M^dagger) in Z[w].  Returns the largest sqrt2-denominator
+// exponent over the entries (0 exactly for Clifford gates: an orthogonal matrix over Z[sqrt2] whose Galois
+// conjugate is orthogonal too has entries 0, +-1 only), or -1 when M is not L x L.
+static int sqrt2_valuation(Zq x) {
+  int v = 0;
+  while (zq_sqrt2_divides(x)) { x = zq_div_sqrt2(x); v++; }
+  return v;
+}
 static Zq zq_conj(const Zq &x) { return {x.a, -x.d, -x.c, -x.b}; }
-bool gate_is_clifford(const HostGate &g) {
+int gate_ptm_sde(const HostGate &g) {
   const int L = 1 << g.nq;
   const std::vector<Zq> &M = g.fwd;
-  if ((int)M.size() != L * L) return false;
+  if ((int)M.size() != L * L) return -1;
+  const int scale = 2 * (g.nq + g.gk);
   std::vector<Zq> MB(L * L), C(L * L);
+  int sde = 0;
   for (int bx = 0; bx < L; bx++)
     for (int bz = 0; bz < L; bz++) {
       // (M B)[r][c] = M[r][c ^ bx] (-1)^(bz . c): B |c> = (-1)^(bz . c) |c ^ bx>
@@ -239,8 +247,7 @@ bool gate_is_clifford(const HostGate &g) {
           if (__builtin_popcount(bz & c) & 1) v = {-v.a, -v.b, -v.c, -v.d};
           MB[r * L + c] = v;
         }
-      // C = (M B) M^dagger
-      for (int r = 0; r < L; r++)
+      for (int r = 0; r < L; r++)       // C = (M B) M^dagger
         for (int c = 0; c < L; c++) {
           Zq acc{0, 0, 0, 0};
           for (int j = 0; j < L; j++) {
@@ -249,21 +256,76 @@ bool gate_is_clifford(const HostGate &g) {
           }
           C[r * L + c] = acc;
         }
-      int hits = 0;
       for (int ax = 0; ax < L; ax++)
         for (int az = 0; az < L; az++) {
-          // Tr(A^dagger C) = sum_c <c ^ ax| C |c> (-1)^(az . c)
-          Zq tr{0, 0, 0, 0};
+          Zq tr{0, 0, 0, 0};          // Tr(A^dagger C) = sum_c <c ^ ax| C |c> (-1)^(az . c), up to a unit
           for (int c = 0; c < L; c++) {
             Zq v = C[(c ^ ax) * L + c];
             if (__builtin_popcount(az & c) & 1) v = {-v.a, -v.b, -v.c, -v.d};
             tr = {tr.a + v.a, tr.b + v.b, tr.c + v.c, tr.d + v.d};
           }
-          hits += !zq_zero(tr);
+          if (zq_zero(tr)) continue;
+          const int need = scale - sqrt2_valuation(tr);
+          if (need > sde) sde = need;
         }
-      if (hits != 1) return false;
     }
-  return true;
+  return sde;
+}
+
+bool gate_is_clifford(const HostGate &g) { return gate_ptm_sde(g) == 0; }
+
+// Operator-Schmidt rank over F_p (w -> 3^((p-1)/8)) of the gate's local matrix across the split of its qubits
+// into those in cut mask A and the rest: rank of the realigned matrix R[(oa, ia)][(ob, ib)] = M[(oa, ob)][(ia, ib)].
+int gate_cut_rank(const HostGate &g, int A) {
+  const int L = 1 << g.nq;
+  if ((int)g.fwd.size() != L * L) return -1;
+  const uint64_t P = kPrime;
+  auto mulm = [&](uint64_t a, uint64_t b) { return a * b % P; };
+  auto powm = [&](uint64_t b, uint64_t e) { uint64_t r = 1; for (b %= P; e; e >>= 1, b = mulm(b, b)) if (e & 1) r = mulm(r, b); return r; };
+  const uint64_t w = powm(3, (P - 1) / 8);
+  const uint64_t wp[4] = {1, w, mulm(w, w), mulm(mulm(w, w), w)};
+  auto img = [&](const Zq &z) {
+    const int64_t cf[4] = {z.a, z.b, z.c, z.d};
+    uint64_t s = 0;
+    for (int i = 0; i < 4; i++) s = (s + mulm((uint64_t)(((cf[i] % (int64_t)P) + (int64_t)P) % (int64_t)P), wp[i])) % P;
+    return s;
+  };
+  int la = 0;                                   // local qubit j of the gate is in A
+  for (int j = 0; j < g.nq; j++) if ((A >> g.q[j]) & 1) la |= 1 << j;
+  const int na = __builtin_popcount(la), nb = g.nq - na;
+  if (!na || !nb) return 1;
+  const int R = 1 << (2 * na), Cc = 1 << (2 * nb);
+  std::vector<uint64_t> m((size_t)R * Cc, 0);
+  auto split = [&](int x, int &xa, int &xb) {
+    xa = xb = 0;
+    int ia = 0, ib = 0;
+    for (int j = 0; j < g.nq; j++) {
+      const int bit = (x >> j) & 1;
+      if ((la >> j) & 1) xa |= bit << ia++; else xb |= bit << ib++;
+    }
+  };
+  for (int o = 0; o < L; o++)
+    for (int i = 0; i < L; i++) {
+      int oa, ob, ia, ib;
+      split(o, oa, ob);
+      split(i, ia, ib);
+      m[(size_t)((oa << na) | ia) * Cc + ((ob << nb) | ib)] = img(g.fwd[o * L + i]);
+    }
+  int rank = 0;
+  for (int col = 0; col < Cc && rank < R; col++) {
+    int piv = -1;
+    for (int r = rank; r < R; r++) if (m[(size_t)r * Cc + col]) { piv = r; break; }
+    if (piv < 0) continue;
+    for (int c = 0; c < Cc; c++) std::swap(m[(size_t)piv * Cc + c], m[(size_t)rank * Cc + c]);
+    const uint64_t inv = powm(m[(size_t)rank * Cc + col], P - 2);
+    for (int r = 0; r < R; r++) {
+      if (r == rank || !m[(size_t)r * Cc + col]) continue;
+      const uint64_t f = mulm(m[(size_t)r * Cc + col], inv);
+      for (int c = 0; c < Cc; c++) m[(size_t)r * Cc + c] = (m[(size_t)r * Cc + c] + P - mulm(f, m[(size_t)rank * Cc + c])) % P;
+    }
+    rank++;
+  }
+  return rank;
 }
 
 }  // namespace chz

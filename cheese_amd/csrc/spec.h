@@ -58,5 +58,9 @@ bool same_gate_set(const HostSpec &a, const HostSpec &b);
 
 // exact: the gate maps every Pauli operator to a multiple of a Pauli operator under conjugation
 bool gate_is_clifford(const HostGate &g);
+// largest sqrt2-denominator exponent of the gate's Pauli transfer matrix (0: Clifford; 1: T), -1 if malformed
+int gate_ptm_sde(const HostGate &g);
+// operator-Schmidt rank over F_p of the gate across cut mask A (1 when all its qubits are on one side)
+int gate_cut_rank(const HostGate &g, int A);
 
 }  // namespace chz

@@ -1,0 +1,100 @@
+"""Host analysis of candidate gates (cheese_amd/csrc/spec.cpp, built here with g++ through tests/host_shim.cpp):
+the Pauli-transfer-matrix exponent that bounds how far a child's T bound can move from its parent's, the Clifford
+test (exponent 0) under which the T bound is inherited exactly, and the per-cut operator-Schmidt ranks that bound
+how far a cut need can drop.  The oracle is the checker: its T bound and F_p cut rank of the gate's own matrix.
+CPU only."""
+import ctypes
+import os
+import subprocess
+
+import pytest
+
+from cheese_amd import workloads as W
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+ROOT = os.path.dirname(HERE)
+
+# local matrices as {(row, col): w-power} with a sqrt2 exponent (bit j of an index = local qubit j)
+EXTRA = {
+    "cs": (0, {(0, 0): 0, (1, 1): 0, (2, 2): 0, (3, 3): 2}),
+    "ct": (0, {(0, 0): 0, (1, 1): 0, (2, 2): 0, (3, 3): 1}),
+    "ch": (1, {(1, 1): 0, (1, 3): 0, (3, 1): 0, (3, 3): 4}),     # control-0 block set to sqrt2 I in _local
+}
+
+
+@pytest.fixture(scope="module")
+def shim(tmp_path_factory):
+    out = tmp_path_factory.mktemp("shim") / "libhostshim.so"
+    subprocess.run(["g++", "-O1", "-std=c++17", "-shared", "-fPIC", "-o", str(out),
+                    os.path.join(HERE, "host_shim.cpp"), os.path.join(ROOT, "cheese_amd", "csrc", "spec.cpp")],
+                   check=True)
+    L = ctypes.CDLL(str(out))
+    L.shim_gate_info.restype = ctypes.c_int
+    L.shim_gate_info.argtypes = [ctypes.c_char_p, ctypes.c_long, ctypes.c_int, ctypes.POINTER(ctypes.c_int),
+                                 ctypes.c_int]
+    return L
+
+
+def _local(name):
+    if name in EXTRA:
+        gk, loc = EXTRA[name]
+        nq = 2
+    else:
+        gk, loc = W._GATES[name]
+        nq = {"cx": 2, "cz": 2, "ccx": 3}.get(name, 1)
+    Ld = 1 << nq
+    M = [[W.ZERO] * Ld for _ in range(Ld)]
+    for (r, c), p in loc.items():
+        M[r][c] = W._wpow(p)
+    if name == "ch":   # control 0 (local bit 0 clear): identity times sqrt2 = w - w^3, scaled back by gk = 1
+        for r in (0, 2):
+            M[r][r] = (0, 1, 0, -1)
+    return nq, gk, M
+
+
+def _conj(z):
+    return (z[0], -z[3], -z[2], -z[1])
+
+
+def _spec(nq, gk, M):
+    Ld = 1 << nq
+    fwd = " ".join(" ".join(map(str, M[r][c])) for r in range(Ld) for c in range(Ld))
+    inv = " ".join(" ".join(map(str, _conj(M[c][r]))) for r in range(Ld) for c in range(Ld))
+    N = Ld
+    ident = "".join(" ".join(" ".join(map(str, W.ONE if i == j else W.ZERO)) for j in range(N)) + "\n"
+                    for i in range(N))
+    qs = " ".join(str(q) for q in range(nq))
+    cuts = "".join(f"1 {q}\n" for q in range(nq))
+    return (f"cheese-spec 1\nn {nq}\nmode unitary\nphase exact\ndepth 1\nstandard 1\ngates 1\n"
+            f"L_g_{'_'.join(str(q) for q in range(nq))} {nq} {qs} {gk} {fwd} {gk} {inv}\n"
+            f"bad2 0\nbad3 0\ntarget 0\n{ident}cuts {nq}\n{cuts}cxtable 0\nend\n")
+
+
+@pytest.mark.parametrize("name", ["x", "z", "h", "s", "sdg", "t", "tdg", "cx", "cz", "ccx", "cs", "ct", "ch"])
+def test_gate_analysis_matches_the_oracle(shim, oracle_lib, name):
+    nq, gk, M = _local(name)
+    spec = _spec(nq, gk, M).encode()
+    out = (ctypes.c_int * 8)()
+    nc = shim.shim_gate_info(spec, len(spec), 0, out, 8)
+    assert nc == nq
+    flat = [c for row in M for z in row for c in z]
+    buf = (ctypes.c_longlong * len(flat))(*flat)
+    sde = oracle_lib.orc_bound(nq, gk, buf, 0, 0)         # T bound of the gate's matrix = its PTM exponent
+    assert out[1] == sde, (name, out[1], sde)
+    assert out[0] == (1 if sde == 0 else 0)
+    for q in range(nq):
+        assert out[2 + q] == oracle_lib.orc_bound(nq, gk, buf, 4, 1 << q), (name, q)
+
+
+def test_known_exponents_and_ranks(shim):
+    """closed forms: Clifford gates 0, T 1, CCX and CS 2 (PTM entries 1/2); CX and CCX have Schmidt rank 2 across
+    any cut they straddle"""
+    want = {"h": 0, "s": 0, "cx": 0, "cz": 0, "t": 1, "tdg": 1, "ccx": 2, "cs": 2}
+    for name, e in want.items():
+        nq, gk, M = _local(name)
+        spec = _spec(nq, gk, M).encode()
+        out = (ctypes.c_int * 8)()
+        assert shim.shim_gate_info(spec, len(spec), 0, out, 8) == nq
+        assert out[1] == e, name
+        if name in ("cx", "ccx"):
+            assert all(out[2 + q] == 2 for q in range(nq)), name

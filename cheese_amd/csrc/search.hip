@@ -66,15 +66,13 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(WPE))) void
                                                                                          long long np) {
   constexpr int N = 1 << NQ, E = N * N, GS = E, CPW = 64 / GS;
   __shared__ uint2 stage_x[TS_STRIDE];
-  __shared__ uint32_t stage_p[6 * TS_STRIDE];
+  __shared__ uint32_t stage_p[8 * TS_STRIDE];
   __shared__ int4 spar[256];                   // each group's parent residual
-  __shared__ int8_t sph[MAXG * 64];            // unit phase table of every candidate
   // per candidate and entry (g * 64 + e), the child entry's terms for unit gates with at most two per column:
   // bits 0-2 source column x0, 3-5 power p0 of w, 6-8 x1, 9-11 p1, bit 12 two terms; bit 15: use the table loop
   __shared__ uint16_t sdesc[MAXG * 64];
   __shared__ GateHdr shdr[MAXG];
   __shared__ uint8_t spos[NQ == 3 ? 16 * 64 : 1];   // n = 3: position of entry le under symmetry y (y * 64 + le)
-  for (int i = threadIdx.x; i < sp.G * 64; i += blockDim.x) sph[i] = sp.gates[i >> 6].ph[i & 63];
   for (int i = threadIdx.x; i < sp.G * GS; i += blockDim.x) {
     const int g = i / GS, e = i % GS, ec = e & (N - 1);
     const DevGate &d = sp.gates[g];
@@ -125,7 +123,9 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(WPE))) void
   const long long nwaves = ((long long)gridDim.x * blockDim.x) >> 6;
   const int G = sp.G;
   // per-instance tallies live in each group's leader lane and are flushed when the group's instance changes
-  // (32-bit: one wave's share of a launch is far below 2^32 children; fewer registers live across the T bound)
+  // (32-bit: one wave's share of a launch is far below 2^32 children).  Every lane of a group holds its group's
+  // tallies, updated outside lane-divergent code, so for n = 3 (one group per wave) they stay wave-uniform and
+  // live in scalar registers.
   uint32_t n_w2 = 0, n_w3 = 0, n_dead = 0, n_found = 0, n_ovf = 0, n_alive = 0, n_eval = 0, n_t = 0;
   uint32_t n_cheap = 0, n_cuts = 0, n_keys = 0;
   int cur = -1;
@@ -190,7 +190,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(WPE))) void
         if (h.nq > 1) lc |= ((c >> h.q1) & 1) << 1;
         if (h.nq > 2) lc |= ((c >> h.q2) & 1) << 2;
         const int cb = c & ~h.mask;
-        const int8_t *ph = sph + g * 64;
+        const int8_t *ph = sp.gates[g].ph;    // table loop only: gates with more than two terms per column
         int4 s;
         bool of = false;
         const uint32_t desc = sdesc[g * 64 + le];
@@ -232,7 +232,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(WPE))) void
           s = make_int4((int)va, (int)vb, (int)vc, (int)vd);
         }
         int k = pk + h.ik;
-        n_eval += le == 0;
+        n_eval++;
         if (gany<GS>(of)) stt = ST_OVF;
         else {
           while (k > 0 && gall<GS>(zhalf(s))) { s = zdown(s); k--; }
@@ -243,7 +243,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(WPE))) void
               int hh, supp = 0, c2 = 0;
               if (sp.lite) hh = k;
               else {
-                n_cheap += le == 0;
+                n_cheap++;
                 const uint64_t nzb = gballot<GS>(!zzero(s));
                 const int hs = h_struct<NQ>(nzb);
                 hh = k > hs ? k : hs;
@@ -264,7 +264,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(WPE))) void
                 if (!sp.lite) {
                   uint32_t needs = (pbw & ~BW_CUTS) >> 8;
                   const uint32_t keepc = (pbw & BW_CUTS) ? (uint32_t)h.loc : 0u;
-                  n_cuts += le == 0 ? sp.ncuts - __popc(keepc & ((1u << sp.ncuts) - 1)) : 0;
+                  n_cuts += sp.ncuts - __popc(keepc & ((1u << sp.ncuts) - 1));
                   c2 = cx_bound<NQ>(s, le, sp, keepc, needs);
                   bw = (needs << 8 & ~BW_CUTS) | (sp.ncuts <= BW_MAXCUTS ? BW_CUTS : 0u) | BW_TUNK;
                 }
@@ -274,7 +274,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(WPE))) void
                   // alive whatever the exact T; its children (budget 0) need no inherited T bound
                   alive = true;
                 } else if (!texact) {
-                  n_t += le == 0;
+                  n_t++;
                   if (NQ + k <= 16) {
                     __builtin_amdgcn_wave_barrier();
                     t_stage<NQ>(st, s, le);
@@ -291,7 +291,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(WPE))) void
               if (alive && texact && T < (int)BW_TUNK) bw = (bw & ~BW_TUNK) | (uint32_t)T;
             }
             if (alive && keep) {
-              n_keys += le == 0;
+              n_keys++;
               if constexpr (NQ == 3) {
                 key = sp.nsym <= 16 ? class_key3(s, k, lane, spos, cjm, sp.nsym) : class_key<NQ>(s, k, le, sp);
               } else {
@@ -303,27 +303,23 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(WPE))) void
           }
         }
       }
+      n_w2 += stt == ST_W2;
+      n_w3 += stt == ST_W3;
+      n_dead += stt == ST_DEAD;
+      n_ovf += stt == ST_OVF;
+      n_alive += stt == ST_ALIVE;
+      n_found += stt == ST_FOUND;
       if (le == 0) {
         a.status[cc] = stt;
-        switch (stt) {
-          case ST_W2: n_w2++; break;
-          case ST_W3: n_w3++; break;
-          case ST_DEAD: n_dead++; break;
-          case ST_OVF: n_ovf++; break;
-          case ST_ALIVE:
-            n_alive++;
-            if (keep) { a.keys[cc] = key; a.info[cc] = bw; }
-            break;
-          case ST_FOUND:
-            n_found++;
-            atomicMin(&a.icnt[(size_t)inst * IC_N + IC_FIRST], (unsigned long long)(a.off * G + cc));
-            break;
-        }
+        if (stt == ST_ALIVE && keep) { a.keys[cc] = key; a.info[cc] = bw; }
+        if (stt == ST_FOUND)
+          atomicMin(&a.icnt[(size_t)inst * IC_N + IC_FIRST], (unsigned long long)(a.off * G + cc));
       }
     }
     __builtin_amdgcn_wave_barrier();
   }
   flush();
+  if (le != 0) n_eval = n_t = n_cheap = n_cuts = n_keys = 0;     // one copy per group
   n_eval = gsum<64>(n_eval);
   n_t = gsum<64>(n_t);
   n_cheap = gsum<64>(n_cheap);
@@ -826,9 +822,9 @@ template <int NQ> static void batch_step(chz_ctx *c) {
     }
     c->ev.begin(K_EXPAND, c->stream);
     switch (sc.var) {
-      case 0: hipLaunchKernelGGL((k_level<NQ, 2>), grid, dim3(256), 0, c->stream, ks, ea, cm); break;
-      case 2: hipLaunchKernelGGL((k_level<NQ, 4>), grid, dim3(256), 0, c->stream, ks, ea, cm); break;
-      default: hipLaunchKernelGGL((k_level<NQ, 3>), grid, dim3(256), 0, c->stream, ks, ea, cm); break;
+      case 0: hipLaunchKernelGGL((k_level<NQ, 4>), grid, dim3(256), 0, c->stream, ks, ea, cm); break;
+      case 2: hipLaunchKernelGGL((k_level<NQ, 6>), grid, dim3(256), 0, c->stream, ks, ea, cm); break;
+      default: hipLaunchKernelGGL((k_level<NQ, 5>), grid, dim3(256), 0, c->stream, ks, ea, cm); break;
     }
     c->ev.end(c->stream);
     if (c->tuning) {
@@ -1164,6 +1160,9 @@ int chz_open(const char *spec_text, size_t len, const chz_opts *opts, chz_ctx **
   if (c->hs.truncated) { chz::set_err("malformed spec"); return 2; }
   if ((int)c->hs.target.size() != c->hs.N() * c->hs.N()) { chz::set_err("spec has no target"); return 2; }
   c->grp = chz::build_symmetry(c->hs, o.sym != 0);
+  // default occupancy: 5 waves per SIMD fits the n = 3 kernel's registers; smaller groups spill there, so 4
+  if (c->hs.n < 3)
+    for (auto &sc : c->sched) sc.var = 0;
   try {
     c->device = o.device;
     HIPCHK(hipSetDevice(o.device));
@@ -1246,7 +1245,7 @@ int chz_tune(chz_ctx *c, int reps, char *out, size_t cap) {
     c->sched[b] = cands[best];
     char buf[128];
     std::snprintf(buf, sizeof buf, "%s\"%d\": {\"waves_per_eu\": %d, \"grid_cap\": %d, \"split_waves\": %d, \"ms\": %.4f}",
-                  first ? "" : ", ", b, cands[best].var + 2, cands[best].cap, cands[best].tw, sum[best][b] / num[best][b]);
+                  first ? "" : ", ", b, cands[best].var + 4, cands[best].cap, cands[best].tw, sum[best][b] / num[best][b]);
     js += buf;
     first = false;
   }

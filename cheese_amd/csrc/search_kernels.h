@@ -267,9 +267,11 @@ __device__ __forceinline__ u16x2 pk(int lo, int hi) { u16x2 r; r.x = (unsigned s
 __device__ __forceinline__ uint32_t bits(u16x2 v) { return (uint32_t)v.x | ((uint32_t)v.y << 16); }
 
 // Per-entry operands of the packed transform, staged in LDS by the lane owning entry e, one array per field
-// (a group's entries are read at a stride of 8 bytes, free of bank conflicts):
-// x[e] = (a, b | c, d) of S[e] mod 2^16, and p[i * TS_STRIDE + e] the six packed multipliers of conj(S[e]) with
-// which S[x] conj(S[e]) = (Qa, Qb | Qc, Qd) = xa P1 + xb P2 + xc P3 + xd P4 | xa P5 + xb P6 + xc P1 + xd P2.
+// (a group's entries are read at a stride of 8 bytes, free of bank conflicts): x[e] = (a, b | c, d) of S[e] mod
+// 2^16, and p[i * TS_STRIDE + e] eight packed multipliers of conj(S[e]) = (a, -d, -c, -b).  With
+// Q = S[x] conj(S[e]) = (Qa, Qb, Qc, Qd) the transform needs only Qa, Qc and Qb -+ Qd (below), in two passes:
+//   (Qa | Qc)           = xa (a | -c)    + xb (b | -d)    + xc (c | a)     + xd (d | b)
+//   (Qb - Qd | Qb + Qd) = xa (b-d | -b-d) + xb (a+c | a-c) + xc (b+d | b-d) + xd (c-a | c+a)
 constexpr int TS_STRIDE = 256;      // entries per field: one per thread of a block
 struct TStage {
   uint2 *x;
@@ -279,12 +281,14 @@ struct TStage {
 template <int NQ> __device__ __forceinline__ void t_stage(TStage st, int4 s, int le) {
   const int a = s.x, b = s.y, c = s.z, d = s.w;
   st.x[le] = make_uint2(bits(pk(a, b)), bits(pk(c, d)));
-  st.p[0 * TS_STRIDE + le] = bits(pk(a, -d));
-  st.p[1 * TS_STRIDE + le] = bits(pk(b, a));
-  st.p[2 * TS_STRIDE + le] = bits(pk(c, b));
-  st.p[3 * TS_STRIDE + le] = bits(pk(d, c));
-  st.p[4 * TS_STRIDE + le] = bits(pk(-c, -b));
-  st.p[5 * TS_STRIDE + le] = bits(pk(-d, -c));
+  st.p[0 * TS_STRIDE + le] = bits(pk(a, -c));
+  st.p[1 * TS_STRIDE + le] = bits(pk(b, -d));
+  st.p[2 * TS_STRIDE + le] = bits(pk(c, a));
+  st.p[3 * TS_STRIDE + le] = bits(pk(d, b));
+  st.p[4 * TS_STRIDE + le] = bits(pk(b - d, -b - d));
+  st.p[5 * TS_STRIDE + le] = bits(pk(a + c, a - c));
+  st.p[6 * TS_STRIDE + le] = bits(pk(b + d, b - d));
+  st.p[7 * TS_STRIDE + le] = bits(pk(c - a, c + a));
 }
 
 __device__ __forceinline__ u16x2 as16(uint32_t v) { u16x2 r; r.x = (unsigned short)v; r.y = (unsigned short)(v >> 16); return r; }
@@ -307,37 +311,39 @@ template <int NQ> __device__ __forceinline__ int t_bound16(TStage st, int4 s, in
   asm volatile("" : "+v"(f));
   const int j = 31 - __clz(f);
   const int lowm = (1 << j) - 1;
-  u16x2 qab[H], qcd[H];
+  // pass 0 transforms (Qa | Qc), pass 1 (Qb - Qd | Qb + Qd); one pass at a time keeps H packed values live
+  uint32_t acc[2];
+#pragma unroll 1
+  for (int pass = 0; pass < 2; pass++) {
+    const uint32_t *pm = st.p + pass * 4 * TS_STRIDE;
+    // the addresses below depend on a value the compiler cannot see through, so they are formed per pass
+    // instead of being hoisted out of the pass loop (64 more live registers)
+    int fp = f, lm = lowm;
+    asm volatile("" : "+v"(fp), "+v"(lm));
+    u16x2 q[H];
 #pragma unroll
-  for (int i = 0; i < H; i++) {
-    const int e = ((i & ~lowm) << 1) | (i & lowm);     // i with a 0 inserted at bit j
-    const uint2 xv = st.x[e ^ f];
-    const u16x2 x01 = as16(xv.x), x23 = as16(xv.y);
-    const u16x2 xa = x01.xx, xb = x01.yy, xc = x23.xx, xd = x23.yy;
-    const u16x2 p1 = as16(st.p[0 * TS_STRIDE + e]), p2 = as16(st.p[1 * TS_STRIDE + e]),
-                p3 = as16(st.p[2 * TS_STRIDE + e]), p4 = as16(st.p[3 * TS_STRIDE + e]),
-                p5 = as16(st.p[4 * TS_STRIDE + e]), p6 = as16(st.p[5 * TS_STRIDE + e]);
-    qab[i] = xa * p1 + xb * p2 + xc * p3 + xd * p4;
-    qcd[i] = xa * p5 + xb * p6 + xc * p1 + xd * p2;
+    for (int i = 0; i < H; i++) {
+      const int e = ((i & ~lm) << 1) | (i & lm);         // i with a 0 inserted at bit j
+      const uint2 xv = st.x[e ^ fp];
+      const u16x2 x01 = as16(xv.x), x23 = as16(xv.y);
+      q[i] = x01.xx * as16(pm[e]) + x01.yy * as16(pm[TS_STRIDE + e]) + x23.xx * as16(pm[2 * TS_STRIDE + e]) +
+             x23.yy * as16(pm[3 * TS_STRIDE + e]);
+    }
+#pragma unroll
+    for (int h = 1; h < H; h <<= 1)
+#pragma unroll
+      for (int i = 0; i < H; i++)
+        if (!(i & h)) {
+          const u16x2 u = q[i], v = q[i | h];
+          q[i] = u + v;
+          q[i | h] = u - v;
+        }
+    uint32_t o = 0;
+#pragma unroll
+    for (int i = 0; i < H; i++) o |= bits(q[i]);
+    acc[pass] = (o & 0xffffu) | (o >> 16);
   }
-#pragma unroll
-  for (int h = 1; h < H; h <<= 1)
-#pragma unroll
-    for (int i = 0; i < H; i++)
-      if (!(i & h)) {
-        const u16x2 u = qab[i], v = qab[i | h];
-        qab[i] = u + v;
-        qab[i | h] = u - v;
-        const u16x2 x = qcd[i], y = qcd[i | h];
-        qcd[i] = x + y;
-        qcd[i | h] = x - y;
-      }
-  uint32_t ev = 0, od = 0;          // low halves: a | c (even part); high halves: (b - d) | (b + d) (odd part)
-#pragma unroll
-  for (int i = 0; i < H; i++) {
-    ev |= bits(qab[i]) | bits(qcd[i]);
-    od |= bits(qab[i] - qcd[i]) | bits(qab[i] + qcd[i]);
-  }
+  const uint32_t ev = acc[0], od = acc[1];     // ev: a, c (even part); od: b - d, b + d (odd part)
   // f = 0: |S[e]|^2 = r + y sqrt2 with r = a^2 + b^2 + c^2 + d^2, y = ab + bc + cd - da
   uint32_t r = (uint32_t)(s.x * s.x + s.y * s.y + s.z * s.z + s.w * s.w);
   uint32_t y = (uint32_t)(s.x * s.y + s.y * s.z + s.z * s.w - s.w * s.x);
@@ -346,7 +352,7 @@ template <int NQ> __device__ __forceinline__ int t_bound16(TStage st, int4 s, in
     const uint32_t ro = __shfl_xor(r, h, GS), yo = __shfl_xor(y, h, GS);
     if (le & h) { r = ro - r; y = yo - y; } else { r = r + ro; y = y + yo; }
   }
-  uint32_t accE = ev & 0xffffu, accF = (od >> 16) | (y & 0xffffu), accG = r & 0xffffu;
+  uint32_t accE = ev, accF = od | (y & 0xffffu), accG = r & 0xffffu;
   accE = gor<GS>(accE);
   accF = gor<GS>(accF);
   accG = gor<GS>(accG);

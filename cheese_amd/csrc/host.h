@@ -5,6 +5,8 @@
 
 #include <algorithm>
 #include <chrono>
+#include <map>
+#include <mutex>
 #include <string>
 #include <vector>
 
@@ -25,18 +27,81 @@ struct DeviceError {};
     }                                                                                  \
   } while (0)
 
-template <typename T> struct DBuf {     // grow-only device array
+// Process-wide cache of device blocks by (device, size class): contexts opened and closed one after another
+// (one search per context, as the command line does) reuse memory instead of paying hipMalloc/hipFree each time.
+// Size classes are 2^k rounded up to a quarter step (at most 25 % slack).  A block enters the cache only when no
+// kernel can still use it: after a device synchronisation on growth, and from closed contexts whose stream has
+// finished.  At most kMaxCached bytes stay cached; the rest is freed.
+struct DevPool {
+  static constexpr size_t kMaxCached = 16ull << 30;
+  std::mutex mu;
+  std::multimap<std::pair<int, size_t>, void *> free_blocks;
+  size_t cached = 0;
+  static size_t size_class(size_t b) {
+    if (b <= 256) return 256;
+    int lg = 63 - __builtin_clzll((unsigned long long)b);
+    const size_t step = (size_t)1 << (lg - 2);
+    return (b + step - 1) / step * step;
+  }
+  void *get(size_t bytes, size_t &got) {
+    int dev = 0;
+    HIPCHK(hipGetDevice(&dev));
+    got = size_class(bytes);
+    {
+      std::lock_guard<std::mutex> g(mu);
+      auto it = free_blocks.find({dev, got});
+      if (it != free_blocks.end()) {
+        void *p = it->second;
+        free_blocks.erase(it);
+        cached -= got;
+        return p;
+      }
+    }
+    void *p = nullptr;
+    if (hipMalloc(&p, got) != hipSuccess) {
+      (void)hipGetLastError();
+      trim();                       // give the cache back to the device and try once more
+      HIPCHK(hipMalloc(&p, got));
+    }
+    return p;
+  }
+  void put(void *p, size_t got) {
+    if (!p) return;
+    int dev = 0;
+    if (hipGetDevice(&dev) != hipSuccess) { (void)hipFree(p); return; }
+    std::lock_guard<std::mutex> g(mu);
+    if (cached + got > kMaxCached) { (void)hipFree(p); return; }
+    free_blocks.insert({{dev, got}, p});
+    cached += got;
+  }
+  void trim() {
+    std::lock_guard<std::mutex> g(mu);
+    for (auto &kv : free_blocks) (void)hipFree(kv.second);
+    free_blocks.clear();
+    cached = 0;
+  }
+};
+inline DevPool &dev_pool() { static DevPool *p = new DevPool(); return *p; }   // never destroyed: no frees at exit
+
+template <typename T> struct DBuf {     // grow-only device array on the block cache
   T *p = nullptr;
-  size_t cap = 0;
+  size_t cap = 0, bytes = 0;
   void need(size_t n) {
     if (n <= cap) return;
-    if (p) HIPCHK(hipFree(p));
-    p = nullptr;
-    const size_t c = std::max<size_t>(n, cap + cap / 2);
-    HIPCHK(hipMalloc((void **)&p, std::max<size_t>(c, 1) * sizeof(T)));
-    cap = c;
+    const size_t c = std::max<size_t>(std::max<size_t>(n, cap + cap / 2), 1);
+    size_t got = 0;
+    T *np = (T *)dev_pool().get(c * sizeof(T), got);
+    if (p) {
+      // kernels already queued may still read the old block
+      HIPCHK(hipDeviceSynchronize());
+      dev_pool().put(p, bytes);
+    }
+    p = np;
+    bytes = got;
+    cap = got / sizeof(T);
   }
-  void release() { if (p) (void)hipFree(p); p = nullptr; cap = 0; }
+  // the caller guarantees that no queued kernel still uses the block (its stream has synchronised)
+  void release() { if (p) dev_pool().put(p, bytes); p = nullptr; cap = 0; bytes = 0; }
 };
 
 inline double now_s() { return std::chrono::duration<double>(std::chrono::steady_clock::now().time_since_epoch()).count(); }

@@ -429,11 +429,15 @@ __global__ __launch_bounds__(256) void k_dedupe_runs(KSpec sp, DedupeArgs a) {
     const long long i0 = a.runs[ri];
     const uint32_t key = a.k[i0];
     int nrep = 0;
+    // the first two representatives' residuals stay in registers: a run is almost always one class, whose
+    // members are then each rebuilt once and compared with the cached first representative
+    int4 c0 = make_int4(0, 0, 0, 0), c1 = c0;
+    int ck0 = 0, ck1 = 0;
     for (long long j = i0; j < a.n && a.k[j] == key; j++) {
-      const uint32_t rank = a.v[j];
+      const uint32_t rank = (uint32_t)wuni<NQ>((int)a.v[j]);
       const uint64_t o = a.surv_o[rank], key64 = a.surv_key[rank];
       const long long pi = (long long)(o / (uint64_t)sp.G);
-      const int g = (int)(o % (uint64_t)sp.G);
+      const int g = wuni<NQ>((int)(o % (uint64_t)sp.G));
       int4 s;
       int ks;
       make_child<NQ>(a.res + pi * E, a.pk[pi], sp.gates[g], le, s, ks);
@@ -446,12 +450,16 @@ __global__ __launch_bounds__(256) void k_dedupe_runs(KSpec sp, DedupeArgs a) {
         if (a.pinst && a.pinst[rpi] != a.pinst[pi]) continue;
         int4 b;
         int kb;
-        make_child<NQ>(a.res + rpi * E, a.pk[rpi], sp.gates[(int)(ro % (uint64_t)sp.G)], le, b, kb);
+        if (r == 0) { b = c0; kb = ck0; }
+        else if (r == 1) { b = c1; kb = ck1; }
+        else make_child<NQ>(a.res + rpi * E, a.pk[rpi], sp.gates[(int)(ro % (uint64_t)sp.G)], le, b, kb);
         dup = same_class<NQ>(s, ks, b, kb, le, sp);
       }
       if (dup) { if (le == 0) a.kept[rank] = 0; }
       else if (nrep < MAXREP) {
         if (le == 0) rep_rank[grp][nrep] = rank;
+        if (nrep == 0) { c0 = s; ck0 = ks; }
+        else if (nrep == 1) { c1 = s; ck1 = ks; }
         nrep++;
       } else if (le == 0) atomicAdd(&a.cnt[C_REPOVF], 1ull);
       __builtin_amdgcn_wave_barrier();

@@ -1367,6 +1367,7 @@ int chz_tc_dedupe_counts(const chz_tc *c, double *out) {
 void chz_tc_free(chz_tc *c) {
   if (!c) return;
   (void)hipSetDevice(c->device);
+  (void)hipDeviceSynchronize();      // cached device blocks must be idle before they are reused
   c->tab.release();
   void *ps[] = {c->d_anti, c->d_plus, c->d_coef, c->d_rv, c->d_rw, c->d_troot_fp, c->d_hkey, c->d_hval};
   for (void *p : ps) if (p) (void)hipFree(p);

@@ -554,6 +554,7 @@ struct chz_ctx {
   hipStream_t stream = nullptr;
   chz::KSpec ks{};
   // spec on the device
+  chz::DBuf<uint8_t> spec_blob;     // the device tables below, one block
   chz::DevGate *d_gates = nullptr;
   uint8_t *d_bad2 = nullptr, *d_symperm = nullptr, *d_symconj = nullptr;
   uint32_t *d_bad3 = nullptr;
@@ -602,8 +603,9 @@ struct chz_ctx {
 
 namespace chz {
 
+// (the context's stream has synchronised: the blocks go back to the process-wide cache)
 static void free_tree(chz_ctx *c) {
-  for (auto &kv : c->tree_blocks) (void)hipFree(kv.first);
+  for (auto &kv : c->tree_blocks) dev_pool().put(kv.first, kv.second);
   c->tree_blocks.clear();
   c->tree_free.clear();
   c->tree_par.clear();
@@ -621,9 +623,9 @@ static void tree_alloc(chz_ctx *c, long long n, uint32_t *&tp, uint8_t *&tg) {
       break;
     }
   if (!blk) {
-    const size_t sz = need + need / 4 + 256;
-    HIPCHK(hipMalloc(&blk, sz));
-    c->tree_blocks[blk] = sz;
+    size_t got = 0;
+    blk = dev_pool().get(need + need / 4 + 256, got);
+    c->tree_blocks[blk] = got;
   }
   tp = (uint32_t *)blk;
   tg = (uint8_t *)(tp + std::max<long long>(n, 1));
@@ -1047,17 +1049,25 @@ static void upload_spec(chz_ctx *c) {
   std::vector<int4> goal(N * N);
   for (int i = 0; i < N * N; i++) goal[i] = make_int4((int)s.goal[i].a, (int)s.goal[i].b, (int)s.goal[i].c, (int)s.goal[i].d);
   std::vector<int> cx = s.cxtable.empty() ? std::vector<int>{0} : s.cxtable;
-  auto up = [&](void **dst, const void *src, size_t bytes) {
-    HIPCHK(hipMalloc(dst, std::max<size_t>(bytes, 1)));
-    if (bytes) HIPCHK(hipMemcpy(*dst, src, bytes, hipMemcpyHostToDevice));
+  // every table in one device block (256-byte aligned sections), one copy
+  std::vector<uint8_t> blob;
+  std::vector<std::pair<void **, size_t>> at;
+  auto add = [&](void **dst, const void *src, size_t bytes) {
+    const size_t off = (blob.size() + 255) & ~(size_t)255;
+    blob.resize(off + std::max<size_t>(bytes, 1), 0);
+    if (bytes) std::memcpy(blob.data() + off, src, bytes);
+    at.push_back({dst, off});
   };
-  up((void **)&c->d_gates, dg.data(), dg.size() * sizeof(DevGate));
-  up((void **)&c->d_bad2, s.bad2.data(), s.bad2.size());
-  up((void **)&c->d_bad3, s.bad3.data(), s.bad3.size() * 4);
-  up((void **)&c->d_goal, goal.data(), goal.size() * sizeof(int4));
-  up((void **)&c->d_cx, cx.data(), cx.size() * 4);
-  up((void **)&c->d_symperm, c->grp.bperm.data(), c->grp.bperm.size());
-  up((void **)&c->d_symconj, c->grp.conj.data(), c->grp.conj.size());
+  add((void **)&c->d_gates, dg.data(), dg.size() * sizeof(DevGate));
+  add((void **)&c->d_bad2, s.bad2.data(), s.bad2.size());
+  add((void **)&c->d_bad3, s.bad3.data(), s.bad3.size() * 4);
+  add((void **)&c->d_goal, goal.data(), goal.size() * sizeof(int4));
+  add((void **)&c->d_cx, cx.data(), cx.size() * 4);
+  add((void **)&c->d_symperm, c->grp.bperm.data(), c->grp.bperm.size());
+  add((void **)&c->d_symconj, c->grp.conj.data(), c->grp.conj.size());
+  c->spec_blob.need(blob.size());
+  HIPCHK(hipMemcpy(c->spec_blob.p, blob.data(), blob.size(), hipMemcpyHostToDevice));
+  for (auto &d : at) *d.first = c->spec_blob.p + d.second;
   KSpec &k = c->ks;
   std::memset(&k, 0, sizeof k);
   k.n = s.n; k.G = G; k.depth = c->depth; k.standard = s.standard; k.lite = s.lite; k.tdiv = s.tdiv;
@@ -1413,8 +1423,7 @@ void chz_close(chz_ctx *c) {
   c->status.release(); c->kept.release(); c->keys.release(); c->surv_o.release(); c->surv_key.release();
   c->surv_bw.release();
   c->info.release(); c->cnt.release(); c->bcount.release(); c->hist.release(); c->runs.release(); c->kidx.release();
-  void *ptrs[] = {c->d_gates, c->d_bad2, c->d_symperm, c->d_symconj, c->d_bad3, c->d_goal, c->d_cx};
-  for (void *p : ptrs) if (p) (void)hipFree(p);
+  c->spec_blob.release();
   c->ev.destroy();
   if (c->stream) (void)hipStreamDestroy(c->stream);
   delete c;

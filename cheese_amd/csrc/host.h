@@ -83,6 +83,33 @@ struct DevPool {
 };
 inline DevPool &dev_pool() { static DevPool *p = new DevPool(); return *p; }   // never destroyed: no frees at exit
 
+// untyped blocks from the cache for code that frees by pointer; dev_free requires that no queued kernel still
+// uses the block (synchronise first)
+struct LiveBlocks {
+  std::mutex mu;
+  std::map<void *, size_t> sizes;
+};
+inline LiveBlocks &live_blocks() { static LiveBlocks *l = new LiveBlocks(); return *l; }
+inline void *dev_alloc(size_t bytes) {
+  size_t got = 0;
+  void *p = dev_pool().get(std::max<size_t>(bytes, 1), got);
+  std::lock_guard<std::mutex> g(live_blocks().mu);
+  live_blocks().sizes[p] = got;
+  return p;
+}
+inline void dev_free(void *p) {
+  if (!p) return;
+  size_t got = 0;
+  {
+    std::lock_guard<std::mutex> g(live_blocks().mu);
+    auto it = live_blocks().sizes.find(p);
+    if (it == live_blocks().sizes.end()) { (void)hipFree(p); return; }
+    got = it->second;
+    live_blocks().sizes.erase(it);
+  }
+  dev_pool().put(p, got);
+}
+
 template <typename T> struct DBuf {     // grow-only device array on the block cache
   T *p = nullptr;
   size_t cap = 0, bytes = 0;

@@ -26,12 +26,14 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <cctype>
 #include <cmath>
 #include <cstdint>
 #include <cstdio>
 #include <cstring>
 #include <sstream>
 #include <string>
+#include <thread>
 #include <vector>
 
 #include "../../include/cheese_hip.h"
@@ -618,7 +620,8 @@ struct PathArgs {
   Forest Fs, Ft;
   int j;                       // search level of the hits
   long long n;
-  const uint32_t *node, *tnode;
+  const uint32_t *node;
+  const uint32_t *hitx;        // table node matched by each search node of level j
   const int *tlevel;
   int *out;                    // n x 2 MAXT: search rotations (levels 1..j), then table rotations (levels 1..c)
 };
@@ -628,7 +631,7 @@ __global__ void k_paths(PathArgs a) {
     int *o = a.out + q * 2 * MAXT;
     uint32_t i = a.node[q];
     for (int l = a.j; l >= 1; l--) { o[l - 1] = a.Fs.rot[l][i]; i = a.Fs.par[l][i]; }
-    i = a.tnode[q];
+    i = a.hitx[a.node[q]];
     for (int l = a.tlevel[q]; l >= 1; l--) { o[MAXT + l - 1] = a.Ft.rot[l][i]; i = a.Ft.par[l][i]; }
   }
 }
@@ -645,35 +648,60 @@ struct Spec {
 };
 
 // .tspec reader (src/tcount.cu:433-450): 2 malformed, 3 n outside 1..3
+// whitespace-separated tokens of a spec text, read in place (a batch parses hundreds of 64 x 64 channels)
+struct Tok {
+  const char *p, *end;
+  void skip() { while (p < end && std::isspace((unsigned char)*p)) p++; }
+  bool word(std::string &w) {
+    skip();
+    const char *b = p;
+    while (p < end && !std::isspace((unsigned char)*p)) p++;
+    w.assign(b, p);
+    return p > b;
+  }
+  template <typename T> bool num(T &v) {
+    skip();
+    if (p >= end) return false;
+    char *q = nullptr;
+    const long long x = std::strtoll(p, &q, 10);
+    if (q == p || q > end) return false;
+    p = q;
+    v = (T)x;
+    return true;
+  }
+};
+
 static int read_spec(const char *text, size_t len, Spec &s, std::string &err) {
-  std::istringstream in(std::string(text, len));
+  // strtoll stops at the terminating NUL of a std::string copy; texts from the C ABI need not end in one
+  const std::string buf(text, len);
+  Tok in{buf.c_str(), buf.c_str() + buf.size()};
   std::string tok;
-  in >> tok;
+  in.word(tok);
   if (tok != "tcount-spec") { err = "not a tcount spec"; return 2; }
   int ver = 0;
-  in >> ver;
+  in.num(ver);
   bool have_ch = false, have_rot = false;
-  while (in >> tok) {
+  while (in.word(tok)) {
     if (tok == "n") {
-      if (!(in >> s.n)) { err = "malformed n"; return 2; }
+      if (!in.num(s.n)) { err = "malformed n"; return 2; }
       if (s.n < 1 || s.n > 3) { err = "n = " + std::to_string(s.n) + " not supported (1 <= n <= 3)"; return 3; }
       s.P = 1 << (2 * s.n);
     } else if (tok == "channel") {
       if (!s.P) { err = "channel before n"; return 2; }
-      in >> s.e;
+      bool ok = in.num(s.e);
       s.x.resize(s.P * s.P);
       s.y.resize(s.P * s.P);
-      for (int i = 0; i < s.P * s.P; i++) in >> s.x[i] >> s.y[i];
-      if (!in) { err = "malformed channel"; return 2; }
+      for (int i = 0; i < s.P * s.P && ok; i++) ok = in.num(s.x[i]) && in.num(s.y[i]);
+      if (!ok) { err = "malformed channel"; return 2; }
       have_ch = true;
     } else if (tok == "rotations") {
       int m = 0;
-      in >> m;
-      if (!s.P || m != s.P - 1) { err = "malformed rotations"; return 2; }
+      if (!in.num(m) || !s.P || m != s.P - 1) { err = "malformed rotations"; return 2; }
       s.tr.resize(m * s.P);
       s.ts.resize(m * s.P);
-      for (int i = 0; i < m * s.P; i++) in >> s.tr[i] >> s.ts[i];
-      if (!in) { err = "malformed rotations"; return 2; }
+      bool ok = true;
+      for (int i = 0; i < m * s.P && ok; i++) ok = in.num(s.tr[i]) && in.num(s.ts[i]);
+      if (!ok) { err = "malformed rotations"; return 2; }
       have_rot = true;
     } else if (tok == "end") {
       break;
@@ -712,19 +740,18 @@ struct Levels {
   std::vector<uint64_t *> keys;
   std::vector<long long> count;
   void release() {
-    for (auto p : par) if (p) (void)hipFree(p);
-    for (auto p : inst) if (p) (void)hipFree(p);
-    for (auto p : rot) if (p) (void)hipFree(p);
-    for (auto p : keys) if (p) (void)hipFree(p);
-    for (size_t l = 1; l < mat.size(); l++) if (mat[l]) (void)hipFree(mat[l]);
+    (void)hipDeviceSynchronize();      // the blocks return to the cache: no kernel may still use them
+    for (auto p : par) chz::dev_free(p);
+    for (auto p : inst) chz::dev_free(p);
+    for (auto p : rot) chz::dev_free(p);
+    for (auto p : keys) chz::dev_free(p);
+    for (size_t l = 1; l < mat.size(); l++) chz::dev_free(mat[l]);
     par.clear(); inst.clear(); rot.clear(); keys.clear(); count.clear(); mat.clear();
   }
 };
 
 template <typename T> static T *dalloc(size_t n) {
-  T *p = nullptr;
-  HIPCHK(hipMalloc((void **)&p, std::max<size_t>(n, 1) * sizeof(T)));
-  return p;
+  return (T *)chz::dev_alloc(std::max<size_t>(n, 1) * sizeof(T));
 }
 
 static unsigned grid1(long long n, long long cap = 65536) { return (unsigned)std::max<long long>(1, std::min(n, cap)); }
@@ -792,7 +819,8 @@ template <int NQ> static void materialise(chz_tc *c, Levels &L, int t) {
   }
   if (t >= 2 && L.mat[t - 1] && L.mat[t]) {
     HIPCHK(hipStreamSynchronize(c->stream));
-    (void)hipFree(L.mat[t - 1]);
+    HIPCHK(hipDeviceSynchronize());
+    chz::dev_free(L.mat[t - 1]);
     L.mat[t - 1] = nullptr;
   }
 }
@@ -939,7 +967,12 @@ static void table_reserve(chz_tc *c, long long need) {
   if (c->d_hkey && (double)need <= 0.6 * (double)c->nslots) return;
   unsigned long long slots = 1ull << 12;
   while ((double)need > 0.45 * (double)slots) slots <<= 1;
-  if (c->d_hkey) { (void)hipFree(c->d_hkey); (void)hipFree(c->d_hval); c->d_hkey = c->d_hval = nullptr; }
+  if (c->d_hkey) {
+    (void)hipDeviceSynchronize();
+    chz::dev_free(c->d_hkey);
+    chz::dev_free(c->d_hval);
+    c->d_hkey = c->d_hval = nullptr;
+  }
   c->d_hkey = dalloc<unsigned long long>(slots);
   c->d_hval = dalloc<unsigned long long>(slots);
   c->nslots = slots;
@@ -1077,7 +1110,7 @@ template <int NQ> static void build_table(chz_tc *c) {
   // the table's nodes are replayed from the identity when a lookup needs them; drop materialised levels
   HIPCHK(hipStreamSynchronize(c->stream));
   for (size_t l = 1; l < T.mat.size(); l++)
-    if (T.mat[l]) { (void)hipFree(T.mat[l]); T.mat[l] = nullptr; }
+    if (T.mat[l]) { HIPCHK(hipDeviceSynchronize()); chz::dev_free(T.mat[l]); T.mat[l] = nullptr; }
 }
 
 // one batch of searches against the table (src/tcount.cu:497-531 per instance)
@@ -1096,7 +1129,7 @@ static void search_batch(chz_tc *c, std::vector<Inst> &ins, int maxt, chz_tc_res
   struct Guard {
     Levels &S;
     std::vector<void *> &extra;
-    ~Guard() { S.release(); for (void *p : extra) if (p) (void)hipFree(p); }
+    ~Guard() { S.release(); for (void *p : extra) chz::dev_free(p); }     // S.release() synchronised
   } guard{S, extra};
   auto keep_ptr = [&](auto *p) { extra.push_back((void *)p); return p; };
   // roots
@@ -1198,7 +1231,7 @@ static void search_batch(chz_tc *c, std::vector<Inst> &ins, int maxt, chz_tc_res
     HIPCHK(hipStreamSynchronize(c->stream));
     c->ev.flush(c->k_ms, c->k_launch);
     // results of this level
-    std::vector<uint32_t> qn, qt;
+    std::vector<uint32_t> qn;
     std::vector<int> ql, qi;
     for (int i = 0; i < ni; i++)
       if (state[i] < 0 && best[i] != ~0ull) {
@@ -1208,16 +1241,12 @@ static void search_batch(chz_tc *c, std::vector<Inst> &ins, int maxt, chz_tc_res
       }
     if (!qn.empty()) {
       const long long nq = (long long)qn.size();
-      qt.resize(nq);
-      for (long long q = 0; q < nq; q++) HIPCHK(hipMemcpy(&qt[q], d_hitx + qn[q], 4, hipMemcpyDeviceToHost));
       uint32_t *dn = keep_ptr(dalloc<uint32_t>(nq));
-      uint32_t *dt = keep_ptr(dalloc<uint32_t>(nq));
       int *dl = keep_ptr(dalloc<int>(nq));
       int *dout = keep_ptr(dalloc<int>(nq * 2 * MAXT));
-      HIPCHK(hipMemcpy(dn, qn.data(), nq * 4, hipMemcpyHostToDevice));
-      HIPCHK(hipMemcpy(dt, qt.data(), nq * 4, hipMemcpyHostToDevice));
-      HIPCHK(hipMemcpy(dl, ql.data(), nq * 4, hipMemcpyHostToDevice));
-      PathArgs pa{forest_of(S), Ft, j, nq, dn, dt, dl, dout};
+      HIPCHK(hipMemcpyAsync(dn, qn.data(), nq * 4, hipMemcpyHostToDevice, c->stream));
+      HIPCHK(hipMemcpyAsync(dl, ql.data(), nq * 4, hipMemcpyHostToDevice, c->stream));
+      PathArgs pa{forest_of(S), Ft, j, nq, dn, d_hitx, dl, dout};
       hipLaunchKernelGGL(k_paths, dim3(gridt(nq)), dim3(256), 0, c->stream, pa);
       std::vector<int> paths(nq * 2 * MAXT);
       HIPCHK(hipMemcpyAsync(paths.data(), dout, paths.size() * 4, hipMemcpyDeviceToHost, c->stream));
@@ -1315,13 +1344,16 @@ int chz_tc_search(chz_tc *c, const char *const *specs, const size_t *lens, int n
   if (!c || ninst < 0 || (ninst > 0 && (!specs || !lens || !res))) { set_err("null argument"); return 3; }
   if (maxt < 0 || maxt > MAXT) { set_err("maxt outside 0..32"); return 3; }
   const double t0 = chz::now_s();
-  std::vector<Inst> ins;
-  for (int i = 0; i < ninst; i++) {
+  // parse and check every target (exact orthogonality is O(P^3) each), spread over host threads
+  std::vector<Inst> all(ninst);
+  std::vector<int> rcs(ninst, 0);
+  std::vector<std::string> errs(ninst);
+  auto prep = [&](int i) {
     std::memset(&res[i], 0, sizeof(chz_tc_result));
     res[i].rc = -1;
-    Inst in;
+    Inst &in = all[i];
     in.slot = i;
-    std::string err;
+    std::string &err = errs[i];
     int rc = read_spec(specs[i], lens[i], in.s, err);
     if (!rc && (in.s.n != c->n || in.s.tr != c->spec.tr || in.s.ts != c->spec.ts)) {
       err = "spec's qubit count or rotation tables differ from the table's";
@@ -1335,8 +1367,21 @@ int chz_tc_search(chz_tc *c, const char *const *specs, const size_t *lens, int n
       err = "channel exponent plus search depth beyond " + std::to_string(EMAX);
       rc = 3;
     }
-    if (rc) { res[i].rc = rc; set_err(err); continue; }
-    ins.push_back(std::move(in));
+    rcs[i] = rc;
+  };
+  const int nth = std::max(1, std::min<int>({ninst / 8, (int)std::thread::hardware_concurrency(), 16}));
+  if (nth <= 1) {
+    for (int i = 0; i < ninst; i++) prep(i);
+  } else {
+    std::vector<std::thread> th;
+    for (int t = 0; t < nth; t++)
+      th.emplace_back([&, t]() { for (int i = t; i < ninst; i += nth) prep(i); });
+    for (auto &x : th) x.join();
+  }
+  std::vector<Inst> ins;
+  for (int i = 0; i < ninst; i++) {
+    if (rcs[i]) { res[i].rc = rcs[i]; set_err(errs[i]); continue; }
+    ins.push_back(std::move(all[i]));
   }
   try {
     HIPCHK(hipSetDevice(c->device));
@@ -1370,7 +1415,7 @@ void chz_tc_free(chz_tc *c) {
   (void)hipDeviceSynchronize();      // cached device blocks must be idle before they are reused
   c->tab.release();
   void *ps[] = {c->d_anti, c->d_plus, c->d_coef, c->d_rv, c->d_rw, c->d_troot_fp, c->d_hkey, c->d_hval};
-  for (void *p : ps) if (p) (void)hipFree(p);
+  for (void *p : ps) chz::dev_free(p);
   c->ckeys.release();
   for (int i = 0; i < 2; i++) { c->sk[i].release(); c->sv[i].release(); }
   c->runs.release(); c->list.release(); c->list2.release();

@@ -131,6 +131,24 @@ template <typename T> struct DBuf {     // grow-only device array on the block c
   void release() { if (p) dev_pool().put(p, bytes); p = nullptr; cap = 0; bytes = 0; }
 };
 
+// Pinned host staging for a context's small per-level transfers (budgets, counters, flags): copies from and to
+// pinned memory are asynchronous DMA, where pageable ones stall the host on a staging copy each.  The caller
+// uses it only between stream synchronisations (no copy through it is in flight when it grows or is rewritten).
+struct Pinned {
+  uint8_t *p = nullptr;
+  size_t cap = 0;
+  uint8_t *need(size_t n) {
+    if (n <= cap) return p;
+    if (p) HIPCHK(hipHostFree(p));
+    p = nullptr;
+    const size_t c = std::max<size_t>(n, std::max<size_t>(4096, cap * 2));
+    HIPCHK(hipHostMalloc((void **)&p, c, hipHostMallocDefault));
+    cap = c;
+    return p;
+  }
+  void release() { if (p) (void)hipHostFree(p); p = nullptr; cap = 0; }
+};
+
 inline double now_s() { return std::chrono::duration<double>(std::chrono::steady_clock::now().time_since_epoch()).count(); }
 
 // HIP-event timing of kernel families on one stream; pairs are read after the stream synchronised

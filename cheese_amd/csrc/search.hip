@@ -583,6 +583,7 @@ struct chz_ctx {
   chz::DBuf<int> d_rem;
   chz::DBuf<uint8_t> d_keepi, d_active;
   chz::DBuf<unsigned long long> icnt, scan_tmp;
+  chz::Pinned pin;                  // staging of the per-level control transfers
   chz::DBuf<long long> d_lo, d_hi;
   // expansion schedule per level-size bucket (log2 of the chunk's parents): occupancy variant, grid cap in
   // blocks, candidate-split threshold in waves; chosen by chz_tune, defaults otherwise
@@ -694,12 +695,14 @@ __global__ void k_fill_ll(long long *p, long long n, long long v) {
 }
 
 static std::vector<long long> slot_counts(chz_ctx *c, const uint32_t *inst, long long m) {
-  std::vector<long long> lo(c->nslots), hi(c->nslots), out(c->nslots);
+  std::vector<long long> out(c->nslots);
+  // (called after the level's control transfers completed: the staging area is free)
+  long long *lo = (long long *)c->pin.need((size_t)c->nslots * 16), *hi = lo + c->nslots;
   hipLaunchKernelGGL(k_fill_ll, dim3(16), dim3(256), 0, c->stream, c->d_lo.p, (long long)c->nslots, 0ll);
   hipLaunchKernelGGL(k_fill_ll, dim3(16), dim3(256), 0, c->stream, c->d_hi.p, (long long)c->nslots, 0ll);
   if (m) hipLaunchKernelGGL(k_slot_bounds, dim3(grid_for(m / 64 + 1)), dim3(256), 0, c->stream, inst, m, c->d_lo.p, c->d_hi.p);
-  HIPCHK(hipMemcpyAsync(lo.data(), c->d_lo.p, c->nslots * 8, hipMemcpyDeviceToHost, c->stream));
-  HIPCHK(hipMemcpyAsync(hi.data(), c->d_hi.p, c->nslots * 8, hipMemcpyDeviceToHost, c->stream));
+  HIPCHK(hipMemcpyAsync(lo, c->d_lo.p, c->nslots * 8, hipMemcpyDeviceToHost, c->stream));
+  HIPCHK(hipMemcpyAsync(hi, c->d_hi.p, c->nslots * 8, hipMemcpyDeviceToHost, c->stream));
   HIPCHK(hipStreamSynchronize(c->stream));
   for (int s = 0; s < c->nslots; s++) out[s] = hi[s] - lo[s];
   return out;
@@ -790,16 +793,25 @@ template <int NQ> static void batch_step(chz_ctx *c) {
   const double tl = now_s();
   const int cur = c->cur;
   const long long m = c->m;
-  // per-slot budgets
-  std::vector<int> rem(c->nslots, 0);
-  std::vector<uint8_t> keepi(c->nslots, 0), active(c->nslots, 0);
-  std::vector<unsigned long long> ic((size_t)c->nslots * IC_N, 0);
-  for (int s = 0; s < c->nslots; s++) ic[(size_t)s * IC_N + IC_FIRST] = ~0ull;
+  // per-slot budgets and counters, staged in pinned memory: [icnt in | icnt out | counters | rem | keepi | active]
+  const int ns = c->nslots;
+  const size_t o_icin = 0, o_icout = o_icin + (size_t)ns * IC_N * 8, o_hc = o_icout + (size_t)ns * IC_N * 8,
+               o_tot = o_hc + C_N * 8, o_rem = o_tot + 8, o_keep = o_rem + (size_t)ns * 4, o_act = o_keep + ns;
+  uint8_t *pin = c->pin.need(o_act + ns);
+  unsigned long long *ic_in = (unsigned long long *)(pin + o_icin), *ic = (unsigned long long *)(pin + o_icout);
+  unsigned long long *hc = (unsigned long long *)(pin + o_hc), *ptot = (unsigned long long *)(pin + o_tot);
+  int *rem = (int *)(pin + o_rem);
+  uint8_t *keepi = pin + o_keep, *active = pin + o_act;
+  for (int s = 0; s < ns; s++) {
+    rem[s] = 0;
+    keepi[s] = active[s] = 0;
+    for (int i = 0; i < IC_N; i++) ic_in[(size_t)s * IC_N + i] = i == IC_FIRST ? ~0ull : 0ull;
+  }
   for (const BInst &b : c->binst)
     if (b.live) { rem[b.slot] = b.depth - (b.t + 1); keepi[b.slot] = b.t + 1 < b.depth; }
-  HIPCHK(hipMemcpyAsync(c->d_rem.p, rem.data(), c->nslots * 4, hipMemcpyHostToDevice, c->stream));
-  HIPCHK(hipMemcpyAsync(c->d_keepi.p, keepi.data(), c->nslots, hipMemcpyHostToDevice, c->stream));
-  HIPCHK(hipMemcpyAsync(c->icnt.p, ic.data(), ic.size() * 8, hipMemcpyHostToDevice, c->stream));
+  HIPCHK(hipMemcpyAsync(c->d_rem.p, rem, ns * 4, hipMemcpyHostToDevice, c->stream));
+  HIPCHK(hipMemcpyAsync(c->d_keepi.p, keepi, ns, hipMemcpyHostToDevice, c->stream));
+  HIPCHK(hipMemcpyAsync(c->icnt.p, ic_in, (size_t)ns * IC_N * 8, hipMemcpyHostToDevice, c->stream));
   const long long chunk = c->opts.chunk > 0 ? c->opts.chunk : (1ll << 20);
   const long long maxcc = std::min<long long>(chunk, m) * G;
   c->status.need(maxcc);
@@ -852,9 +864,8 @@ template <int NQ> static void batch_step(chz_ctx *c) {
     c->ev.end(c->stream);
     c->k_work[K_COMPACT] += (double)ncc;
   }
-  unsigned long long hc[C_N];
   HIPCHK(hipMemcpyAsync(hc, c->cnt.p, C_N * 8, hipMemcpyDeviceToHost, c->stream));
-  HIPCHK(hipMemcpyAsync(ic.data(), c->icnt.p, ic.size() * 8, hipMemcpyDeviceToHost, c->stream));
+  HIPCHK(hipMemcpyAsync(ic, c->icnt.p, (size_t)ns * IC_N * 8, hipMemcpyDeviceToHost, c->stream));
   HIPCHK(hipStreamSynchronize(c->stream));
   c->ev.flush(c->k_ms, c->k_launch);
   const long long nsurv = (long long)hc[C_SURV];
@@ -868,7 +879,7 @@ template <int NQ> static void batch_step(chz_ctx *c) {
       const unsigned long long *x = &ic[(size_t)b.slot * IC_N];
       active[b.slot] = !x[IC_FOUND] && !x[IC_OVF] && keepi[b.slot];
     }
-  HIPCHK(hipMemcpyAsync(c->d_active.p, active.data(), c->nslots, hipMemcpyHostToDevice, c->stream));
+  HIPCHK(hipMemcpyAsync(c->d_active.p, active, ns, hipMemcpyHostToDevice, c->stream));
   long long distinct = 0;
   const int nxt = cur ^ 1;
   if (nsurv > 0) {
@@ -904,12 +915,11 @@ template <int NQ> static void batch_step(chz_ctx *c) {
     hipLaunchKernelGGL(k_scatter_idx<FlagKept>, dim3((unsigned)nb), dim3(CT_THREADS), 0, c->stream, fk, nsurv,
                        c->bcount.p, c->kidx.p);
     c->ev.end(c->stream);
-    unsigned long long tot = 0;
-    HIPCHK(hipMemcpyAsync(&tot, c->bcount.p + nb, 8, hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(hipMemcpyAsync(ptot, c->bcount.p + nb, 8, hipMemcpyDeviceToHost, c->stream));
     HIPCHK(hipMemcpyAsync(hc, c->cnt.p, C_N * 8, hipMemcpyDeviceToHost, c->stream));
     HIPCHK(hipStreamSynchronize(c->stream));
     if (hc[C_REPOVF]) { set_err("a dedupe run holds more distinct classes than its buffer"); throw DeviceError(); }
-    distinct = (long long)tot;
+    distinct = (long long)*ptot;
   }
   // next frontier: kept children, then the searches admitted since the last step
   const long long mn = distinct + (long long)c->pending.size();
@@ -1420,6 +1430,7 @@ void chz_close(chz_ctx *c) {
   }
   c->finst[0].release(); c->finst[1].release(); c->d_rem.release(); c->d_keepi.release(); c->d_active.release();
   c->icnt.release(); c->scan_tmp.release(); c->d_lo.release(); c->d_hi.release();
+  c->pin.release();
   c->status.release(); c->kept.release(); c->keys.release(); c->surv_o.release(); c->surv_key.release();
   c->surv_bw.release();
   c->info.release(); c->cnt.release(); c->bcount.release(); c->hist.release(); c->runs.release(); c->kidx.release();

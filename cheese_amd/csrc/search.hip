@@ -66,7 +66,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(WPE))) void
                                                                                          long long np) {
   constexpr int N = 1 << NQ, E = N * N, GS = E, CPW = 64 / GS;
   __shared__ uint2 stage_x[TS_STRIDE];
-  __shared__ uint32_t stage_p[8 * TS_STRIDE];
+  __shared__ uint4 stage_p[2 * TS_STRIDE];
   __shared__ int4 spar[256];                   // each group's parent residual
   // per candidate and entry (g * 64 + e), the child entry's terms for unit gates with at most two per column:
   // bits 0-2 source column x0, 3-5 power p0 of w, 6-8 x1, 9-11 p1, bit 12 two terms; bit 15: use the table loop

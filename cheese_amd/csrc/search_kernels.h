@@ -268,27 +268,23 @@ __device__ __forceinline__ uint32_t bits(u16x2 v) { return (uint32_t)v.x | ((uin
 
 // Per-entry operands of the packed transform, staged in LDS by the lane owning entry e, one array per field
 // (a group's entries are read at a stride of 8 bytes, free of bank conflicts): x[e] = (a, b | c, d) of S[e] mod
-// 2^16, and p[i * TS_STRIDE + e] eight packed multipliers of conj(S[e]) = (a, -d, -c, -b).  With
+// 2^16, and p[pass * TS_STRIDE + e] the four packed multipliers of conj(S[e]) = (a, -d, -c, -b) for a pass (one
+// 128-bit read per entry and pass).  With
 // Q = S[x] conj(S[e]) = (Qa, Qb, Qc, Qd) the transform needs only Qa, Qc and Qb -+ Qd (below), in two passes:
 //   (Qa | Qc)           = xa (a | -c)    + xb (b | -d)    + xc (c | a)     + xd (d | b)
 //   (Qb - Qd | Qb + Qd) = xa (b-d | -b-d) + xb (a+c | a-c) + xc (b+d | b-d) + xd (c-a | c+a)
 constexpr int TS_STRIDE = 256;      // entries per field: one per thread of a block
 struct TStage {
   uint2 *x;
-  uint32_t *p;
+  uint4 *p;
 };
 
 template <int NQ> __device__ __forceinline__ void t_stage(TStage st, int4 s, int le) {
   const int a = s.x, b = s.y, c = s.z, d = s.w;
   st.x[le] = make_uint2(bits(pk(a, b)), bits(pk(c, d)));
-  st.p[0 * TS_STRIDE + le] = bits(pk(a, -c));
-  st.p[1 * TS_STRIDE + le] = bits(pk(b, -d));
-  st.p[2 * TS_STRIDE + le] = bits(pk(c, a));
-  st.p[3 * TS_STRIDE + le] = bits(pk(d, b));
-  st.p[4 * TS_STRIDE + le] = bits(pk(b - d, -b - d));
-  st.p[5 * TS_STRIDE + le] = bits(pk(a + c, a - c));
-  st.p[6 * TS_STRIDE + le] = bits(pk(b + d, b - d));
-  st.p[7 * TS_STRIDE + le] = bits(pk(c - a, c + a));
+  st.p[le] = make_uint4(bits(pk(a, -c)), bits(pk(b, -d)), bits(pk(c, a)), bits(pk(d, b)));
+  st.p[TS_STRIDE + le] = make_uint4(bits(pk(b - d, -b - d)), bits(pk(a + c, a - c)), bits(pk(b + d, b - d)),
+                                    bits(pk(c - a, c + a)));
 }
 
 __device__ __forceinline__ u16x2 as16(uint32_t v) { u16x2 r; r.x = (unsigned short)v; r.y = (unsigned short)(v >> 16); return r; }
@@ -315,7 +311,7 @@ template <int NQ> __device__ __forceinline__ int t_bound16(TStage st, int4 s, in
   uint32_t acc[2];
 #pragma unroll 1
   for (int pass = 0; pass < 2; pass++) {
-    const uint32_t *pm = st.p + pass * 4 * TS_STRIDE;
+    const uint4 *pm = st.p + pass * TS_STRIDE;
     // the addresses below depend on a value the compiler cannot see through, so they are formed per pass
     // instead of being hoisted out of the pass loop (64 more live registers)
     int fp = f, lm = lowm;
@@ -326,8 +322,8 @@ template <int NQ> __device__ __forceinline__ int t_bound16(TStage st, int4 s, in
       const int e = ((i & ~lm) << 1) | (i & lm);         // i with a 0 inserted at bit j
       const uint2 xv = st.x[e ^ fp];
       const u16x2 x01 = as16(xv.x), x23 = as16(xv.y);
-      q[i] = x01.xx * as16(pm[e]) + x01.yy * as16(pm[TS_STRIDE + e]) + x23.xx * as16(pm[2 * TS_STRIDE + e]) +
-             x23.yy * as16(pm[3 * TS_STRIDE + e]);
+      const uint4 m = pm[e];
+      q[i] = x01.xx * as16(m.x) + x01.yy * as16(m.y) + x23.xx * as16(m.z) + x23.yy * as16(m.w);
     }
 #pragma unroll
     for (int h = 1; h < H; h <<= 1)

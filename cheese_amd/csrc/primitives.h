@@ -15,8 +15,16 @@ using namespace chz;
 // ordered stream compaction: tiles of 256 threads x 16 items, counts -> one-block scan -> scatter
 constexpr int CT_THREADS = 256, CT_ITEMS = 16, CT_TILE = CT_THREADS * CT_ITEMS;
 
+// Kernels taking `nd` read their element count from device memory when it is not null (launches sized for a
+// capacity, as in a captured graph); n is then the capacity the grid was sized for.
+__device__ __forceinline__ long long dev_n(long long n, const unsigned long long *nd) {
+  return nd ? (long long)*nd : n;
+}
+
 template <typename F>
-__global__ __launch_bounds__(CT_THREADS) void k_flag_count(F f, long long n, unsigned long long *bcount) {
+__global__ __launch_bounds__(CT_THREADS) void k_flag_count(F f, long long n, unsigned long long *bcount,
+                                                           const unsigned long long *nd = nullptr) {
+  n = dev_n(n, nd);
   const long long t0 = (long long)blockIdx.x * CT_TILE;
   int c = 0;
   for (int i = 0; i < CT_ITEMS; i++) {
@@ -81,7 +89,8 @@ __device__ __forceinline__ int block_flag_scan(bool flag, int &excl, int *ws) {
 // flagged positions -> ordered index list (kept survivor ranks, children needing the T bound)
 template <typename F>
 __global__ __launch_bounds__(CT_THREADS) void k_scatter_idx(F f, long long n, const unsigned long long *boff,
-                                                            uint32_t *out) {
+                                                            uint32_t *out, const unsigned long long *nd = nullptr) {
+  n = dev_n(n, nd);
   __shared__ int ws[CT_THREADS / 64];
   const long long t0 = (long long)blockIdx.x * CT_TILE;
   long long pos = (long long)boff[blockIdx.x];
@@ -100,7 +109,9 @@ __global__ __launch_bounds__(CT_THREADS) void k_scatter_idx(F f, long long n, co
 constexpr int RS_THREADS = 256, RS_ROUNDS = 8, RS_TILE = RS_THREADS * RS_ROUNDS;
 
 __global__ __launch_bounds__(RS_THREADS) void k_radix_hist(const uint32_t *keys, long long n, int shift,
-                                                           unsigned long long *hist, long long nblocks) {
+                                                           unsigned long long *hist, long long nblocks,
+                                                           const unsigned long long *nd = nullptr) {
+  n = dev_n(n, nd);
   __shared__ unsigned int h[256];
   h[threadIdx.x] = 0;
   __syncthreads();
@@ -115,7 +126,9 @@ __global__ __launch_bounds__(RS_THREADS) void k_radix_hist(const uint32_t *keys,
 
 __global__ __launch_bounds__(RS_THREADS) void k_radix_scatter(const uint32_t *kin, const uint32_t *vin, long long n,
                                                               int shift, const unsigned long long *hist,
-                                                              long long nblocks, uint32_t *kout, uint32_t *vout) {
+                                                              long long nblocks, uint32_t *kout, uint32_t *vout,
+                                                              const unsigned long long *nd = nullptr) {
+  n = dev_n(n, nd);
   __shared__ unsigned long long base[256];
   __shared__ unsigned int wc[RS_THREADS / 64][256];
   const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
@@ -212,18 +225,20 @@ static inline hipError_t scan_excl(unsigned long long *v, long long n, unsigned 
 
 // stable sort of n (u32 key, u32 value) pairs, 4 passes of 8 bits: (k0, v0) -> (k0, v0) through (k1, v1).
 // hist holds at least 256 * ceil(n / RS_TILE) + 1 entries, tmp what scan_excl needs for that many.
+// With nd the count is read on the device and n is the capacity the launches are sized for.
 static inline hipError_t radix_sort_pairs(uint32_t *k0, uint32_t *v0, uint32_t *k1, uint32_t *v1, long long n,
-                                          unsigned long long *hist, unsigned long long *tmp, hipStream_t s) {
+                                          unsigned long long *hist, unsigned long long *tmp, hipStream_t s,
+                                          const unsigned long long *nd = nullptr) {
   if (n <= 1) return hipSuccess;
   const long long nb = (n + RS_TILE - 1) / RS_TILE;
   uint32_t *ks[2] = {k0, k1}, *vs[2] = {v0, v1};
   for (int pass = 0; pass < 4; pass++) {
     const int a = pass & 1, b = a ^ 1;
-    hipLaunchKernelGGL(k_radix_hist, dim3((unsigned)nb), dim3(RS_THREADS), 0, s, ks[a], n, pass * 8, hist, nb);
+    hipLaunchKernelGGL(k_radix_hist, dim3((unsigned)nb), dim3(RS_THREADS), 0, s, ks[a], n, pass * 8, hist, nb, nd);
     hipError_t e = scan_excl(hist, nb * 256, tmp, s);
     if (e != hipSuccess) return e;
     hipLaunchKernelGGL(k_radix_scatter, dim3((unsigned)nb), dim3(RS_THREADS), 0, s, ks[a], vs[a], n, pass * 8, hist,
-                       nb, ks[b], vs[b]);
+                       nb, ks[b], vs[b], nd);
   }
   return hipGetLastError();
 }

@@ -434,17 +434,29 @@ struct DedupeArgs {
 };
 constexpr int MAXREP = 32;
 
+// value of lane q of the lane group (q uniform across the group)
+template <int GS> __device__ __forceinline__ uint32_t gget(uint32_t v, int q) {
+  if constexpr (GS == 64) return (uint32_t)__builtin_amdgcn_readlane((int)v, q);
+  else return (uint32_t)__shfl((int)v, q, GS);
+}
+template <int GS> __device__ __forceinline__ uint64_t gget64(uint64_t v, int q) {
+  return (uint64_t)gget<GS>((uint32_t)v, q) | ((uint64_t)gget<GS>((uint32_t)(v >> 32), q) << 32);
+}
+
 template <int NQ>
 __global__ __launch_bounds__(256) void k_dedupe_runs(KSpec sp, DedupeArgs a) {
   constexpr int N = 1 << NQ, E = N * N, GS = E, CPW = 64 / GS;
   constexpr int GPB = 256 / GS;     // groups per block
-  __shared__ uint32_t rep_rank[GPB][MAXREP];
+  __shared__ uint64_t rep_key[GPB][MAXREP], rep_o[GPB][MAXREP];
+  __shared__ uint32_t rep_inst[GPB][MAXREP];
+  (void)N;
   const int lane = threadIdx.x & 63, le = lane & (GS - 1), slot = lane / GS;
   const int grp = (threadIdx.x >> 6) * CPW + slot;
   const long long wave = ((long long)blockIdx.x * blockDim.x + threadIdx.x) >> 6;
   const long long nwaves = ((long long)gridDim.x * blockDim.x) >> 6;
   const long long nruns = (long long)a.cnt[C_RUNS];
   const long long n = dev_n(a.n, a.nd);
+  const uint64_t G = (uint64_t)sp.G;
   for (long long base = wave * CPW; base < nruns; base += nwaves * CPW) {
     const long long ri = base + slot;
     if (ri >= nruns) continue;
@@ -455,36 +467,61 @@ __global__ __launch_bounds__(256) void k_dedupe_runs(KSpec sp, DedupeArgs a) {
     // members are then each rebuilt once and compared with the cached first representative
     int4 c0 = make_int4(0, 0, 0, 0), c1 = c0;
     int ck0 = 0, ck1 = 0;
-    for (long long j = i0; j < n && a.k[j] == key; j++) {
-      const uint32_t rank = (uint32_t)wuni<NQ>((int)a.v[j]);
-      const uint64_t o = a.surv_o[rank], key64 = a.surv_key[rank];
-      const long long pi = (long long)(o / (uint64_t)sp.G);
-      const int g = wuni<NQ>((int)(o % (uint64_t)sp.G));
-      int4 s;
-      int ks;
-      make_child<NQ>(a.res + pi * E, a.pk[pi], sp.gates[g], le, s, ks);
-      bool dup = false;
-      for (int r = 0; r < nrep && !dup; r++) {
-        const uint32_t rr = rep_rank[grp][r];
-        if (a.surv_key[rr] != key64) continue;
-        const uint64_t ro = a.surv_o[rr];
-        const long long rpi = (long long)(ro / (uint64_t)sp.G);
-        if (a.pinst && a.pinst[rpi] != a.pinst[pi]) continue;
-        int4 b;
-        int kb;
-        if (r == 0) { b = c0; kb = ck0; }
-        else if (r == 1) { b = c1; kb = ck1; }
-        else make_child<NQ>(a.res + rpi * E, a.pk[rpi], sp.gates[(int)(ro % (uint64_t)sp.G)], le, b, kb);
-        dup = same_class<NQ>(s, ks, b, kb, le, sp);
+    // members in chunks of GS: lane le loads member j0 + le's rank, child, key, parent exponent and slot at once,
+    // so each member then waits on one load (its parent's residual) instead of a chain of three
+    for (long long j0 = i0; j0 < n; j0 += GS) {
+      const long long jj = j0 + le;
+      const bool in = jj < n && a.k[jj] == key;
+      const uint64_t inb = gballot<GS>(in);
+      const int nm = ~inb ? __ffsll((long long)~inb) - 1 : GS;     // members are contiguous from j0
+      uint32_t m_rank = 0, m_inst = 0;
+      uint64_t m_o = 0, m_key = 0;
+      int m_pk = 0;
+      if (le < nm) {
+        m_rank = a.v[jj];
+        m_o = a.surv_o[m_rank];
+        m_key = a.surv_key[m_rank];
+        const long long pi = (long long)(m_o / G);
+        m_pk = a.pk[pi];
+        m_inst = a.pinst ? a.pinst[pi] : 0u;
       }
-      if (dup) { if (le == 0) a.kept[rank] = 0; }
-      else if (nrep < MAXREP) {
-        if (le == 0) rep_rank[grp][nrep] = rank;
-        if (nrep == 0) { c0 = s; ck0 = ks; }
-        else if (nrep == 1) { c1 = s; ck1 = ks; }
-        nrep++;
-      } else if (le == 0) atomicAdd(&a.cnt[C_REPOVF], 1ull);
-      __builtin_amdgcn_wave_barrier();
+      for (int q = 0; q < nm; q++) {
+        const uint32_t rank = gget<GS>(m_rank, q), inst = gget<GS>(m_inst, q);
+        const uint64_t o = gget64<GS>(m_o, q), key64 = gget64<GS>(m_key, q);
+        const int pk = (int)gget<GS>((uint32_t)m_pk, q);
+        const long long pi = (long long)(o / G);
+        const int g = (int)(o % G);
+        int4 s;
+        int ks;
+        make_child<NQ>(a.res + pi * E, pk, sp.gates[g], le, s, ks);
+        bool dup = false;
+        for (int r = 0; r < nrep && !dup; r++) {
+          if (rep_key[grp][r] != key64 || rep_inst[grp][r] != inst) continue;
+          int4 b;
+          int kb;
+          if (r == 0) { b = c0; kb = ck0; }
+          else if (r == 1) { b = c1; kb = ck1; }
+          else {
+            const uint64_t ro = rep_o[grp][r];
+            const long long rpi = (long long)(ro / G);
+            make_child<NQ>(a.res + rpi * E, a.pk[rpi], sp.gates[(int)(ro % G)], le, b, kb);
+          }
+          dup = same_class<NQ>(s, ks, b, kb, le, sp);
+        }
+        if (dup) { if (le == 0) a.kept[rank] = 0; }
+        else if (nrep < MAXREP) {
+          if (le == 0) {
+            rep_key[grp][nrep] = key64;
+            rep_o[grp][nrep] = o;
+            rep_inst[grp][nrep] = inst;
+          }
+          if (nrep == 0) { c0 = s; ck0 = ks; }
+          else if (nrep == 1) { c1 = s; ck1 = ks; }
+          nrep++;
+        } else if (le == 0) atomicAdd(&a.cnt[C_REPOVF], 1ull);
+        __builtin_amdgcn_wave_barrier();
+      }
+      if (nm < GS) break;      // the run ended inside this chunk
     }
   }
 }

@@ -410,13 +410,33 @@ __global__ void k_add_total(unsigned long long *acc, const unsigned long long *t
 // segmented dedupe over the sorted survivors: runs of equal 32-bit key prefix are partitioned into exact
 // classes, keeping the smallest survivor rank (= lexicographically first prefix) of each class
 // (src/cheese.cu:1334-1343, :1046-1065)
-__global__ void k_run_heads(const uint32_t *k, const uint32_t *v, long long n, uint8_t *kept, uint32_t *runs,
-                            unsigned long long *cnt, const unsigned long long *nd) {
+// every survivor starts kept (a sequential fill: setting kept[v[i]] here would scatter byte writes over the array)
+__global__ void k_fill_kept(uint8_t *kept, long long n, const unsigned long long *nd) {
   n = dev_n(n, nd);
-  for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (long long)gridDim.x * blockDim.x) {
-    kept[v[i]] = 1;
-    const bool head = i == 0 || k[i] != k[i - 1];
-    if (head && i + 1 < n && k[i + 1] == k[i]) runs[atomicAdd(&cnt[C_RUNS], 1ull)] = (uint32_t)i;
+  for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (long long)gridDim.x * blockDim.x)
+    kept[i] = 1;
+}
+
+__global__ void k_run_heads(const uint32_t *k, long long n, uint32_t *runs, unsigned long long *cnt,
+                            const unsigned long long *nd) {
+  n = dev_n(n, nd);
+  // one atomic per wave (the runs' order does not matter: each is deduplicated on its own)
+  const int lane = threadIdx.x & 63;
+  const long long stride = (long long)gridDim.x * blockDim.x;
+  for (long long b0 = (long long)blockIdx.x * blockDim.x + (threadIdx.x & ~63); b0 < n; b0 += stride) {
+    const long long i = b0 + lane;
+    bool start = false;
+    if (i < n) {
+      const bool head = i == 0 || k[i] != k[i - 1];
+      start = head && i + 1 < n && k[i + 1] == k[i];
+    }
+    const uint64_t bal = __ballot(start);
+    if (!bal) continue;
+    const int leader = __ffsll((long long)bal) - 1;
+    unsigned long long at = 0;
+    if (lane == leader) at = atomicAdd(&cnt[C_RUNS], (unsigned long long)__popcll(bal));
+    at = (unsigned long long)__shfl((long long)at, leader);
+    if (start) runs[at + __popcll(bal & ((1ull << lane) - 1))] = (uint32_t)i;
   }
 }
 
@@ -900,8 +920,9 @@ static void step_graph(chz_ctx *c, int cur, long long m, const unsigned long lon
                        c->sv[0].p, dsurv);
     hipError_t e = radix_sort_pairs(c->sk[0].p, c->sv[0].p, c->sk[1].p, c->sv[1].p, ncap, c->hist.p, c->scan_tmp.p, s,
                                     dsurv);
-    hipLaunchKernelGGL(k_run_heads, dim3(grid_for(ncap / 64 + 1)), dim3(256), 0, s, c->sk[0].p, c->sv[0].p, ncap,
-                       c->kept.p, c->runs.p, c->cnt.p, dsurv);
+    hipLaunchKernelGGL(k_fill_kept, dim3(grid_for(ncap / 256 + 1)), dim3(256), 0, s, c->kept.p, ncap, dsurv);
+    hipLaunchKernelGGL(k_run_heads, dim3(grid_for(ncap / 64 + 1)), dim3(256), 0, s, c->sk[0].p, ncap, c->runs.p,
+                       c->cnt.p, dsurv);
     const DedupeArgs da{c->sk[0].p, c->sv[0].p, ncap, c->surv_o.p, c->surv_key.p, c->runs.p, c->res[cur].p,
                         c->pk[cur].p, c->kept.p, c->cnt.p, pinst, dsurv};
     hipLaunchKernelGGL(k_dedupe_runs<NQ>, dim3(grid_for(ncap / CPW + 1, 4096)), dim3(256), 0, s, ks, da);
@@ -1099,8 +1120,10 @@ template <int NQ> static void batch_step(chz_ctx *c) {
       c->k_work[K_SORT] += (double)nsurv * (16.0 + 4 * (4.0 + 16.0));
       c->ev.begin(K_DEDUPE, c->stream);
       HIPCHK(hipMemsetAsync(c->cnt.p + C_RUNS, 0, 8, c->stream));
-      hipLaunchKernelGGL(k_run_heads, dim3(grid_for(nsurv / 64 + 1)), dim3(256), 0, c->stream, c->sk[0].p,
-                         c->sv[0].p, nsurv, c->kept.p, c->runs.p, c->cnt.p, nullptr);
+      hipLaunchKernelGGL(k_fill_kept, dim3(grid_for(nsurv / 256 + 1)), dim3(256), 0, c->stream, c->kept.p, nsurv,
+                         nullptr);
+      hipLaunchKernelGGL(k_run_heads, dim3(grid_for(nsurv / 64 + 1)), dim3(256), 0, c->stream, c->sk[0].p, nsurv,
+                         c->runs.p, c->cnt.p, nullptr);
       const DedupeArgs da{c->sk[0].p, c->sv[0].p, nsurv, c->surv_o.p, c->surv_key.p, c->runs.p,
                           c->res[cur].p, c->pk[cur].p, c->kept.p, c->cnt.p, pinst, nullptr};
       hipLaunchKernelGGL(k_dedupe_runs<NQ>, dim3(grid_for(nsurv / CPW + 1, 4096)), dim3(256), 0, c->stream, ks, da);

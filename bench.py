@@ -63,6 +63,32 @@ def copy_probe(nbytes):
     hip.hipFree(b)
 
 
+def toffoli_d9(runs=5):
+    """C1: the Toffoli budget-9 impossibility proof on one context, log checked against the reference's golden log
+    every run; median wall time per proof"""
+    import cheese_amd
+    with open(os.path.join(ROOT, "tests", "golden", "specs", "toffoli-d9.spec")) as f:
+        spec = f.read()
+    with open(os.path.join(ROOT, "tests", "golden", "logs", "toffoli-d9.log")) as f:
+        golden = _strip(f.read().strip().splitlines())
+    s = cheese_amd.Searcher(spec)
+    try:
+        s.run()
+        times, prefixes = [], 0
+        for _ in range(runs):
+            t = time.perf_counter()
+            out = s.run()
+            times.append(time.perf_counter() - t)
+            if out["rc"] != 1 or _strip([cheese_amd.level_line(l) for l in out["levels"]]) != golden:
+                print("bench.py: Toffoli d9 log differs from the reference golden log", file=sys.stderr)
+                sys.exit(1)
+            prefixes = sum(l["prefixes"] for l in out["levels"])
+    finally:
+        s.close()
+    ms = sorted(times)[len(times) // 2] * 1e3
+    return {"ms_per_proof": round(ms, 3), "prefixes": prefixes, "prefixes_per_s": round(prefixes / (ms / 1e3), 1)}
+
+
 def random_circuits(n=32):
     """the second half of the workload: seeded random 3-qubit Clifford+T circuits of 10..13 gates, each
     synthesised at budget = its length; every found circuit is checked exactly against its target"""
@@ -269,6 +295,7 @@ def main():
         "kernels_ms_per_step": {k: round(v["ms"] / a.steps, 3) for k, v in kstats.items()},
         "autotune": {b: [v["waves_per_eu"], v["grid_cap"], v["split_waves"]] for b, v in tuned.items()} if tuned else None,
     }
+    line["toffoli_d9"] = toffoli_d9()
     if a.random:
         line["random32"] = random_circuits()
     if a.clifford or a.tcount or a.serve:

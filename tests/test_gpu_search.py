@@ -68,3 +68,24 @@ def test_depth_zero_identity():
     skel = workloads.skeleton(2)
     out = cheese_amd.search(skel)
     assert out["rc"] == 0 and out["circuit"] == []
+
+
+def test_expansion_stage_counts_add_up():
+    """the expansion's stage counters (the factors of its work model) against identities of the level lines: every
+    child not cut by a window is formed, every survivor of a level that keeps its survivors gets a class key, T-bound
+    evaluations and cut ranks happen only on formed children"""
+    spec = read_spec("toffoli-d10")
+    s = cheese_amd.Searcher(spec)
+    try:
+        out = s.run(stats=True)
+    finally:
+        s.close()
+    st, lv = out["expand_stages"], out["levels"]
+    depth = int(spec.split("\ndepth ")[1].split()[0])
+    assert out["rc"] == 1 and lv
+    assert st["residuals"] == sum(l["prefixes"] - l["cut_w3"] for l in lv)
+    # a level's survivors get keys while its search goes on (t < depth)
+    assert st["keys"] == sum(l["survivors"] for l in lv if l["t"] < depth)
+    assert st["structural"] <= st["residuals"]
+    assert 0 < st["t_bounds"] < st["structural"]
+    assert 0 < st["cut_ranks"] <= 3 * st["structural"]

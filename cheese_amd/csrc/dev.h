@@ -36,6 +36,13 @@ struct KSpec {                                 // device view of a spec (kernel 
   const int *cxtable;
   const uint8_t *symperm;                      // nsym x N basis maps
   const uint8_t *symconj;                      // nsym
+  // goal pre-images: xg[g] (N x N, sqrt2 exponent xgk[g], nonzero-entry mask xgm[g]) is the one normalised
+  // residual that candidate g maps to the goal (xgk -1: none); a parent whose coefficients are all at most
+  // maglim has no child outside int32
+  const int4 *xg;
+  const uint64_t *xgm;
+  const int *xgk;
+  uint32_t maglim;
 };
 
 // ---- lane groups: GS lanes (a power of two dividing 64) cooperate on one matrix ----

@@ -51,6 +51,7 @@ struct ExpandArgs {
   int gsplit;               // a parent's candidates are split among this many lane groups (small levels)
   const unsigned long long *npd;   // parents read on the device (captured step; gsplit then from tw), or null
   int tw;                   // candidate-split threshold in waves (with npd)
+  const uint32_t *gflag;    // per slot: some parent of the level has a goal child (null: not tracked)
 };
 
 // One level's children, fused (src/cheese.cu:662-720, :959-1021): a lane group per parent holds the parent's
@@ -167,6 +168,13 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(WPE))) void
     const int pk = wuni<NQ>(a.pk[pi]), g1 = wuni<NQ>((int)a.g1[pi]), g2 = wuni<NQ>((int)a.g2[pi]);
     const uint32_t pbw = (uint32_t)wuni<NQ>((int)a.pb[pi]);
     const bool small = gall<GS>(zmag(S) < (1u << 28));
+    // A search whose circuit is found at this level ends with it: its parents without a goal child (no
+    // BW_GOAL) are passed over once their coefficients rule out an overflowing child.  Goal children come only
+    // from BW_GOAL parents, so the level's found count, first circuit and overflow test are unchanged.
+    if (a.gflag && !(pbw & BW_GOAL) && wuni<NQ>((int)a.gflag[inst]) != 0 && gall<GS>(zmag(S) <= sp.maglim)) {
+      for (int g = gbeg + le; g < gend; g += GS) a.status[pl * G + g] = ST_DEAD;
+      continue;
+    }
     __builtin_amdgcn_wave_barrier();
     P[le] = S;
     __builtin_amdgcn_wave_barrier();
@@ -268,11 +276,11 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(WPE))) void
               const bool tknown = tpar && h.tdel == 0;
               int T = tpar ? (Tp > h.tdel ? Tp - h.tdel : 0) : 0;   // exact if tknown, else a lower bound
               bool texact = tknown;
-              const int c2lo = !sp.lite && sp.cx_mono && (pbw & BW_CUTS) ? cx_lower(sp, (pbw & ~BW_CUTS) >> 8, (uint32_t)h.cdel) : 0;
+              const int c2lo = !sp.lite && sp.cx_mono && (pbw & BW_CUTS) ? cx_lower(sp, (pbw & BW_NEEDS) >> 8, (uint32_t)h.cdel) : 0;
               if (combine_bound(hh, (T + sp.tdiv - 1) / sp.tdiv, c2lo, supp) > rem) alive = false;
               else {
                 if (!sp.lite) {
-                  uint32_t needs = (pbw & ~BW_CUTS) >> 8;
+                  uint32_t needs = (pbw & BW_NEEDS) >> 8;
                   const uint32_t keepc = (pbw & BW_CUTS) ? (uint32_t)h.loc : 0u;
                   n_cuts += sp.ncuts - __popc(keepc & ((1u << sp.ncuts) - 1));
                   c2 = cx_bound<NQ>(s, le, sp, keepc, needs);
@@ -563,6 +571,7 @@ struct GatherArgs {
   const uint32_t *surv_bw;        // bound words of the survivors
   uint32_t *pb_next;
   const unsigned long long *nkd;  // kept count read on the device (captured step), or null
+  uint32_t *gflag;                // per slot: set when a kept child has a goal child (null: not tracked)
 };
 
 // next frontier: tree node, window state and materialised residual of every kept child
@@ -583,6 +592,27 @@ __global__ __launch_bounds__(256) void k_gather(KSpec sp, GatherArgs a) {
     int k;
     make_child<NQ>(a.res + pi * E, a.pk[pi], sp.gates[g], le, s, k);
     a.res_next[p * E + le] = s;
+    // goal child: the node equals the pre-image of a candidate its windows allow.  One lane per candidate
+    // filters by exponent and nonzero pattern; the few that pass are compared exactly.
+    bool adj = false;
+    if (a.gflag) {
+      const uint64_t nzb = gballot<GS>(!zzero(s));
+      const int gp = a.g1[pi];
+      for (int b0 = 0; b0 < sp.G && !adj; b0 += GS) {
+        const int x = b0 + le;
+        uint64_t cm = gballot<GS>(x < sp.G && sp.xgk[x] == k && sp.xgm[x] == nzb);
+        while (cm && !adj) {
+          const int xx = b0 + __builtin_ctzll(cm);
+          cm &= cm - 1;
+          bool ok = sp.bad2[g * sp.G + xx] == 0;
+          if (ok && gp != 255) {
+            const uint32_t tri = ((uint32_t)gp * sp.G + g) * sp.G + xx;
+            ok = ((sp.bad3[tri >> 5] >> (tri & 31)) & 1) == 0;
+          }
+          if (ok) adj = gall<GS>(zeq(s, sp.xg[xx * E + le]));
+        }
+      }
+    }
     if (le == 0) {
       a.pk_next[p] = k;
       a.g1_next[p] = (uint8_t)g;
@@ -590,7 +620,8 @@ __global__ __launch_bounds__(256) void k_gather(KSpec sp, GatherArgs a) {
       a.tree_par[p] = (uint32_t)pi;
       a.tree_gate[p] = (uint8_t)g;
       a.inst_next[p] = a.pinst ? a.pinst[pi] : 0u;
-      a.pb_next[p] = a.surv_bw[a.kidx[p]];
+      a.pb_next[p] = a.surv_bw[a.kidx[p]] | (adj ? BW_GOAL : 0u);
+      if (adj) a.gflag[a.pinst ? a.pinst[pi] : 0u] = 1u;
     }
   }
 }
@@ -642,8 +673,9 @@ struct chz_ctx {
   chz::DevGate *d_gates = nullptr;
   uint8_t *d_bad2 = nullptr, *d_symperm = nullptr, *d_symconj = nullptr;
   uint32_t *d_bad3 = nullptr;
-  int4 *d_goal = nullptr;
-  int *d_cx = nullptr;
+  int4 *d_goal = nullptr, *d_xg = nullptr;
+  uint64_t *d_xgm = nullptr;
+  int *d_cx = nullptr, *d_xgk = nullptr;
   // frontier (current and next generation) and the tree of all levels
   chz::DBuf<int4> res[2];
   chz::DBuf<int> pk[2];
@@ -666,6 +698,7 @@ struct chz_ctx {
   int cur = 0, gstep = -1, nslots = 0;
   chz::DBuf<int> d_rem;
   chz::DBuf<uint8_t> d_keepi, d_active;
+  chz::DBuf<uint32_t> gflag;        // per slot: a node of the current frontier has a goal child
   chz::DBuf<unsigned long long> icnt, scan_tmp;
   chz::Pinned pin;                  // staging of the per-level control transfers
   // captured level steps (one per frontier parity) and the buffer addresses and schedule they were captured with
@@ -763,6 +796,8 @@ static void reserve_slots(chz_ctx *c, int need) {
   c->d_rem.need(ns);
   c->d_keepi.need(ns);
   c->d_active.need(ns);
+  c->gflag.need(ns);
+  HIPCHK(hipMemsetAsync(c->gflag.p, 0, (size_t)ns * 4, c->stream));   // fresh slots: no goal child known
   c->icnt.need((size_t)ns * IC_N);
   c->d_lo.need(ns);
   c->d_hi.need(ns);
@@ -893,7 +928,8 @@ static void step_graph(chz_ctx *c, int cur, long long m, const unsigned long lon
       (uintptr_t)c->bcount.p, (uintptr_t)c->surv_o.p, (uintptr_t)c->surv_key.p, (uintptr_t)c->surv_bw.p,
       (uintptr_t)c->kept.p, (uintptr_t)c->kidx.p, (uintptr_t)c->runs.p, (uintptr_t)c->sk[0].p, (uintptr_t)c->sk[1].p,
       (uintptr_t)c->sv[0].p, (uintptr_t)c->sv[1].p, (uintptr_t)c->hist.p, (uintptr_t)c->scan_tmp.p,
-      (uintptr_t)c->d_active.p, (uintptr_t)ns, (uintptr_t)sc.var, (uintptr_t)sc.cap, (uintptr_t)sc.tw};
+      (uintptr_t)c->d_active.p, (uintptr_t)c->gflag.p, (uintptr_t)ns, (uintptr_t)sc.var, (uintptr_t)sc.cap,
+      (uintptr_t)sc.tw};
   chz_ctx::StepGraph &sg = c->sgraph[cur];
   if (!sg.exec || sg.key != key) {
     if (sg.exec) { HIPCHK(hipGraphExecDestroy(sg.exec)); sg.exec = nullptr; }
@@ -903,7 +939,7 @@ static void step_graph(chz_ctx *c, int cur, long long m, const unsigned long lon
     HIPCHK(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
     ExpandArgs ea{c->res[cur].p, c->pk[cur].p, c->g1[cur].p, c->g2[cur].p, 0, ncap, 0, 0,
                   c->status.p, c->keys.p, c->info.p, c->pb[cur].p, c->cnt.p, pinst, c->d_rem.p, c->d_keepi.p,
-                  c->icnt.p, 1, dm, sc.tw};
+                  c->icnt.p, 1, dm, sc.tw, c->gflag.p};
     const dim3 grid(grid_for((cap * G + CPW - 1) / CPW, sc.cap));
     switch (sc.var) {
       case 0: hipLaunchKernelGGL((k_level<NQ, 4>), grid, dim3(256), 0, s, ks, ea, cap); break;
@@ -950,7 +986,8 @@ static void step_graph(chz_ctx *c, int cur, long long m, const unsigned long lon
   c->ev.end(c->stream);
   const GatherArgs ga{c->kidx.p, ncap, c->surv_o.p, c->res[cur].p, c->pk[cur].p, c->g1[cur].p,
                       c->res[nxt].p, c->pk[nxt].p, c->g1[nxt].p, c->g2[nxt].p, tp, tg, pinst, c->finst[nxt].p,
-                      c->surv_bw.p, c->pb[nxt].p, c->bcount.p + nbA};
+                      c->surv_bw.p, c->pb[nxt].p, c->bcount.p + nbA, c->gflag.p};
+  HIPCHK(hipMemsetAsync(c->gflag.p, 0, (size_t)ns * 4, c->stream));   // the graph's expansion has read them
   c->ev.begin(K_GATHER, c->stream);
   hipLaunchKernelGGL(k_gather<NQ>, dim3(grid_for((ncap + CPW - 1) / CPW)), dim3(256), 0, c->stream, ks, ga);
   c->ev.end(c->stream);
@@ -972,6 +1009,7 @@ template <int NQ> static void batch_step(chz_ctx *c) {
   const int G = ks.G;
   if (c->m == 0 && c->pending.empty()) return;
   if (c->m == 0) {   // a frontier of roots only
+    if (c->nslots) HIPCHK(hipMemsetAsync(c->gflag.p, 0, (size_t)c->nslots * 4, c->stream));
     const long long nr = (long long)c->pending.size();
     const int nxt = c->cur ^ 1;
     c->res[nxt].need(nr * E); c->pk[nxt].need(nr); c->g1[nxt].need(nr); c->g2[nxt].need(nr); c->finst[nxt].need(nr);
@@ -1059,7 +1097,7 @@ template <int NQ> static void batch_step(chz_ctx *c) {
     const int gsplit = (int)std::max<long long>(1, std::min<long long>(G, ((long long)sc.tw * CPW + cm - 1) / cm));
     ExpandArgs ea{c->res[cur].p, c->pk[cur].p, c->g1[cur].p, c->g2[cur].p, off, ncc, 0, 0,
                   c->status.p, c->keys.p, c->info.p, c->pb[cur].p, c->cnt.p, pinst, c->d_rem.p, c->d_keepi.p,
-                  c->icnt.p, gsplit, nullptr, 0};
+                  c->icnt.p, gsplit, nullptr, 0, c->gflag.p};
     const dim3 grid(grid_for((cm * gsplit + CPW - 1) / CPW, sc.cap));
     hipEvent_t te0 = nullptr, te1 = nullptr;
     if (c->tuning) {
@@ -1148,7 +1186,9 @@ template <int NQ> static void batch_step(chz_ctx *c) {
     distinct = (long long)*ptot;
   }
   }   // normal path
-  // next frontier: kept children, then the searches admitted since the last step
+  // next frontier: kept children, then the searches admitted since the last step.  The goal flags describe the
+  // frontier being expanded until here; the gather sets them for the next one.
+  if (!graphed) HIPCHK(hipMemsetAsync(c->gflag.p, 0, (size_t)ns * 4, c->stream));
   const long long mn = distinct + (long long)c->pending.size();
   std::vector<long long> cnts(c->nslots, 0);
   if (mn) {
@@ -1161,7 +1201,7 @@ template <int NQ> static void batch_step(chz_ctx *c) {
     if (distinct && !graphed) {
       const GatherArgs ga{c->kidx.p, distinct, c->surv_o.p, c->res[cur].p, c->pk[cur].p, c->g1[cur].p,
                           c->res[nxt].p, c->pk[nxt].p, c->g1[nxt].p, c->g2[nxt].p, tp, tg, pinst, c->finst[nxt].p,
-                          c->surv_bw.p, c->pb[nxt].p, nullptr};
+                          c->surv_bw.p, c->pb[nxt].p, nullptr, c->gflag.p};
       c->ev.begin(K_GATHER, c->stream);
       hipLaunchKernelGGL(k_gather<NQ>, dim3(grid_for((distinct + CPW - 1) / CPW)), dim3(256), 0, c->stream, ks, ga);
       c->ev.end(c->stream);
@@ -1303,6 +1343,36 @@ static void upload_spec(chz_ctx *c) {
   add((void **)&c->d_cx, cx.data(), cx.size() * 4);
   add((void **)&c->d_symperm, c->grp.bperm.data(), c->grp.bperm.size());
   add((void **)&c->d_symconj, c->grp.conj.data(), c->grp.conj.size());
+  // goal pre-images of every candidate, and the coefficient limit below which no child leaves int32: a child
+  // coefficient is a sum over the gate column of parent coefficients times the column's Z[w] coefficients
+  std::vector<int4> xg((size_t)G * N * N, make_int4(0, 0, 0, 0));
+  std::vector<uint64_t> xgm(G, 0);
+  std::vector<int> xgk(G, -1);
+  uint64_t gmax = 1;
+  for (int g = 0; g < G; g++) {
+    const HostGate &h = s.gates[g];
+    HostMat X;
+    if (N * N <= 64 && host_goal_preimage(s, g, X)) {
+      xgk[g] = X.k;
+      for (int i = 0; i < N * N; i++) {
+        const Zq &z = X.e[i];
+        xg[(size_t)g * N * N + i] = make_int4((int)z.a, (int)z.b, (int)z.c, (int)z.d);
+        if (z.a | z.b | z.c | z.d) xgm[g] |= 1ull << i;
+      }
+    }
+    const int L = 1 << h.nq;
+    for (int lc = 0; lc < L; lc++) {
+      uint64_t col = 0;
+      for (int m = 0; m < L; m++) {
+        const Zq &z = h.inv[m * L + lc];
+        col += (uint64_t)(std::llabs(z.a) + std::llabs(z.b) + std::llabs(z.c) + std::llabs(z.d));
+      }
+      gmax = std::max(gmax, col);
+    }
+  }
+  add((void **)&c->d_xg, xg.data(), xg.size() * sizeof(int4));
+  add((void **)&c->d_xgm, xgm.data(), xgm.size() * 8);
+  add((void **)&c->d_xgk, xgk.data(), xgk.size() * 4);
   c->spec_blob.need(blob.size());
   HIPCHK(hipMemcpy(c->spec_blob.p, blob.data(), blob.size(), hipMemcpyHostToDevice));
   for (auto &d : at) *d.first = c->spec_blob.p + d.second;
@@ -1336,6 +1406,8 @@ static void upload_spec(chz_ctx *c) {
   for (int i = 0; i < 4; i++) { k.wm[i] = (uint32_t)mulm(wp, R); wp = mulm(wp, w); }
   k.gates = c->d_gates; k.bad2 = c->d_bad2; k.bad3 = c->d_bad3; k.goal = c->d_goal; k.cxtable = c->d_cx;
   k.symperm = c->d_symperm; k.symconj = c->d_symconj;
+  k.xg = c->d_xg; k.xgm = c->d_xgm; k.xgk = c->d_xgk;
+  k.maglim = (uint32_t)(0x7fffffffull / gmax);
 }
 
 // a new search of the batch: the depth-0 cases end at once (src/cheese.cu:1794-1801), the rest wait to join
@@ -1656,7 +1728,7 @@ void chz_close(chz_ctx *c) {
     c->pb[b].release();
     c->sk[b].release(); c->sv[b].release();
   }
-  c->finst[0].release(); c->finst[1].release(); c->d_rem.release(); c->d_keepi.release(); c->d_active.release();
+  c->finst[0].release(); c->finst[1].release(); c->d_rem.release(); c->d_keepi.release(); c->d_active.release(); c->gflag.release();
   c->icnt.release(); c->scan_tmp.release(); c->d_lo.release(); c->d_hi.release();
   c->pin.release();
   for (auto &g : c->sgraph) if (g.exec) { (void)hipGraphExecDestroy(g.exec); g.exec = nullptr; }

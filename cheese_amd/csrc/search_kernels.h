@@ -174,9 +174,10 @@ template <int NQ> __device__ __forceinline__ int cut_rank(uint32_t fp, int le, i
 }
 
 // Bound word carried by every frontier node: bits 0-7 its raw T bound (BW_TUNK: not known), bits 8 + 3 i the
-// need of cut i (ceil log2 of its clipped rank, at most 2 for n <= 3), bit 31 set when the cut needs are known.
-// Roots carry BW_TUNK.  A child copies what its gate cannot change and computes the rest.
-constexpr uint32_t BW_TUNK = 0xffu, BW_CUTS = 1u << 31;
+// need of cut i (ceil log2 of its clipped rank, at most 2 for n <= 3), bit 30 set when one of its children is
+// the goal, bit 31 set when the cut needs are known.  Roots carry BW_TUNK.  A child copies what its gate cannot
+// change and computes the rest.
+constexpr uint32_t BW_TUNK = 0xffu, BW_CUTS = 1u << 31, BW_GOAL = 1u << 30, BW_NEEDS = 0x1fffff00u;
 constexpr int BW_MAXCUTS = 7;
 
 // CX/CZ lower bound from the cut ranks and the ILP table (src/cheese.cu:585-597).  Cuts in `keepc` take the

@@ -201,6 +201,40 @@ bool host_equal(const HostMat &S, const std::vector<Zq> &M, int k) {
   return true;
 }
 
+bool host_goal_preimage(const HostSpec &s, int gi, HostMat &X) {
+  const HostGate &g = s.gates[gi];
+  const int N = s.N(), L = 1 << g.nq;
+  if ((int)s.goal.size() != N * N || (int)g.fwd.size() != L * L) return false;
+  int mask = 0;
+  for (int j = 0; j < g.nq; j++) mask |= 1 << g.q[j];
+  // X = goal G: entry (r, c) = sum_m goal[r][x_m] fwd[m][lc], x_m = c with the gate's bits set to m
+  X.e.assign((size_t)N * N, Zq{0, 0, 0, 0});
+  X.k = s.gk + g.gk;
+  X.ovf = false;
+  for (int r = 0; r < N; r++)
+    for (int c = 0; c < N; c++) {
+      int lc = 0;
+      for (int j = 0; j < g.nq; j++) lc |= ((c >> g.q[j]) & 1) << j;
+      Zq acc{0, 0, 0, 0};
+      for (int m = 0; m < L; m++) {
+        int x = c & ~mask;
+        for (int j = 0; j < g.nq; j++) if ((m >> j) & 1) x |= 1 << g.q[j];
+        const Zq p = zq_mul(s.goal[r * N + x], g.fwd[m * L + lc]);
+        acc = {acc.a + p.a, acc.b + p.b, acc.c + p.c, acc.d + p.d};
+      }
+      X.e[r * N + c] = acc;
+    }
+  while (X.k > 0 && std::all_of(X.e.begin(), X.e.end(), zq_sqrt2_divides)) {
+    for (Zq &z : X.e) z = zq_div_sqrt2(z);
+    X.k--;
+  }
+  for (const Zq &z : X.e)
+    if (z.a != (int32_t)z.a || z.b != (int32_t)z.b || z.c != (int32_t)z.c || z.d != (int32_t)z.d) return false;
+  HostMat Y = X;
+  host_apply_inverse(Y, g, s.n);
+  return !Y.ovf && host_equal(Y, s.goal, s.gk);
+}
+
 static bool zq_vec_eq(const std::vector<Zq> &a, const std::vector<Zq> &b) {
   if (a.size() != b.size()) return false;
   for (size_t i = 0; i < a.size(); i++)

@@ -50,6 +50,9 @@ SymGroup build_symmetry(const HostSpec &s, bool on);
 struct HostMat { std::vector<Zq> e; int k = 0; bool ovf = false; };
 void host_apply_inverse(HostMat &S, const HostGate &g, int n);   // S <- S g^-1, normalised
 bool host_equal(const HostMat &S, const std::vector<Zq> &M, int k);
+// X with X g^-1 = goal, in normal form: the one residual from which candidate gi reaches the goal.  False when
+// the product check fails or a coefficient leaves int32.
+bool host_goal_preimage(const HostSpec &s, int gi, HostMat &X);
 
 // two specs share everything the search kernels read except the target and the depth: qubit count, mode,
 // candidates and their matrices, windows, cuts, cx table, bound flags and goal (searches of such specs can run

@@ -15,3 +15,17 @@ extern "C" int shim_gate_info(const char *spec, long len, int g, int *out, int c
   for (int i = 0; i < nc; i++) out[2 + i] = chz::gate_cut_rank(s.gates[g], s.cutmask[i]);
   return nc;
 }
+
+// goal pre-image of candidate g: its sqrt2 exponent (or -1) and N x N entries as (a, b, c, d) in out
+extern "C" int shim_goal_preimage(const char *spec, long len, int g, long long *out, int cap) {
+  chz::HostSpec s;
+  std::string err;
+  if (chz::parse_spec(std::string(spec, (size_t)len), s, err) || g < 0 || g >= s.G()) return -2;
+  chz::HostMat X;
+  if (!chz::host_goal_preimage(s, g, X)) return -1;
+  if (cap < 4 * (int)X.e.size()) return -2;
+  for (size_t i = 0; i < X.e.size(); i++) {
+    out[4 * i] = X.e[i].a; out[4 * i + 1] = X.e[i].b; out[4 * i + 2] = X.e[i].c; out[4 * i + 3] = X.e[i].d;
+  }
+  return X.k;
+}

@@ -29,6 +29,9 @@ def shim(tmp_path_factory):
                     os.path.join(HERE, "host_shim.cpp"), os.path.join(ROOT, "cheese_amd", "csrc", "spec.cpp")],
                    check=True)
     L = ctypes.CDLL(str(out))
+    L.shim_goal_preimage.restype = ctypes.c_int
+    L.shim_goal_preimage.argtypes = [ctypes.c_char_p, ctypes.c_long, ctypes.c_int, ctypes.POINTER(ctypes.c_longlong),
+                                     ctypes.c_int]
     L.shim_gate_info.restype = ctypes.c_int
     L.shim_gate_info.argtypes = [ctypes.c_char_p, ctypes.c_long, ctypes.c_int, ctypes.POINTER(ctypes.c_int),
                                  ctypes.c_int]
@@ -98,3 +101,23 @@ def test_known_exponents_and_ranks(shim):
         assert out[1] == e, name
         if name in ("cx", "ccx"):
             assert all(out[2 + q] == 2 for q in range(nq)), name
+
+
+@pytest.mark.parametrize("n", [2, 3])
+def test_goal_preimage_is_the_candidate_itself(shim, n):
+    """the residual from which one candidate reaches the goal (the identity) is that candidate's own matrix:
+    closed form, built here from the gate tables of the workload generator, not from the spec's matrices.  A
+    residual one gate short of the identity is what flags a parent whose level ends the search."""
+    skel = W.skeleton(n)
+    spec = W.spec_from_skeleton(skel, W.unitary(n, []), 5).encode()
+    names = [ln.split()[0] for ln in skel.splitlines() if ln.startswith("L_")]
+    N = 1 << n
+    buf = (ctypes.c_longlong * (4 * N * N))()
+    for g, name in enumerate(names):
+        parts = name.split("_")
+        gate, qs = parts[1], [int(q) for q in parts[2:]]
+        k = shim.shim_goal_preimage(spec, len(spec), g, buf, len(buf))
+        want_k, want = W.unitary(n, [(gate, qs)])
+        assert k == want_k, name
+        got = [tuple(buf[4 * i:4 * i + 4]) for i in range(N * N)]
+        assert got == [x for row in want for x in row], name

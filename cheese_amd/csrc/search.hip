@@ -572,9 +572,13 @@ struct GatherArgs {
   uint32_t *pb_next;
   const unsigned long long *nkd;  // kept count read on the device (captured step), or null
   uint32_t *gflag;                // per slot: set when a kept child has a goal child (null: not tracked)
+  int tn;                         // kept nodes per wave tile (a multiple of the nodes per wave, at most 64)
 };
 
-// next frontier: tree node, window state and materialised residual of every kept child
+// next frontier: tree node, window state and materialised residual of every kept child.  A wave takes tiles
+// of a.tn consecutive kept nodes: one lane per node resolves the node's index chain (kept rank -> survivor ->
+// parent and gate) and writes its small fields, then the lane groups build the tile's residuals from the
+// parents, with the chain already in registers.
 template <int NQ>
 __global__ __launch_bounds__(256) void k_gather(KSpec sp, GatherArgs a) {
   constexpr int N = 1 << NQ, E = N * N, GS = E, CPW = 64 / GS;
@@ -582,48 +586,71 @@ __global__ __launch_bounds__(256) void k_gather(KSpec sp, GatherArgs a) {
   const long long wave = ((long long)blockIdx.x * blockDim.x + threadIdx.x) >> 6;
   const long long nwaves = ((long long)gridDim.x * blockDim.x) >> 6;
   const long long nk = dev_n(a.nk, a.nkd);
-  for (long long base = wave * CPW; base < nk; base += nwaves * CPW) {
-    const long long p = base + slot;
-    if (p >= nk) continue;
-    const uint64_t o = a.surv_o[a.kidx[p]];
-    const long long pi = (long long)(o / (uint64_t)sp.G);
-    const int g = (int)(o % (uint64_t)sp.G);
-    int4 s;
-    int k;
-    make_child<NQ>(a.res + pi * E, a.pk[pi], sp.gates[g], le, s, k);
-    a.res_next[p * E + le] = s;
-    // goal child: the node equals the pre-image of a candidate its windows allow.  One lane per candidate
-    // filters by exponent and nonzero pattern; the few that pass are compared exactly.
-    bool adj = false;
-    if (a.gflag) {
-      const uint64_t nzb = gballot<GS>(!zzero(s));
-      const int gp = a.g1[pi];
-      for (int b0 = 0; b0 < sp.G && !adj; b0 += GS) {
-        const int x = b0 + le;
-        uint64_t cm = gballot<GS>(x < sp.G && sp.xgk[x] == k && sp.xgm[x] == nzb);
-        while (cm && !adj) {
-          const int xx = b0 + __builtin_ctzll(cm);
-          cm &= cm - 1;
-          bool ok = sp.bad2[g * sp.G + xx] == 0;
-          if (ok && gp != 255) {
-            const uint32_t tri = ((uint32_t)gp * sp.G + g) * sp.G + xx;
-            ok = ((sp.bad3[tri >> 5] >> (tri & 31)) & 1) == 0;
+  const int tn = a.tn;
+  for (long long tb = wave * tn; tb < nk; tb += nwaves * tn) {
+    const long long pl = tb + lane;
+    uint32_t pi_l = 0, bw_l = 0;
+    int g_l = 0, gp_l = 255;
+    if (lane < tn && pl < nk) {
+      const uint32_t kid = a.kidx[pl];
+      const uint64_t o = a.surv_o[kid];
+      pi_l = (uint32_t)(o / (uint64_t)sp.G);
+      g_l = (int)(o % (uint64_t)sp.G);
+      bw_l = a.surv_bw[kid];
+      gp_l = a.g1[pi_l];
+      a.g1_next[pl] = (uint8_t)g_l;
+      a.g2_next[pl] = (uint8_t)gp_l;
+      a.tree_par[pl] = pi_l;
+      a.tree_gate[pl] = (uint8_t)g_l;
+      a.inst_next[pl] = a.pinst ? a.pinst[pi_l] : 0u;
+    }
+    for (int j = 0; j < tn; j += CPW) {
+      const int src = j + slot;
+      const long long pi = (long long)(uint32_t)__shfl((int)pi_l, src);
+      const int g = __shfl(g_l, src), gp = __shfl(gp_l, src);
+      const uint32_t bwp = (uint32_t)__shfl((int)bw_l, src);
+      const long long p = tb + src;
+      if (src >= tn || p >= nk) continue;
+      int4 s;
+      int k;
+      make_child<NQ>(a.res + pi * E, a.pk[pi], sp.gates[g], le, s, k);
+      a.res_next[p * E + le] = s;
+      // goal child: the node equals the pre-image of a candidate its windows allow.  One lane per candidate
+      // filters by exponent and nonzero pattern; the few that pass are compared exactly.
+      bool adj = false;
+      if (a.gflag) {
+        const uint64_t nzb = gballot<GS>(!zzero(s));
+        for (int b0 = 0; b0 < sp.G && !adj; b0 += GS) {
+          const int x = b0 + le;
+          uint64_t cm = gballot<GS>(x < sp.G && sp.xgk[x] == k && sp.xgm[x] == nzb);
+          while (cm && !adj) {
+            const int xx = b0 + __builtin_ctzll(cm);
+            cm &= cm - 1;
+            bool ok = sp.bad2[g * sp.G + xx] == 0;
+            if (ok && gp != 255) {
+              const uint32_t tri = ((uint32_t)gp * sp.G + g) * sp.G + xx;
+              ok = ((sp.bad3[tri >> 5] >> (tri & 31)) & 1) == 0;
+            }
+            if (ok) adj = gall<GS>(zeq(s, sp.xg[xx * E + le]));
           }
-          if (ok) adj = gall<GS>(zeq(s, sp.xg[xx * E + le]));
         }
       }
-    }
-    if (le == 0) {
-      a.pk_next[p] = k;
-      a.g1_next[p] = (uint8_t)g;
-      a.g2_next[p] = a.g1[pi];
-      a.tree_par[p] = (uint32_t)pi;
-      a.tree_gate[p] = (uint8_t)g;
-      a.inst_next[p] = a.pinst ? a.pinst[pi] : 0u;
-      a.pb_next[p] = a.surv_bw[a.kidx[p]] | (adj ? BW_GOAL : 0u);
-      if (adj) a.gflag[a.pinst ? a.pinst[pi] : 0u] = 1u;
+      if (le == 0) {
+        a.pk_next[p] = k;
+        a.pb_next[p] = bwp | (adj ? BW_GOAL : 0u);
+        if (adj) a.gflag[a.pinst ? a.pinst[pi] : 0u] = 1u;
+      }
     }
   }
+}
+
+// nodes per gather tile: one per lane group while the launch has enough waves for all of them, up to 64
+template <int NQ> static int gather_tile(long long nk) {
+  constexpr int CPW = 64 / (1 << (2 * NQ));
+  const long long waves = 8192ll * 4;
+  long long tn = (nk + waves - 1) / waves;
+  tn = (tn + CPW - 1) / CPW * CPW;
+  return (int)std::max<long long>(CPW, std::min<long long>(64, tn));
 }
 
 
@@ -986,7 +1013,7 @@ static void step_graph(chz_ctx *c, int cur, long long m, const unsigned long lon
   c->ev.end(c->stream);
   const GatherArgs ga{c->kidx.p, ncap, c->surv_o.p, c->res[cur].p, c->pk[cur].p, c->g1[cur].p,
                       c->res[nxt].p, c->pk[nxt].p, c->g1[nxt].p, c->g2[nxt].p, tp, tg, pinst, c->finst[nxt].p,
-                      c->surv_bw.p, c->pb[nxt].p, c->bcount.p + nbA, c->gflag.p};
+                      c->surv_bw.p, c->pb[nxt].p, c->bcount.p + nbA, c->gflag.p, CPW};
   HIPCHK(hipMemsetAsync(c->gflag.p, 0, (size_t)ns * 4, c->stream));   // the graph's expansion has read them
   c->ev.begin(K_GATHER, c->stream);
   hipLaunchKernelGGL(k_gather<NQ>, dim3(grid_for((ncap + CPW - 1) / CPW)), dim3(256), 0, c->stream, ks, ga);
@@ -1201,9 +1228,10 @@ template <int NQ> static void batch_step(chz_ctx *c) {
     if (distinct && !graphed) {
       const GatherArgs ga{c->kidx.p, distinct, c->surv_o.p, c->res[cur].p, c->pk[cur].p, c->g1[cur].p,
                           c->res[nxt].p, c->pk[nxt].p, c->g1[nxt].p, c->g2[nxt].p, tp, tg, pinst, c->finst[nxt].p,
-                          c->surv_bw.p, c->pb[nxt].p, nullptr, c->gflag.p};
+                          c->surv_bw.p, c->pb[nxt].p, nullptr, c->gflag.p, gather_tile<NQ>(distinct)};
       c->ev.begin(K_GATHER, c->stream);
-      hipLaunchKernelGGL(k_gather<NQ>, dim3(grid_for((distinct + CPW - 1) / CPW)), dim3(256), 0, c->stream, ks, ga);
+      hipLaunchKernelGGL(k_gather<NQ>, dim3(grid_for((distinct + ga.tn - 1) / ga.tn)), dim3(256), 0, c->stream, ks,
+                         ga);
       c->ev.end(c->stream);
       c->k_work[K_GATHER] += (double)distinct * (E * 16.0 * 2.0 + 16.0);
     }

@@ -460,7 +460,7 @@ struct DedupeArgs {
   const uint32_t *pinst;          // parents' instance slots (null: one instance)
   const unsigned long long *nd;   // survivors read on the device (captured step), or null
 };
-constexpr int MAXREP = 32;
+constexpr int MAXREP = 32, MAXSYM = 32;
 
 // value of lane q of the lane group (q uniform across the group)
 template <int GS> __device__ __forceinline__ uint32_t gget(uint32_t v, int q) {
@@ -472,12 +472,25 @@ template <int GS> __device__ __forceinline__ uint64_t gget64(uint64_t v, int q) 
 }
 
 template <int NQ>
-__global__ __launch_bounds__(256) void k_dedupe_runs(KSpec sp, DedupeArgs a) {
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(6))) void k_dedupe_runs(KSpec sp,
+                                                                                              DedupeArgs a) {
   constexpr int N = 1 << NQ, E = N * N, GS = E, CPW = 64 / GS;
   constexpr int GPB = 256 / GS;     // groups per block
   __shared__ uint64_t rep_key[GPB][MAXREP], rep_o[GPB][MAXREP];
   __shared__ uint32_t rep_inst[GPB][MAXREP];
-  (void)N;
+  __shared__ uint8_t spos[MAXSYM * E];
+  __shared__ int4 rep_res[GPB][2][E];     // the first two representatives' residuals
+  __shared__ int4 par_stage[GPB][E];      // the current member's parent residual
+  // the symmetry group's entry maps, staged once per block (the class test tries the elements in order)
+  const int nsym = sp.nsym < MAXSYM ? sp.nsym : MAXSYM;
+  const bool lds_sym = nsym == sp.nsym;
+  uint32_t cjm = 0;
+  for (int i = threadIdx.x; i < nsym * E; i += blockDim.x) {
+    const int y = i / E, e = i % E;
+    spos[i] = (uint8_t)(sp.symperm[y * N + (e >> NQ)] * N + sp.symperm[y * N + (e & (N - 1))]);
+  }
+  for (int y = 0; y < nsym; y++) cjm |= (uint32_t)(sp.symconj[y] != 0) << y;
+  __syncthreads();
   const int lane = threadIdx.x & 63, le = lane & (GS - 1), slot = lane / GS;
   const int grp = (threadIdx.x >> 6) * CPW + slot;
   const long long wave = ((long long)blockIdx.x * blockDim.x + threadIdx.x) >> 6;
@@ -491,12 +504,12 @@ __global__ __launch_bounds__(256) void k_dedupe_runs(KSpec sp, DedupeArgs a) {
     const long long i0 = a.runs[ri];
     const uint32_t key = a.k[i0];
     int nrep = 0;
-    // the first two representatives' residuals stay in registers: a run is almost always one class, whose
-    // members are then each rebuilt once and compared with the cached first representative
-    int4 c0 = make_int4(0, 0, 0, 0), c1 = c0;
+    // a run is almost always one class: its members are each rebuilt once and compared with the first
+    // representative, whose residual (and the second's) stays in LDS
     int ck0 = 0, ck1 = 0;
     // members in chunks of GS: lane le loads member j0 + le's rank, child, key, parent exponent and slot at once,
-    // so each member then waits on one load (its parent's residual) instead of a chain of three
+    // so each member then waits on one load (its parent's residual), issued while the member before it is
+    // compared
     for (long long j0 = i0; j0 < n; j0 += GS) {
       const long long jj = j0 + le;
       const bool in = jj < n && a.k[jj] == key;
@@ -513,28 +526,36 @@ __global__ __launch_bounds__(256) void k_dedupe_runs(KSpec sp, DedupeArgs a) {
         m_pk = a.pk[pi];
         m_inst = a.pinst ? a.pinst[pi] : 0u;
       }
+      int4 nextv = make_int4(0, 0, 0, 0);
+      if (nm > 0) nextv = a.res[(long long)(gget64<GS>(m_o, 0) / G) * E + le];
       for (int q = 0; q < nm; q++) {
         const uint32_t rank = gget<GS>(m_rank, q), inst = gget<GS>(m_inst, q);
         const uint64_t o = gget64<GS>(m_o, q), key64 = gget64<GS>(m_key, q);
         const int pk = (int)gget<GS>((uint32_t)m_pk, q);
-        const long long pi = (long long)(o / G);
         const int g = (int)(o % G);
+        __builtin_amdgcn_wave_barrier();
+        par_stage[grp][le] = nextv;
+        __builtin_amdgcn_wave_barrier();
+        if (q + 1 < nm) nextv = a.res[(long long)(gget64<GS>(m_o, q + 1) / G) * E + le];
         int4 s;
         int ks;
-        make_child<NQ>(a.res + pi * E, pk, sp.gates[g], le, s, ks);
+        make_child<NQ>(par_stage[grp], pk, sp.gates[g], le, s, ks);
         bool dup = false;
         for (int r = 0; r < nrep && !dup; r++) {
           if (rep_key[grp][r] != key64 || rep_inst[grp][r] != inst) continue;
-          int4 b;
-          int kb;
-          if (r == 0) { b = c0; kb = ck0; }
-          else if (r == 1) { b = c1; kb = ck1; }
-          else {
-            const uint64_t ro = rep_o[grp][r];
-            const long long rpi = (long long)(ro / G);
-            make_child<NQ>(a.res + rpi * E, a.pk[rpi], sp.gates[(int)(ro % G)], le, b, kb);
+          if (r < 2 && lds_sym) {
+            dup = same_class_rep<NQ>(s, ks, rep_res[grp][r], r ? ck1 : ck0, le, spos, cjm, nsym);
+          } else {
+            int4 b;
+            int kb;
+            if (r < 2) { b = rep_res[grp][r][le]; kb = r ? ck1 : ck0; }
+            else {
+              const uint64_t ro = rep_o[grp][r];
+              const long long rpi = (long long)(ro / G);
+              make_child<NQ>(a.res + rpi * E, a.pk[rpi], sp.gates[(int)(ro % G)], le, b, kb);
+            }
+            dup = lds_sym ? same_class_lds<NQ>(s, ks, b, kb, le, spos, cjm, nsym) : same_class<NQ>(s, ks, b, kb, le, sp);
           }
-          dup = same_class<NQ>(s, ks, b, kb, le, sp);
         }
         if (dup) { if (le == 0) a.kept[rank] = 0; }
         else if (nrep < MAXREP) {
@@ -543,8 +564,11 @@ __global__ __launch_bounds__(256) void k_dedupe_runs(KSpec sp, DedupeArgs a) {
             rep_o[grp][nrep] = o;
             rep_inst[grp][nrep] = inst;
           }
-          if (nrep == 0) { c0 = s; ck0 = ks; }
-          else if (nrep == 1) { c1 = s; ck1 = ks; }
+          if (nrep < 2) {
+            rep_res[grp][nrep][le] = s;
+            if (nrep == 0) ck0 = ks;
+            else ck1 = ks;
+          }
           nrep++;
         } else if (le == 0) atomicAdd(&a.cnt[C_REPOVF], 1ull);
         __builtin_amdgcn_wave_barrier();

@@ -466,6 +466,35 @@ __device__ __forceinline__ uint64_t class_key3(int4 s, int k, int lane, const ui
 }
 
 // exact: B = sigma(A) for some group element (src/cheese.cu:242-255)
+// the same test with the group's entry maps staged in LDS: spos[y * E + e] is the entry of b that lands on entry e
+// under group element y, bit y of cjm its conjugation flag
+template <int NQ> __device__ __forceinline__ bool same_class_lds(int4 a, int ka, int4 b, int kb, int le,
+                                                                const uint8_t *spos, uint32_t cjm, int nsym) {
+  constexpr int N = 1 << NQ, GS = N * N;
+  if (ka != kb) return false;
+  const int4 ac = zconj(a);
+  for (int y = 0; y < nsym; y++) {
+    const int4 bv = gshfl<GS>(b, spos[y * GS + le]);
+    const int4 av = (cjm >> y) & 1 ? ac : a;
+    if (gall<GS>(zeq(av, bv))) return true;
+  }
+  return false;
+}
+
+// the same test against a residual held in LDS (rep: E entries), read through the staged maps
+template <int NQ> __device__ __forceinline__ bool same_class_rep(int4 a, int ka, const int4 *rep, int kb, int le,
+                                                                const uint8_t *spos, uint32_t cjm, int nsym) {
+  constexpr int GS = (1 << NQ) * (1 << NQ);
+  if (ka != kb) return false;
+  const int4 ac = zconj(a);
+  for (int y = 0; y < nsym; y++) {
+    const int4 bv = rep[spos[y * GS + le]];
+    const int4 av = (cjm >> y) & 1 ? ac : a;
+    if (gall<GS>(zeq(av, bv))) return true;
+  }
+  return false;
+}
+
 template <int NQ> __device__ __forceinline__ bool same_class(int4 a, int ka, int4 b, int kb, int le, const KSpec &sp) {
   constexpr int N = 1 << NQ, GS = N * N;
   if (ka != kb) return false;

@@ -221,6 +221,8 @@ def main():
                     help="also build the 3-qubit Clifford table and answer 1e5 queries")
     ap.add_argument("--tcount", action=argparse.BooleanOptionalAction, default=True,
                     help="also build the 3-qubit T-count table and search 256 batched targets")
+    ap.add_argument("--d9", action=argparse.BooleanOptionalAction, default=True,
+                    help="also time the Toffoli budget-9 proof")
     ap.add_argument("--tune", action=argparse.BooleanOptionalAction, default=True,
                     help="autotune the expansion schedule on the workload before the timed proofs")
     ap.add_argument("--serve", type=int, default=400, metavar="N",
@@ -295,7 +297,8 @@ def main():
         "kernels_ms_per_step": {k: round(v["ms"] / a.steps, 3) for k, v in kstats.items()},
         "autotune": {b: [v["waves_per_eu"], v["grid_cap"], v["split_waves"]] for b, v in tuned.items()} if tuned else None,
     }
-    line["toffoli_d9"] = toffoli_d9()
+    if a.d9:
+        line["toffoli_d9"] = toffoli_d9()
     if a.random:
         line["random32"] = random_circuits()
     if a.clifford or a.tcount or a.serve:

@@ -425,26 +425,44 @@ __global__ void k_fill_kept(uint8_t *kept, long long n, const unsigned long long
     kept[i] = 1;
 }
 
-__global__ void k_run_heads(const uint32_t *k, long long n, uint32_t *runs, unsigned long long *cnt,
-                            const unsigned long long *nd) {
+// run starts (a position whose key differs from the one before and equals the one after), appended in tiles:
+// each thread tests RH_ITEMS positions, a block scan places the tile's starts and one atomic per tile reserves
+// their slots (the runs' order does not matter: each is deduplicated on its own)
+constexpr int RH_ITEMS = 16;
+__global__ __launch_bounds__(256) void k_run_heads(const uint32_t *k, long long n, uint32_t *runs,
+                                                   unsigned long long *cnt, const unsigned long long *nd) {
   n = dev_n(n, nd);
-  // one atomic per wave (the runs' order does not matter: each is deduplicated on its own)
-  const int lane = threadIdx.x & 63;
-  const long long stride = (long long)gridDim.x * blockDim.x;
-  for (long long b0 = (long long)blockIdx.x * blockDim.x + (threadIdx.x & ~63); b0 < n; b0 += stride) {
-    const long long i = b0 + lane;
-    bool start = false;
-    if (i < n) {
-      const bool head = i == 0 || k[i] != k[i - 1];
-      start = head && i + 1 < n && k[i + 1] == k[i];
+  __shared__ unsigned long long s_wave[4], s_base;
+  const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
+  const long long tile = 256ll * RH_ITEMS;
+  for (long long t0 = (long long)blockIdx.x * tile; t0 < n; t0 += (long long)gridDim.x * tile) {
+    uint32_t mask = 0;
+    for (int it = 0; it < RH_ITEMS; it++) {
+      const long long i = t0 + it * 256ll + tid;
+      if (i < n) {
+        const uint32_t ki = k[i];
+        const bool head = i == 0 || k[i - 1] != ki;
+        if (head && i + 1 < n && k[i + 1] == ki) mask |= 1u << it;
+      }
     }
-    const uint64_t bal = __ballot(start);
-    if (!bal) continue;
-    const int leader = __ffsll((long long)bal) - 1;
-    unsigned long long at = 0;
-    if (lane == leader) at = atomicAdd(&cnt[C_RUNS], (unsigned long long)__popcll(bal));
-    at = (unsigned long long)__shfl((long long)at, leader);
-    if (start) runs[at + __popcll(bal & ((1ull << lane) - 1))] = (uint32_t)i;
+    const uint32_t c = (uint32_t)__popc(mask);
+    uint32_t x = c;                                  // inclusive scan of the starts over the wave
+    for (int d = 1; d < 64; d <<= 1) {
+      const uint32_t y = (uint32_t)__shfl_up((int)x, d);
+      if (lane >= d) x += y;
+    }
+    if (lane == 63) s_wave[w] = x;
+    __syncthreads();
+    if (tid == 0) {
+      unsigned long long tot = 0;
+      for (int j = 0; j < 4; j++) { const unsigned long long v = s_wave[j]; s_wave[j] = tot; tot += v; }
+      s_base = tot ? atomicAdd(&cnt[C_RUNS], tot) : 0ull;
+    }
+    __syncthreads();
+    unsigned long long pos = s_base + s_wave[w] + (x - c);
+    for (int it = 0; it < RH_ITEMS; it++)
+      if ((mask >> it) & 1u) runs[pos++] = (uint32_t)(t0 + it * 256ll + tid);
+    __syncthreads();                                 // s_wave and s_base are reused by the next tile
   }
 }
 
@@ -1008,8 +1026,8 @@ static void step_graph(chz_ctx *c, int cur, long long m, const unsigned long lon
     hipError_t e = radix_sort_pairs(c->sk[0].p, c->sv[0].p, c->sk[1].p, c->sv[1].p, ncap, c->hist.p, c->scan_tmp.p, s,
                                     dsurv);
     hipLaunchKernelGGL(k_fill_kept, dim3(grid_for(ncap / 256 + 1)), dim3(256), 0, s, c->kept.p, ncap, dsurv);
-    hipLaunchKernelGGL(k_run_heads, dim3(grid_for(ncap / 64 + 1)), dim3(256), 0, s, c->sk[0].p, ncap, c->runs.p,
-                       c->cnt.p, dsurv);
+    hipLaunchKernelGGL(k_run_heads, dim3(grid_for(ncap / (64 * RH_ITEMS) + 1)), dim3(256), 0, s, c->sk[0].p, ncap,
+                       c->runs.p, c->cnt.p, dsurv);
     const DedupeArgs da{c->sk[0].p, c->sv[0].p, ncap, c->surv_o.p, c->surv_key.p, c->runs.p, c->res[cur].p,
                         c->pk[cur].p, c->kept.p, c->cnt.p, pinst, dsurv};
     hipLaunchKernelGGL(k_dedupe_runs<NQ>, dim3(grid_for(ncap / CPW + 1, 4096)), dim3(256), 0, s, ks, da);
@@ -1211,7 +1229,7 @@ template <int NQ> static void batch_step(chz_ctx *c) {
       HIPCHK(hipMemsetAsync(c->cnt.p + C_RUNS, 0, 8, c->stream));
       hipLaunchKernelGGL(k_fill_kept, dim3(grid_for(nsurv / 256 + 1)), dim3(256), 0, c->stream, c->kept.p, nsurv,
                          nullptr);
-      hipLaunchKernelGGL(k_run_heads, dim3(grid_for(nsurv / 64 + 1)), dim3(256), 0, c->stream, c->sk[0].p, nsurv,
+      hipLaunchKernelGGL(k_run_heads, dim3(grid_for(nsurv / (64 * RH_ITEMS) + 1)), dim3(256), 0, c->stream, c->sk[0].p, nsurv,
                          c->runs.p, c->cnt.p, nullptr);
       const DedupeArgs da{c->sk[0].p, c->sv[0].p, nsurv, c->surv_o.p, c->surv_key.p, c->runs.p,
                           c->res[cur].p, c->pk[cur].p, c->kept.p, c->cnt.p, pinst, nullptr};

@@ -490,8 +490,8 @@ template <int GS> __device__ __forceinline__ uint64_t gget64(uint64_t v, int q) 
 }
 
 template <int NQ>
-__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(6))) void k_dedupe_runs(KSpec sp,
-                                                                                              DedupeArgs a) {
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(NQ > 1 ? 6 : 1)))
+void k_dedupe_runs(KSpec sp, DedupeArgs a) {
   constexpr int N = 1 << NQ, E = N * N, GS = E, CPW = 64 / GS;
   constexpr int GPB = 256 / GS;     // groups per block
   __shared__ uint64_t rep_key[GPB][MAXREP], rep_o[GPB][MAXREP];

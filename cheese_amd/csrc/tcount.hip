@@ -339,8 +339,24 @@ struct DedupeArgs {
   unsigned long long *stat;    // [0] children visited in runs, [1] exact comparisons, [2] duplicates
 };
 
+// A child (pa, p) whose parent is pa = (gp, q) with p < q and P_p, P_q commuting is the same matrix as the child
+// (pb, q) of pa's sibling pb = (gp, p): R(P_p) and R(P_q) commute.  pb precedes pa (siblings are ordered by
+// rotation, between them at most q - p - 1 others), so that child has the smaller index and this one is a
+// duplicate whenever pb is a node of the level (it may have been dropped as a duplicate or as a coset seen at a
+// lower level).  One lane per candidate position of pb.
+__device__ bool swap_duplicate(const Forest &F, const Rot &R, int t, uint32_t pa, int p, int lane) {
+  if (t < 1) return false;
+  const int q = F.rot[t][pa];
+  if (p >= q || ((R.anti[p - 1] >> q) & 1)) return false;
+  const uint32_t gp = F.par[t][pa];
+  const long long i = (long long)pa - 1 - lane;
+  const bool hit = lane < q - p && i >= 0 && F.par[t][i] == gp && F.rot[t][i] == p;
+  return __ballot(hit) != 0;
+}
+
 // exact dedupe: within each run of equal prefixes (children in increasing index), the first child of every
-// exact coset stays (src/tcount.cu:327-360: the smallest index of each coset)
+// exact coset stays (src/tcount.cu:327-360: the smallest index of each coset).  Commuting swaps of an earlier
+// child are marked without a comparison.
 template <int NQ> __global__ __launch_bounds__(64) void k_dedupe(DedupeArgs a) {
   constexpr int P = 1 << (2 * NQ), G = P - 1;
   __shared__ uint32_t A[P * P], B[P * P];
@@ -361,7 +377,7 @@ template <int NQ> __global__ __launch_bounds__(64) void k_dedupe(DedupeArgs a) {
       const uint32_t ch = a.v32[j];
       const uint64_t key = a.keys[ch];
       const uint32_t inst = inst_of(a.F, a.t, parent(ch));
-      bool dup = false, built = false;
+      bool dup = swap_duplicate(a.F, a.R, a.t, parent(ch), (int)(ch % G) + 1, lane), built = false;
       uint64_t ka = 0;
       for (int q = 0; q < nrep && !dup; q++) {
         const uint32_t rc = rep[q];

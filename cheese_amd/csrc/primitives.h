@@ -104,6 +104,52 @@ __global__ __launch_bounds__(CT_THREADS) void k_scatter_idx(F f, long long n, co
   }
 }
 
+// one tile (n <= CT_TILE) in one launch: flagged positions -> ordered index list, total written to *tot
+template <typename F>
+__global__ __launch_bounds__(CT_THREADS) void k_compact_one(F f, long long n, uint32_t *out, unsigned long long *tot) {
+  __shared__ int ws[CT_THREADS / 64];
+  long long pos = 0;
+  for (int i = 0; i < CT_ITEMS; i++) {
+    const long long x = (long long)i * CT_THREADS + threadIdx.x;
+    const bool fl = x < n && f(x);
+    int ex;
+    const int t = block_flag_scan(fl, ex, ws);
+    if (fl) out[pos + ex] = (uint32_t)x;
+    pos += t;
+  }
+  if (threadIdx.x == 0) *tot = (unsigned long long)pos;
+}
+
+// ------------------------------------------------------------------------------------------------------
+// small sorts in one launch: n <= SS_MAX (key prefix, rank) pairs as 64-bit composites (prefix << 32 | rank,
+// all distinct) by a bitonic network in LDS.  Ascending composites are the stable order of the prefixes, so the
+// result is the radix sort's.
+constexpr int SS_MAX = 4096;
+__attribute__((unused)) __global__ __launch_bounds__(1024) void k_sort_small(const uint64_t *key, long long n,
+                                                                           uint32_t *kout, uint32_t *vout) {
+  __shared__ uint64_t s[SS_MAX];
+  int np = 1;
+  while (np < n) np <<= 1;
+  for (int i = threadIdx.x; i < np; i += blockDim.x)
+    s[i] = i < n ? ((key[i] >> 32) << 32) | (uint64_t)i : ~0ull;
+  __syncthreads();
+  for (int k = 2; k <= np; k <<= 1)
+    for (int j = k >> 1; j > 0; j >>= 1) {
+      for (int i = threadIdx.x; i < np; i += blockDim.x) {
+        const int l = i ^ j;
+        if (l > i) {
+          const uint64_t a = s[i], b = s[l];
+          if ((a > b) == ((i & k) == 0)) { s[i] = b; s[l] = a; }
+        }
+      }
+      __syncthreads();
+    }
+  for (int i = threadIdx.x; i < n; i += blockDim.x) {
+    kout[i] = (uint32_t)(s[i] >> 32);
+    vout[i] = (uint32_t)s[i];
+  }
+}
+
 // ------------------------------------------------------------------------------------------------------
 // LSD radix sort of (u32 key, u32 value) pairs, 8-bit digits, stable: tiles of 256 threads x 8 rounds
 constexpr int RS_THREADS = 256, RS_ROUNDS = 8, RS_TILE = RS_THREADS * RS_ROUNDS;

@@ -401,6 +401,30 @@ __global__ __launch_bounds__(CT_THREADS) void k_scatter_alive(FlagAlive f, long 
   }
 }
 
+// the same for a chunk of one tile (n <= CT_TILE) in one launch: the level's survivor count *out0 is read at the
+// start and advanced at the end
+__global__ __launch_bounds__(CT_THREADS) void k_scatter_alive_one(FlagAlive f, long long n, const uint64_t *keys,
+                                                                  long long child0, unsigned long long *out0,
+                                                                  uint64_t *surv_o, uint64_t *surv_key,
+                                                                  const uint32_t *info, uint32_t *surv_bw) {
+  __shared__ int ws[CT_THREADS / 64];
+  const long long p0 = (long long)*out0;
+  long long pos = p0;
+  for (int i = 0; i < CT_ITEMS; i++) {
+    const long long x = (long long)i * CT_THREADS + threadIdx.x;
+    const bool fl = x < n && f(x);
+    int ex;
+    const int tot = block_flag_scan(fl, ex, ws);
+    if (fl) {
+      surv_o[pos + ex] = (uint64_t)(child0 + x);
+      surv_key[pos + ex] = keys[x];
+      surv_bw[pos + ex] = info[x];
+    }
+    pos += tot;
+  }
+  if (threadIdx.x == 0) *out0 = (unsigned long long)pos;
+}
+
 // searches that continue after this level: no circuit, no overflow, budget left (captured step)
 __global__ void k_active(const unsigned long long *icnt, const uint8_t *keepi, int nslots, uint8_t *active) {
   for (int s = blockIdx.x * blockDim.x + threadIdx.x; s < nslots; s += gridDim.x * blockDim.x) {
@@ -1188,12 +1212,18 @@ template <int NQ> static void batch_step(chz_ctx *c) {
     const long long nb = (ncc + CT_TILE - 1) / CT_TILE;
     const FlagAlive fa{c->status.p, pinst, c->d_keepi.p, off, G};
     c->ev.begin(K_COMPACT, c->stream);
-    hipLaunchKernelGGL(k_flag_count<FlagAlive>, dim3((unsigned)nb), dim3(CT_THREADS), 0, c->stream, fa, ncc, c->bcount.p);
-    hipLaunchKernelGGL(k_scan_one_block, dim3(1), dim3(1024), 0, c->stream, c->bcount.p, nb);
-    hipLaunchKernelGGL(k_scatter_alive, dim3((unsigned)nb), dim3(CT_THREADS), 0, c->stream, fa, ncc, c->bcount.p,
-                       c->keys.p, off * G, c->cnt.p + C_SURV, c->surv_o.p, c->surv_key.p, c->info.p, c->surv_bw.p,
-                       nullptr);
-    hipLaunchKernelGGL(k_add_total, dim3(1), dim3(64), 0, c->stream, c->cnt.p + C_SURV, c->bcount.p + nb);
+    if (nb == 1) {   // one tile: count, place and advance the survivor count in one launch
+      hipLaunchKernelGGL(k_scatter_alive_one, dim3(1), dim3(CT_THREADS), 0, c->stream, fa, ncc, c->keys.p, off * G,
+                         c->cnt.p + C_SURV, c->surv_o.p, c->surv_key.p, c->info.p, c->surv_bw.p);
+    } else {
+      hipLaunchKernelGGL(k_flag_count<FlagAlive>, dim3((unsigned)nb), dim3(CT_THREADS), 0, c->stream, fa, ncc,
+                         c->bcount.p);
+      hipLaunchKernelGGL(k_scan_one_block, dim3(1), dim3(1024), 0, c->stream, c->bcount.p, nb);
+      hipLaunchKernelGGL(k_scatter_alive, dim3((unsigned)nb), dim3(CT_THREADS), 0, c->stream, fa, ncc, c->bcount.p,
+                         c->keys.p, off * G, c->cnt.p + C_SURV, c->surv_o.p, c->surv_key.p, c->info.p, c->surv_bw.p,
+                         nullptr);
+      hipLaunchKernelGGL(k_add_total, dim3(1), dim3(64), 0, c->stream, c->cnt.p + C_SURV, c->bcount.p + nb);
+    }
     c->ev.end(c->stream);
     c->k_work[K_COMPACT] += (double)ncc;
   }
@@ -1220,9 +1250,14 @@ template <int NQ> static void batch_step(chz_ctx *c) {
       for (int bb = 0; bb < 2; bb++) { c->sk[bb].need(nsurv); c->sv[bb].need(nsurv); }
       c->runs.need(nsurv);
       c->ev.begin(K_SORT, c->stream);
-      hipLaunchKernelGGL(k_init_sort, dim3(grid_for(nsurv / 64 + 1)), dim3(256), 0, c->stream, c->surv_key.p, nsurv,
-                         c->sk[0].p, c->sv[0].p, nullptr);
-      sort_pairs(c, nsurv);
+      if (nsurv <= SS_MAX) {
+        hipLaunchKernelGGL(k_sort_small, dim3(1), dim3(1024), 0, c->stream, c->surv_key.p, nsurv, c->sk[0].p,
+                           c->sv[0].p);
+      } else {
+        hipLaunchKernelGGL(k_init_sort, dim3(grid_for(nsurv / 64 + 1)), dim3(256), 0, c->stream, c->surv_key.p,
+                           nsurv, c->sk[0].p, c->sv[0].p, nullptr);
+        sort_pairs(c, nsurv);
+      }
       c->ev.end(c->stream);
       c->k_work[K_SORT] += (double)nsurv * (16.0 + 4 * (4.0 + 16.0));
       c->ev.begin(K_DEDUPE, c->stream);
@@ -1243,10 +1278,16 @@ template <int NQ> static void batch_step(chz_ctx *c) {
     c->bcount.need(nb + 1);
     const FlagKept fk{c->kept.p, c->surv_o.p, pinst, c->d_active.p, G};
     c->ev.begin(K_COMPACT, c->stream);
-    hipLaunchKernelGGL(k_flag_count<FlagKept>, dim3((unsigned)nb), dim3(CT_THREADS), 0, c->stream, fk, nsurv, c->bcount.p);
-    hipLaunchKernelGGL(k_scan_one_block, dim3(1), dim3(1024), 0, c->stream, c->bcount.p, nb);
-    hipLaunchKernelGGL(k_scatter_idx<FlagKept>, dim3((unsigned)nb), dim3(CT_THREADS), 0, c->stream, fk, nsurv,
-                       c->bcount.p, c->kidx.p);
+    if (nb == 1) {
+      hipLaunchKernelGGL(k_compact_one<FlagKept>, dim3(1), dim3(CT_THREADS), 0, c->stream, fk, nsurv, c->kidx.p,
+                         c->bcount.p + nb);
+    } else {
+      hipLaunchKernelGGL(k_flag_count<FlagKept>, dim3((unsigned)nb), dim3(CT_THREADS), 0, c->stream, fk, nsurv,
+                         c->bcount.p);
+      hipLaunchKernelGGL(k_scan_one_block, dim3(1), dim3(1024), 0, c->stream, c->bcount.p, nb);
+      hipLaunchKernelGGL(k_scatter_idx<FlagKept>, dim3((unsigned)nb), dim3(CT_THREADS), 0, c->stream, fk, nsurv,
+                         c->bcount.p, c->kidx.p);
+    }
     c->ev.end(c->stream);
     HIPCHK(hipMemcpyAsync(ptot, c->bcount.p + nb, 8, hipMemcpyDeviceToHost, c->stream));
     HIPCHK(hipMemcpyAsync(hc, c->cnt.p, C_N * 8, hipMemcpyDeviceToHost, c->stream));

@@ -443,10 +443,14 @@ __global__ void k_add_total(unsigned long long *acc, const unsigned long long *t
 // classes, keeping the smallest survivor rank (= lexicographically first prefix) of each class
 // (src/cheese.cu:1334-1343, :1046-1065)
 // every survivor starts kept (a sequential fill: setting kept[v[i]] here would scatter byte writes over the array)
-__global__ void k_fill_kept(uint8_t *kept, long long n, const unsigned long long *nd) {
+// (it runs after the level's expansion has read the goal flags, before the gather sets them for the next level:
+// gflag, when given, is cleared here)
+__global__ void k_fill_kept(uint8_t *kept, long long n, const unsigned long long *nd, uint32_t *gflag, int ns) {
   n = dev_n(n, nd);
-  for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (long long)gridDim.x * blockDim.x)
-    kept[i] = 1;
+  const long long i0 = (long long)blockIdx.x * blockDim.x + threadIdx.x, stride = (long long)gridDim.x * blockDim.x;
+  for (long long i = i0; i < n; i += stride) kept[i] = 1;
+  if (gflag)
+    for (long long i = i0; i < ns; i += stride) gflag[i] = 0u;
 }
 
 // run starts (a position whose key differs from the one before and equals the one after), appended in tiles:
@@ -780,7 +784,7 @@ struct chz_ctx {
   chz::DBuf<uint8_t> status, kept;
   chz::DBuf<uint64_t> keys, surv_o, surv_key;
   chz::DBuf<uint32_t> info, surv_bw;
-  chz::DBuf<unsigned long long> cnt, bcount, hist;
+  chz::DBuf<unsigned long long> bcount, hist;
   chz::DBuf<uint32_t> sk[2], sv[2], runs, kidx;
   // batch state: searches, frontier instance slots, per-slot device arrays, queued events
   std::vector<chz::BInst> binst;
@@ -789,10 +793,15 @@ struct chz_ctx {
   chz::DBuf<uint32_t> finst[2];
   long long m = 0;             // nodes in the current frontier (buffer cur)
   int cur = 0, gstep = -1, nslots = 0;
-  chz::DBuf<int> d_rem;
-  chz::DBuf<uint8_t> d_keepi, d_active;
+  // level control block, uploaded in one copy at the start of every level and read back in one copy:
+  // [cnt: C_N counters | icnt: per-slot IC_N counters | d_rem: per-slot budgets | d_keepi: per-slot keep flags]
+  chz::DBuf<unsigned long long> ctl;
+  unsigned long long *cnt = nullptr, *icnt = nullptr;
+  int *d_rem = nullptr;
+  uint8_t *d_keepi = nullptr;
+  chz::DBuf<uint8_t> d_active;
   chz::DBuf<uint32_t> gflag;        // per slot: a node of the current frontier has a goal child
-  chz::DBuf<unsigned long long> icnt, scan_tmp;
+  chz::DBuf<unsigned long long> scan_tmp;
   chz::Pinned pin;                  // staging of the per-level control transfers
   // captured level steps (one per frontier parity) and the buffer addresses and schedule they were captured with
   struct StepGraph { hipGraphExec_t exec = nullptr; std::vector<uintptr_t> key; };
@@ -881,17 +890,22 @@ static void sort_pairs(chz_ctx *c, long long n) {
   HIPCHK(radix_sort_pairs(c->sk[0].p, c->sv[0].p, c->sk[1].p, c->sv[1].p, n, c->hist.p, c->scan_tmp.p, c->stream));
 }
 
+// bytes of the level control block for ns slots (its layout is the pinned staging area's)
+static size_t ctl_in_bytes(int ns) { return ((size_t)C_N + (size_t)ns * IC_N) * 8 + (size_t)ns * 4 + (size_t)ns; }
+
 // per-slot device arrays sized for `need` instance slots
 static void reserve_slots(chz_ctx *c, int need) {
   if (need <= c->nslots) return;
   int ns = std::max(16, c->nslots);
   while (ns < need) ns *= 2;
-  c->d_rem.need(ns);
-  c->d_keepi.need(ns);
+  c->ctl.need((ctl_in_bytes(ns) + 7) / 8);
+  c->cnt = c->ctl.p;
+  c->icnt = c->cnt + C_N;
+  c->d_rem = (int *)(c->icnt + (size_t)ns * IC_N);
+  c->d_keepi = (uint8_t *)(c->d_rem + ns);
   c->d_active.need(ns);
   c->gflag.need(ns);
   HIPCHK(hipMemsetAsync(c->gflag.p, 0, (size_t)ns * 4, c->stream));   // fresh slots: no goal child known
-  c->icnt.need((size_t)ns * IC_N);
   c->d_lo.need(ns);
   c->d_hi.need(ns);
   c->nslots = ns;
@@ -1001,7 +1015,6 @@ static void step_graph(chz_ctx *c, int cur, long long m, const unsigned long lon
   const long long cap = STEP_CAP, ncap = cap * G;
   const long long nbA = (ncap + CT_TILE - 1) / CT_TILE, nbS = (ncap + RS_TILE - 1) / RS_TILE;
   const long long mx = ncap + (long long)c->pending.size();
-  c->cnt.need(C_N);
   c->status.need(ncap); c->keys.need(ncap); c->info.need(ncap);
   c->bcount.need(nbA + 1);
   c->surv_o.need(ncap); c->surv_key.need(ncap); c->surv_bw.need(ncap);
@@ -1017,7 +1030,7 @@ static void step_graph(chz_ctx *c, int cur, long long m, const unsigned long lon
   const std::vector<uintptr_t> key = {
       (uintptr_t)c->res[cur].p, (uintptr_t)c->pk[cur].p, (uintptr_t)c->g1[cur].p, (uintptr_t)c->g2[cur].p,
       (uintptr_t)c->pb[cur].p, (uintptr_t)pinst, (uintptr_t)c->status.p, (uintptr_t)c->keys.p, (uintptr_t)c->info.p,
-      (uintptr_t)c->cnt.p, (uintptr_t)c->d_rem.p, (uintptr_t)c->d_keepi.p, (uintptr_t)c->icnt.p,
+      (uintptr_t)c->cnt, (uintptr_t)c->d_rem, (uintptr_t)c->d_keepi, (uintptr_t)c->icnt,
       (uintptr_t)c->bcount.p, (uintptr_t)c->surv_o.p, (uintptr_t)c->surv_key.p, (uintptr_t)c->surv_bw.p,
       (uintptr_t)c->kept.p, (uintptr_t)c->kidx.p, (uintptr_t)c->runs.p, (uintptr_t)c->sk[0].p, (uintptr_t)c->sk[1].p,
       (uintptr_t)c->sv[0].p, (uintptr_t)c->sv[1].p, (uintptr_t)c->hist.p, (uintptr_t)c->scan_tmp.p,
@@ -1026,36 +1039,37 @@ static void step_graph(chz_ctx *c, int cur, long long m, const unsigned long lon
   chz_ctx::StepGraph &sg = c->sgraph[cur];
   if (!sg.exec || sg.key != key) {
     if (sg.exec) { HIPCHK(hipGraphExecDestroy(sg.exec)); sg.exec = nullptr; }
-    const unsigned long long *dm = c->cnt.p + C_M, *dncc = c->cnt.p + C_NCC, *dsurv = c->cnt.p + C_SURV;
+    const unsigned long long *dm = c->cnt + C_M, *dncc = c->cnt + C_NCC, *dsurv = c->cnt + C_SURV;
     hipStream_t s = c->stream;
     hipGraph_t graph = nullptr;
     HIPCHK(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
     ExpandArgs ea{c->res[cur].p, c->pk[cur].p, c->g1[cur].p, c->g2[cur].p, 0, ncap, 0, 0,
-                  c->status.p, c->keys.p, c->info.p, c->pb[cur].p, c->cnt.p, pinst, c->d_rem.p, c->d_keepi.p,
-                  c->icnt.p, 1, dm, sc.tw, c->gflag.p};
+                  c->status.p, c->keys.p, c->info.p, c->pb[cur].p, c->cnt, pinst, c->d_rem, c->d_keepi,
+                  c->icnt, 1, dm, sc.tw, c->gflag.p};
     const dim3 grid(grid_for((cap * G + CPW - 1) / CPW, sc.cap));
     switch (sc.var) {
       case 0: hipLaunchKernelGGL((k_level<NQ, 4>), grid, dim3(256), 0, s, ks, ea, cap); break;
       case 2: hipLaunchKernelGGL((k_level<NQ, 6>), grid, dim3(256), 0, s, ks, ea, cap); break;
       default: hipLaunchKernelGGL((k_level<NQ, 5>), grid, dim3(256), 0, s, ks, ea, cap); break;
     }
-    const FlagAlive fa{c->status.p, pinst, c->d_keepi.p, 0, G};
+    const FlagAlive fa{c->status.p, pinst, c->d_keepi, 0, G};
     hipLaunchKernelGGL(k_flag_count<FlagAlive>, dim3((unsigned)nbA), dim3(CT_THREADS), 0, s, fa, ncap, c->bcount.p, dncc);
     hipLaunchKernelGGL(k_scan_one_block, dim3(1), dim3(1024), 0, s, c->bcount.p, nbA);
     hipLaunchKernelGGL(k_scatter_alive, dim3((unsigned)nbA), dim3(CT_THREADS), 0, s, fa, ncap, c->bcount.p, c->keys.p,
-                       0ll, c->cnt.p + C_SURV, c->surv_o.p, c->surv_key.p, c->info.p, c->surv_bw.p, dncc);
-    hipLaunchKernelGGL(k_add_total, dim3(1), dim3(64), 0, s, c->cnt.p + C_SURV, c->bcount.p + nbA);
+                       0ll, c->cnt + C_SURV, c->surv_o.p, c->surv_key.p, c->info.p, c->surv_bw.p, dncc);
+    hipLaunchKernelGGL(k_add_total, dim3(1), dim3(64), 0, s, c->cnt + C_SURV, c->bcount.p + nbA);
     hipLaunchKernelGGL(k_init_sort, dim3(grid_for(ncap / 64 + 1)), dim3(256), 0, s, c->surv_key.p, ncap, c->sk[0].p,
                        c->sv[0].p, dsurv);
     hipError_t e = radix_sort_pairs(c->sk[0].p, c->sv[0].p, c->sk[1].p, c->sv[1].p, ncap, c->hist.p, c->scan_tmp.p, s,
                                     dsurv);
-    hipLaunchKernelGGL(k_fill_kept, dim3(grid_for(ncap / 256 + 1)), dim3(256), 0, s, c->kept.p, ncap, dsurv);
+    hipLaunchKernelGGL(k_fill_kept, dim3(grid_for(ncap / 256 + 1)), dim3(256), 0, s, c->kept.p, ncap, dsurv, nullptr,
+                       0);
     hipLaunchKernelGGL(k_run_heads, dim3(grid_for(ncap / (64 * RH_ITEMS) + 1)), dim3(256), 0, s, c->sk[0].p, ncap,
-                       c->runs.p, c->cnt.p, dsurv);
+                       c->runs.p, c->cnt, dsurv);
     const DedupeArgs da{c->sk[0].p, c->sv[0].p, ncap, c->surv_o.p, c->surv_key.p, c->runs.p, c->res[cur].p,
-                        c->pk[cur].p, c->kept.p, c->cnt.p, pinst, dsurv};
+                        c->pk[cur].p, c->kept.p, c->cnt, pinst, dsurv};
     hipLaunchKernelGGL(k_dedupe_runs<NQ>, dim3(grid_for(ncap / CPW + 1, 4096)), dim3(256), 0, s, ks, da);
-    hipLaunchKernelGGL(k_active, dim3(1), dim3(256), 0, s, c->icnt.p, c->d_keepi.p, ns, c->d_active.p);
+    hipLaunchKernelGGL(k_active, dim3(1), dim3(256), 0, s, c->icnt, c->d_keepi, ns, c->d_active.p);
     const FlagKept fk{c->kept.p, c->surv_o.p, pinst, c->d_active.p, G};
     hipLaunchKernelGGL(k_flag_count<FlagKept>, dim3((unsigned)nbA), dim3(CT_THREADS), 0, s, fk, ncap, c->bcount.p, dsurv);
     hipLaunchKernelGGL(k_scan_one_block, dim3(1), dim3(1024), 0, s, c->bcount.p, nbA);
@@ -1073,7 +1087,7 @@ static void step_graph(chz_ctx *c, int cur, long long m, const unsigned long lon
     }
     sg.key = key;
   }
-  HIPCHK(hipMemcpyAsync(c->cnt.p, hc_in, C_N * 8, hipMemcpyHostToDevice, c->stream));
+  HIPCHK(hipMemcpyAsync(c->cnt, hc_in, C_N * 8, hipMemcpyHostToDevice, c->stream));
   c->ev.begin(K_STEP, c->stream);
   HIPCHK(hipGraphLaunch(sg.exec, c->stream));
   c->ev.end(c->stream);
@@ -1084,8 +1098,8 @@ static void step_graph(chz_ctx *c, int cur, long long m, const unsigned long lon
   c->ev.begin(K_GATHER, c->stream);
   hipLaunchKernelGGL(k_gather<NQ>, dim3(grid_for((ncap + CPW - 1) / CPW)), dim3(256), 0, c->stream, ks, ga);
   c->ev.end(c->stream);
-  HIPCHK(hipMemcpyAsync(hc, c->cnt.p, C_N * 8, hipMemcpyDeviceToHost, c->stream));
-  HIPCHK(hipMemcpyAsync(ic, c->icnt.p, (size_t)ns * IC_N * 8, hipMemcpyDeviceToHost, c->stream));
+  HIPCHK(hipMemcpyAsync(hc, c->cnt, C_N * 8, hipMemcpyDeviceToHost, c->stream));
+  HIPCHK(hipMemcpyAsync(ic, c->icnt, (size_t)ns * IC_N * 8, hipMemcpyDeviceToHost, c->stream));
   HIPCHK(hipMemcpyAsync(ptot, c->bcount.p + nbA, 8, hipMemcpyDeviceToHost, c->stream));
   HIPCHK(hipStreamSynchronize(c->stream));
   c->ev.flush(c->k_ms, c->k_launch);
@@ -1124,17 +1138,19 @@ template <int NQ> static void batch_step(chz_ctx *c) {
   const double tl = now_s();
   const int cur = c->cur;
   const long long m = c->m;
-  // per-slot budgets and counters, staged in pinned memory: [icnt in | icnt out | counters | rem | keepi | active]
+  // the level's control block staged in pinned memory with the device block's layout, uploaded in one copy:
+  // [counters in | per-slot counters in | rem | keepi], then [counters out | per-slot counters out] (read back
+  // in one copy), the kept total and the per-slot active flags
   const int ns = c->nslots;
-  const size_t o_icin = 0, o_icout = o_icin + (size_t)ns * IC_N * 8, o_hc = o_icout + (size_t)ns * IC_N * 8,
-               o_hcin = o_hc + C_N * 8, o_tot = o_hcin + C_N * 8, o_rem = o_tot + 8, o_keep = o_rem + (size_t)ns * 4,
-               o_act = o_keep + ns;
+  const size_t in_bytes = ctl_in_bytes(ns), o_out = (in_bytes + 63) & ~(size_t)63,
+               o_tot = o_out + ((size_t)C_N + (size_t)ns * IC_N) * 8, o_act = o_tot + 8;
   uint8_t *pin = c->pin.need(o_act + ns);
-  unsigned long long *hc_in = (unsigned long long *)(pin + o_hcin);
-  unsigned long long *ic_in = (unsigned long long *)(pin + o_icin), *ic = (unsigned long long *)(pin + o_icout);
-  unsigned long long *hc = (unsigned long long *)(pin + o_hc), *ptot = (unsigned long long *)(pin + o_tot);
-  int *rem = (int *)(pin + o_rem);
-  uint8_t *keepi = pin + o_keep, *active = pin + o_act;
+  unsigned long long *hc_in = (unsigned long long *)pin, *ic_in = hc_in + C_N;
+  unsigned long long *hc = (unsigned long long *)(pin + o_out), *ic = hc + C_N;
+  unsigned long long *ptot = (unsigned long long *)(pin + o_tot);
+  int *rem = (int *)(ic_in + (size_t)ns * IC_N);
+  uint8_t *keepi = (uint8_t *)(rem + ns), *active = pin + o_act;
+  for (int i = 0; i < C_N; i++) hc_in[i] = 0;
   for (int s = 0; s < ns; s++) {
     rem[s] = 0;
     keepi[s] = active[s] = 0;
@@ -1142,13 +1158,12 @@ template <int NQ> static void batch_step(chz_ctx *c) {
   }
   for (const BInst &b : c->binst)
     if (b.live) { rem[b.slot] = b.depth - (b.t + 1); keepi[b.slot] = b.t + 1 < b.depth; }
-  HIPCHK(hipMemcpyAsync(c->d_rem.p, rem, ns * 4, hipMemcpyHostToDevice, c->stream));
-  HIPCHK(hipMemcpyAsync(c->d_keepi.p, keepi, ns, hipMemcpyHostToDevice, c->stream));
-  HIPCHK(hipMemcpyAsync(c->icnt.p, ic_in, (size_t)ns * IC_N * 8, hipMemcpyHostToDevice, c->stream));
+  HIPCHK(hipMemcpyAsync(c->ctl.p, pin, in_bytes, hipMemcpyHostToDevice, c->stream));
   const long long chunk = c->opts.chunk > 0 ? c->opts.chunk : (1ll << 20);
   const uint32_t *pinst = c->finst[cur].p;
   const int nxt = cur ^ 1;
   long long nsurv = 0, distinct = 0;
+  bool gflag_clear = false;          // the goal flags were cleared by this level's kept fill
   uint32_t *tp = nullptr;
   uint8_t *tg = nullptr;
   // small levels run as one captured graph (expansion through kept compaction) plus the gather: one
@@ -1175,8 +1190,6 @@ template <int NQ> static void batch_step(chz_ctx *c) {
   c->info.need(maxcc);
   c->keys.need(maxcc);
   c->bcount.need((maxcc + CT_TILE - 1) / CT_TILE + 1);
-  c->cnt.need(C_N);
-  HIPCHK(hipMemsetAsync(c->cnt.p, 0, C_N * 8, c->stream));
   // survivors of every chunk append at the device-side count C_SURV: the buffers hold every child of the level
   c->surv_o.need((size_t)m * G);
   c->surv_key.need((size_t)m * G);
@@ -1189,8 +1202,8 @@ template <int NQ> static void batch_step(chz_ctx *c) {
     const chz_ctx::Sched sc = c->tuning ? c->force : c->sched[bucket];
     const int gsplit = (int)std::max<long long>(1, std::min<long long>(G, ((long long)sc.tw * CPW + cm - 1) / cm));
     ExpandArgs ea{c->res[cur].p, c->pk[cur].p, c->g1[cur].p, c->g2[cur].p, off, ncc, 0, 0,
-                  c->status.p, c->keys.p, c->info.p, c->pb[cur].p, c->cnt.p, pinst, c->d_rem.p, c->d_keepi.p,
-                  c->icnt.p, gsplit, nullptr, 0, c->gflag.p};
+                  c->status.p, c->keys.p, c->info.p, c->pb[cur].p, c->cnt, pinst, c->d_rem, c->d_keepi,
+                  c->icnt, gsplit, nullptr, 0, c->gflag.p};
     const dim3 grid(grid_for((cm * gsplit + CPW - 1) / CPW, sc.cap));
     hipEvent_t te0 = nullptr, te1 = nullptr;
     if (c->tuning) {
@@ -1210,25 +1223,24 @@ template <int NQ> static void batch_step(chz_ctx *c) {
       c->tune_ev.push_back({bucket, {te0, te1}});
     }
     const long long nb = (ncc + CT_TILE - 1) / CT_TILE;
-    const FlagAlive fa{c->status.p, pinst, c->d_keepi.p, off, G};
+    const FlagAlive fa{c->status.p, pinst, c->d_keepi, off, G};
     c->ev.begin(K_COMPACT, c->stream);
     if (nb == 1) {   // one tile: count, place and advance the survivor count in one launch
       hipLaunchKernelGGL(k_scatter_alive_one, dim3(1), dim3(CT_THREADS), 0, c->stream, fa, ncc, c->keys.p, off * G,
-                         c->cnt.p + C_SURV, c->surv_o.p, c->surv_key.p, c->info.p, c->surv_bw.p);
+                         c->cnt + C_SURV, c->surv_o.p, c->surv_key.p, c->info.p, c->surv_bw.p);
     } else {
       hipLaunchKernelGGL(k_flag_count<FlagAlive>, dim3((unsigned)nb), dim3(CT_THREADS), 0, c->stream, fa, ncc,
                          c->bcount.p);
       hipLaunchKernelGGL(k_scan_one_block, dim3(1), dim3(1024), 0, c->stream, c->bcount.p, nb);
       hipLaunchKernelGGL(k_scatter_alive, dim3((unsigned)nb), dim3(CT_THREADS), 0, c->stream, fa, ncc, c->bcount.p,
-                         c->keys.p, off * G, c->cnt.p + C_SURV, c->surv_o.p, c->surv_key.p, c->info.p, c->surv_bw.p,
+                         c->keys.p, off * G, c->cnt + C_SURV, c->surv_o.p, c->surv_key.p, c->info.p, c->surv_bw.p,
                          nullptr);
-      hipLaunchKernelGGL(k_add_total, dim3(1), dim3(64), 0, c->stream, c->cnt.p + C_SURV, c->bcount.p + nb);
+      hipLaunchKernelGGL(k_add_total, dim3(1), dim3(64), 0, c->stream, c->cnt + C_SURV, c->bcount.p + nb);
     }
     c->ev.end(c->stream);
     c->k_work[K_COMPACT] += (double)ncc;
   }
-  HIPCHK(hipMemcpyAsync(hc, c->cnt.p, C_N * 8, hipMemcpyDeviceToHost, c->stream));
-  HIPCHK(hipMemcpyAsync(ic, c->icnt.p, (size_t)ns * IC_N * 8, hipMemcpyDeviceToHost, c->stream));
+  HIPCHK(hipMemcpyAsync(hc, c->cnt, ((size_t)C_N + (size_t)ns * IC_N) * 8, hipMemcpyDeviceToHost, c->stream));
   HIPCHK(hipStreamSynchronize(c->stream));
   c->ev.flush(c->k_ms, c->k_launch);
   nsurv = (long long)hc[C_SURV];
@@ -1261,13 +1273,13 @@ template <int NQ> static void batch_step(chz_ctx *c) {
       c->ev.end(c->stream);
       c->k_work[K_SORT] += (double)nsurv * (16.0 + 4 * (4.0 + 16.0));
       c->ev.begin(K_DEDUPE, c->stream);
-      HIPCHK(hipMemsetAsync(c->cnt.p + C_RUNS, 0, 8, c->stream));
       hipLaunchKernelGGL(k_fill_kept, dim3(grid_for(nsurv / 256 + 1)), dim3(256), 0, c->stream, c->kept.p, nsurv,
-                         nullptr);
+                         nullptr, c->gflag.p, ns);
+      gflag_clear = true;
       hipLaunchKernelGGL(k_run_heads, dim3(grid_for(nsurv / (64 * RH_ITEMS) + 1)), dim3(256), 0, c->stream, c->sk[0].p, nsurv,
-                         c->runs.p, c->cnt.p, nullptr);
+                         c->runs.p, c->cnt, nullptr);
       const DedupeArgs da{c->sk[0].p, c->sv[0].p, nsurv, c->surv_o.p, c->surv_key.p, c->runs.p,
-                          c->res[cur].p, c->pk[cur].p, c->kept.p, c->cnt.p, pinst, nullptr};
+                          c->res[cur].p, c->pk[cur].p, c->kept.p, c->cnt, pinst, nullptr};
       hipLaunchKernelGGL(k_dedupe_runs<NQ>, dim3(grid_for(nsurv / CPW + 1, 4096)), dim3(256), 0, c->stream, ks, da);
       c->ev.end(c->stream);
       c->k_work[K_DEDUPE] += (double)nsurv * 9.0;
@@ -1290,7 +1302,7 @@ template <int NQ> static void batch_step(chz_ctx *c) {
     }
     c->ev.end(c->stream);
     HIPCHK(hipMemcpyAsync(ptot, c->bcount.p + nb, 8, hipMemcpyDeviceToHost, c->stream));
-    HIPCHK(hipMemcpyAsync(hc, c->cnt.p, C_N * 8, hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(hipMemcpyAsync(hc, c->cnt, C_N * 8, hipMemcpyDeviceToHost, c->stream));
     HIPCHK(hipStreamSynchronize(c->stream));
     if (hc[C_REPOVF]) { set_err("a dedupe run holds more distinct classes than its buffer"); throw DeviceError(); }
     distinct = (long long)*ptot;
@@ -1298,7 +1310,7 @@ template <int NQ> static void batch_step(chz_ctx *c) {
   }   // normal path
   // next frontier: kept children, then the searches admitted since the last step.  The goal flags describe the
   // frontier being expanded until here; the gather sets them for the next one.
-  if (!graphed) HIPCHK(hipMemsetAsync(c->gflag.p, 0, (size_t)ns * 4, c->stream));
+  if (!graphed && !gflag_clear) HIPCHK(hipMemsetAsync(c->gflag.p, 0, (size_t)ns * 4, c->stream));
   const long long mn = distinct + (long long)c->pending.size();
   std::vector<long long> cnts(c->nslots, 0);
   if (mn) {
@@ -1839,13 +1851,13 @@ void chz_close(chz_ctx *c) {
     c->pb[b].release();
     c->sk[b].release(); c->sv[b].release();
   }
-  c->finst[0].release(); c->finst[1].release(); c->d_rem.release(); c->d_keepi.release(); c->d_active.release(); c->gflag.release();
-  c->icnt.release(); c->scan_tmp.release(); c->d_lo.release(); c->d_hi.release();
+  c->finst[0].release(); c->finst[1].release(); c->ctl.release(); c->d_active.release(); c->gflag.release();
+  c->scan_tmp.release(); c->d_lo.release(); c->d_hi.release();
   c->pin.release();
   for (auto &g : c->sgraph) if (g.exec) { (void)hipGraphExecDestroy(g.exec); g.exec = nullptr; }
   c->status.release(); c->kept.release(); c->keys.release(); c->surv_o.release(); c->surv_key.release();
   c->surv_bw.release();
-  c->info.release(); c->cnt.release(); c->bcount.release(); c->hist.release(); c->runs.release(); c->kidx.release();
+  c->info.release(); c->bcount.release(); c->hist.release(); c->runs.release(); c->kidx.release();
   c->spec_blob.release();
   c->ev.destroy();
   if (c->stream) (void)hipStreamDestroy(c->stream);

@@ -12,9 +12,9 @@ that ends with exit 1 is a proof (no circuit of <= D gates) and the loop goes on
 re-checked exactly against the target (chz_verify_spec, exact Z[w] product on the host, as
 python/cheese/verify.py:20-45 does) and written as OpenQASM 2.0.  Any other exit ends the loop with that code.
 
-A QASM target is turned into a .spec from this package's skeleton of the default candidate set (cx h s sdg t
-tdg x z, the set python/cheese/spec.py:356-364 writes for such targets); a target with other gates needs the
-reference's spec writer and ends with exit 3.
+A QASM target is turned into a .spec by cheese_amd/specgen.py, the restated writer (python/cheese/spec.py): the
+default candidate set (cx h s sdg t tdg x z) plus the target's own gates.  Targets on more than three qubits end
+with exit 3 (the search covers one to three).
 """
 from __future__ import annotations
 
@@ -23,7 +23,7 @@ import os
 import sys
 import time
 
-from . import level_line, result_line, search, verify_spec
+from . import level_line, result_line, search, specgen, verify_spec
 from . import workloads as W
 
 
@@ -74,13 +74,9 @@ def target_spec(path: str):
     if path.endswith(".spec"):
         return name, spec_qubits(text), None, text
     n, circ = W.parse_qasm(text)
-    if n not in (2, 3):
-        raise NotImplementedError(f"{n} qubits: the skeletons cover 2 and 3")
-    extra = sorted({g for g, _ in circ} - set(W.CLIFFORD_T))
-    if extra:
-        raise NotImplementedError("gates outside cx h s sdg t tdg x z (" + " ".join(extra) + ") need the reference's "
-                                  "spec writer")
-    return name, n, len(circ), W.spec_from_skeleton(W.skeleton(n), W.unitary(n, circ), max(1, len(circ)))
+    if n not in (1, 2, 3):
+        raise NotImplementedError(f"{n} qubits: the search covers one to three")
+    return name, n, len(circ), specgen.spec_text(n, circ, max(1, len(circ)))
 
 
 def run(path: str, first: int = 1, only: int | None = None, out: str | None = None, results: str = "results",

@@ -1,5 +1,5 @@
 """The outer budget loop (cheese_amd/run.py, the reference's run.sh:1-64): QASM targets become the reference's own
-.spec text (byte-identical to the golden specs its writer produced), found circuits are written as OpenQASM that
+.spec text (byte-identical to the golden specs its writer produced, through cheese_amd/specgen.py), found circuits are written as OpenQASM that
 reads back to the same gates, and the loop's logs and optimum match the reference's golden logs on the GPU."""
 import json
 import os
@@ -21,10 +21,7 @@ def test_spec_of_a_qasm_target_is_the_references(case):
     m = manifest()[case]
     if "qasm" not in m or m.get("flags"):
         pytest.skip("no QASM source, or a spec written with extra writer options")
-    try:
-        _, _, _, base = R.target_spec(_qasm(m["qasm"][:-5]))
-    except NotImplementedError:
-        pytest.skip("gates outside the default candidate set")
+    _, _, _, base = R.target_spec(_qasm(m["qasm"][:-5]))
     assert R.with_depth(base, m["depth"]) == read_spec(case)
 
 
@@ -34,12 +31,12 @@ def test_qasm_writer_reads_back():
     assert n == 3 and circ == R.gates_of(names)
 
 
-def test_unsupported_gates_and_missing_files(tmp_path):
-    q = tmp_path / "y.qasm"
-    q.write_text('OPENQASM 2.0;\ninclude "qelib1.inc";\nqreg q[2];\ny q[0];\ncx q[0],q[1];\n')
+def test_unsupported_targets_and_missing_files(tmp_path):
+    q = tmp_path / "four.qasm"
+    q.write_text('OPENQASM 2.0;\ninclude "qelib1.inc";\nqreg q[4];\nh q[3];\ncx q[0],q[3];\n')
     msgs = []
     assert R.run(str(q), results=str(tmp_path / "r"), echo=msgs.append) == 3
-    assert "need the reference's spec writer" in msgs[-1]
+    assert "one to three" in msgs[-1]
     assert R.run(str(tmp_path / "none.qasm"), results=str(tmp_path / "r"), echo=msgs.append) == 2
 
 

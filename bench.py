@@ -63,13 +63,13 @@ def copy_probe(nbytes):
     hip.hipFree(b)
 
 
-def toffoli_d9(runs=5):
-    """C1: the Toffoli budget-9 impossibility proof on one context, log checked against the reference's golden log
-    every run; median wall time per proof"""
+def toffoli_proof(depth, runs=5):
+    """a Toffoli impossibility proof at one budget on one context (budget 9: C1; budget 14: the optimum-15 proof),
+    log checked against the reference's golden log every run; median wall time per proof"""
     import cheese_amd
-    with open(os.path.join(ROOT, "tests", "golden", "specs", "toffoli-d9.spec")) as f:
+    with open(os.path.join(ROOT, "tests", "golden", "specs", f"toffoli-d{depth}.spec")) as f:
         spec = f.read()
-    with open(os.path.join(ROOT, "tests", "golden", "logs", "toffoli-d9.log")) as f:
+    with open(os.path.join(ROOT, "tests", "golden", "logs", f"toffoli-d{depth}.log")) as f:
         golden = _strip(f.read().strip().splitlines())
     s = cheese_amd.Searcher(spec)
     try:
@@ -80,7 +80,7 @@ def toffoli_d9(runs=5):
             out = s.run()
             times.append(time.perf_counter() - t)
             if out["rc"] != 1 or _strip([cheese_amd.level_line(l) for l in out["levels"]]) != golden:
-                print("bench.py: Toffoli d9 log differs from the reference golden log", file=sys.stderr)
+                print(f"bench.py: Toffoli d{depth} log differs from the reference golden log", file=sys.stderr)
                 sys.exit(1)
             prefixes = sum(l["prefixes"] for l in out["levels"])
     finally:
@@ -222,7 +222,7 @@ def main():
     ap.add_argument("--tcount", action=argparse.BooleanOptionalAction, default=True,
                     help="also build the 3-qubit T-count table and search 256 batched targets")
     ap.add_argument("--d9", action=argparse.BooleanOptionalAction, default=True,
-                    help="also time the Toffoli budget-9 proof")
+                    help="also time the Toffoli budget-9 and budget-14 proofs")
     ap.add_argument("--tune", action=argparse.BooleanOptionalAction, default=True,
                     help="autotune the expansion schedule on the workload before the timed proofs")
     ap.add_argument("--serve", type=int, default=400, metavar="N",
@@ -298,7 +298,9 @@ def main():
         "autotune": {b: [v["waves_per_eu"], v["grid_cap"], v["split_waves"]] for b, v in tuned.items()} if tuned else None,
     }
     if a.d9:
-        line["toffoli_d9"] = toffoli_d9()
+        line["toffoli_d9"] = toffoli_proof(9)
+        if os.path.exists(os.path.join(ROOT, "tests", "golden", "logs", "toffoli-d14.log")):
+            line["toffoli_d14"] = toffoli_proof(14, runs=3)
     if a.random:
         line["random32"] = random_circuits()
     if a.clifford or a.tcount or a.serve:

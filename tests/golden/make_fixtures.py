@@ -1,12 +1,12 @@
 """Writes the golden fixtures of the gate-count search.  Runs only where the reference tree is mounted
-(/root/reference): it imports the reference's stdlib-only spec writer to produce .spec files and runs the
-reference CPU build (oracle/_ref/cheese-cpu, built by `make -C oracle ref`) for their per-level logs.
+(/root/reference): it writes the .spec files with cheese_amd/specgen.py (the restated writer, byte-identical to
+the reference's, as tests/test_specgen.py checks) and runs the reference CPU build (oracle/_ref/cheese-cpu,
+built by `make -C oracle ref`) for their per-level logs.
 
 Outputs (committed data): tests/golden/specs/*.spec, tests/golden/logs/*.log, tests/golden/manifest.json,
 and cheese_amd/skeletons/n{2,3}.spec (the default gate set's spec with an identity target, from which
 targets are spliced in).  Usage: python tests/golden/make_fixtures.py [--only NAME ...]
 """
-import io
 import json
 import os
 import subprocess
@@ -15,14 +15,12 @@ import time
 
 REF = "/root/reference"
 ROOT = os.path.dirname(os.path.dirname(os.path.dirname(os.path.abspath(__file__))))
-sys.path.insert(0, os.path.join(REF, "python"))
 sys.path.insert(0, ROOT)
-from cheese import spec as ref_spec  # noqa: E402  (reference, this container only)
-from cheese_amd import workloads  # noqa: E402
+from cheese_amd import specgen, workloads  # noqa: E402
 
 HERE = os.path.join(ROOT, "tests", "golden")
 BIN = os.path.join(ROOT, "oracle", "_ref", "cheese-cpu")
-QASM = os.path.join(REF, "tests", "qasm")
+QASM = os.path.join(HERE, "qasm")      # the reference's tests/qasm targets, committed as data
 
 # (name, qasm file, depth, extra flags): the reference's own regression and timing cases
 # (tests/check.sh:16-22, tests/bench.sh:7) plus deeper Toffoli budgets and flag variants
@@ -32,6 +30,7 @@ CASES = [
     ("toffoli-d8", "toffoli.qasm", 8, []), ("toffoli-d9", "toffoli.qasm", 9, []),
     ("toffoli-d10", "toffoli.qasm", 10, []), ("toffoli-d11", "toffoli.qasm", 11, []),
     ("toffoli-d12", "toffoli.qasm", 12, []),
+    ("toffoli-d13", "toffoli.qasm", 13, []), ("toffoli-d14", "toffoli.qasm", 14, []),
     ("toffoli-d9-nosym", "toffoli.qasm", 9, ["--no-sym"]),
     ("toffoli-d11-nosym", "toffoli.qasm", 11, ["--no-sym"]),
     ("toffoli-d8-nodedupe", "toffoli.qasm", 8, ["--no-dedupe"]),
@@ -47,18 +46,17 @@ NRANDOM = 32   # seeded random 3-qubit Clifford+T circuits of 10..13 gates, budg
 
 
 def spec_text(n, circ, depth):
-    buf = io.StringIO()
-    ref_spec.write_spec(buf, n, circ, depth)
-    return buf.getvalue()
+    return specgen.spec_text(n, circ, depth)
 
 
 REF_TIMEOUT = 150   # seconds of reference CPU time per case; slower cases keep their spec without a log
+LONG = {"toffoli-d13": 1200, "toffoli-d14": 2400}    # the Toffoli optimum-15 proofs: run to the end
 
 
-def run_ref(path, flags):
+def run_ref(path, flags, timeout=REF_TIMEOUT):
     t = time.time()
     try:
-        p = subprocess.run([BIN, path] + flags, capture_output=True, text=True, timeout=REF_TIMEOUT)
+        p = subprocess.run([BIN, path] + flags, capture_output=True, text=True, timeout=timeout)
     except subprocess.TimeoutExpired:
         return None, None, time.time() - t
     return p.returncode, p.stdout, time.time() - t
@@ -73,7 +71,8 @@ def main():
             f.write(spec_text(n, [], 0))
     jobs = []
     for name, q, d, flags in CASES:
-        n, circ = ref_spec.parse_qasm(os.path.join(QASM, q))
+        with open(os.path.join(QASM, q)) as f:
+            n, circ = workloads.parse_qasm(f.read())
         jobs.append((name, n, circ, d, flags, {"qasm": q}))
     for s in range(NRANDOM):
         circ = workloads.random_circuit(s, 3, 10, 13)
@@ -89,7 +88,7 @@ def main():
         lp = os.path.join(HERE, "logs", name + ".log")
         if name in manifest and os.path.exists(lp) and not force:
             continue
-        rc, out, sec = run_ref(sp, flags)
+        rc, out, sec = run_ref(sp, flags, LONG.get(name, REF_TIMEOUT))
         if rc is not None:
             with open(lp, "w") as f:
                 f.write(out)

@@ -17,8 +17,10 @@ def _lines(out):
     return "\n".join(lines) + "\n"
 
 
-@pytest.mark.parametrize("name", golden_cases(max_seconds=200.0))
+@pytest.mark.parametrize("name", golden_cases())
 def test_hip_matches_reference_log(name):
+    """every golden log, the Toffoli optimum-15 proofs (budgets 13 and 14) and the random circuits the reference
+    needed up to 24 minutes of CPU for included"""
     m = manifest()[name]
     print(f"case {name}", flush=True)
     out = cheese_amd.search(read_spec(name), **flags_to_kwargs(m["flags"]))

@@ -128,6 +128,7 @@ constexpr int SS_MAX = 4096;
 __attribute__((unused)) __global__ __launch_bounds__(1024) void k_sort_small(const uint64_t *key, long long n,
                                                                            uint32_t *kout, uint32_t *vout) {
   __shared__ uint64_t s[SS_MAX];
+  if (n > SS_MAX) return;                  // callers take the radix sort beyond one block's LDS
   int np = 1;
   while (np < n) np <<= 1;
   for (int i = threadIdx.x; i < np; i += blockDim.x)

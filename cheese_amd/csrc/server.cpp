@@ -80,6 +80,17 @@ chz_response base_resp(chz_server *s, const Req &q) {
 // bytes a gate-count search holds per frontier node and per child while a level is expanded
 constexpr double kNodeBytes[2] = {16.0 * 16 + 16, 16.0 * 64 + 16};
 constexpr double kChildBytes = 48.0;
+// admission plans for a next frontier up to kGrowth times the current one (the level's gather holds both), lets
+// at most kAdmitPerStep searches join one step (so growth is seen before more join), and admits nothing while
+// the device has less than kFreeReserve bytes free
+constexpr double kGrowth = 4.0;
+constexpr int kAdmitPerStep = 64;
+constexpr double kFreeReserve = 24e9;
+// Searches already admitted keep growing for several levels, which no admission-time estimate sees; without a
+// caller's max_batch, a batch holds at most this many live searches (2 and 3 qubits)
+constexpr int kMaxLive[2] = {1024, 96};
+
+double per_node_bytes(int qi, int G) { return kNodeBytes[qi] * (1.0 + kGrowth) + G * kChildBytes; }
 
 // ---- gate-count searches: one batch per qubit count, stepped level by level
 void gate_worker(chz_server *s, int qi) {
@@ -95,13 +106,20 @@ void gate_worker(chz_server *s, int qi) {
         s->cv.wait_for(lk, std::chrono::milliseconds(50), [&] { return s->stop || !s->q_gates[qi].empty(); });
       if (s->stop && live.empty() && s->q_gates[qi].empty()) return;
       // admission: queued searches join while the batch's frontier memory estimate stays in budget; a search
-      // joins as one root and is charged for its first two levels of children
+      // joins as one root and is charged for its first two levels of children, and every admitted search counts
+      // against the budget at once
       const double budget = s->o.mem_budget_gb * 1e9;
+      const double per_node = per_node_bytes(qi, G);
+      double held = (double)chz_batch_nodes(ctx) * per_node;
+      const double want = (double)G * G * per_node;
+      size_t free_b = 0, total_b = 0;
+      const bool low_mem = hipMemGetInfo(&free_b, &total_b) == hipSuccess && (double)free_b < kFreeReserve;
+      int admitted = 0;
       while (!s->q_gates[qi].empty()) {
-        const double held = (double)chz_batch_nodes(ctx) * (kNodeBytes[qi] + G * kChildBytes);
-        const double want = (double)G * G * (kNodeBytes[qi] + G * kChildBytes);
-        if (!live.empty() && held + want > budget) break;
-        if (s->o.max_batch > 0 && (int)live.size() >= s->o.max_batch) break;
+        if (!live.empty() && (held + want > budget || admitted >= kAdmitPerStep || low_mem)) break;
+        if ((int)live.size() >= (s->o.max_batch > 0 ? s->o.max_batch : kMaxLive[qi])) break;
+        held += want;
+        admitted++;
         Req q = std::move(s->q_gates[qi].front());
         s->q_gates[qi].pop_front();
         lk.unlock();
@@ -119,7 +137,7 @@ void gate_worker(chz_server *s, int qi) {
       }
     }
     if (chz_batch_live(ctx)) {
-      if (chz_batch_step(ctx)) {   // a device error: every search of this batch ends with it
+      if (chz_batch_step(ctx)) {   // a device error: every live search of this batch ends with it
         {
           std::lock_guard<std::mutex> lk(s->mu);
           s->err = chz_last_error();
@@ -131,6 +149,15 @@ void gate_worker(chz_server *s, int qi) {
           finish(s, r);
         }
         live.clear();
+        // queued searches go on in a fresh context; if none can be opened they end with the error too
+        chz_close(ctx);
+        ctx = s->gctx[qi] = nullptr;
+        chz_opts co{-1, 1, 1, 0, 0, s->o.device};
+        chz_ctx *fresh = nullptr;
+        if (chz_open(s->base[qi].data(), s->base[qi].size(), &co, &fresh) == 0) {
+          ctx = s->gctx[qi] = fresh;
+          continue;
+        }
         std::lock_guard<std::mutex> lk(s->mu);
         while (!s->q_gates[qi].empty()) {
           chz_response r = base_resp(s, s->q_gates[qi].front());
@@ -142,7 +169,7 @@ void gate_worker(chz_server *s, int qi) {
         }
         return;
       }
-      const long long mem = (long long)((double)chz_batch_nodes(ctx) * (kNodeBytes[qi] + G * kChildBytes));
+      const long long mem = (long long)((double)chz_batch_nodes(ctx) * per_node_bytes(qi, G));
       if (mem > s->mem_peak) s->mem_peak = mem;
     }
     int n;
@@ -364,6 +391,11 @@ int chz_server_poll(chz_server *s, chz_response *out, int max) {
 long long chz_server_outstanding(const chz_server *s) { return s->outstanding.load(); }
 
 long long chz_server_peak_bytes(const chz_server *s) { return s->mem_peak; }
+
+const char *chz_server_error(chz_server *s) {
+  std::lock_guard<std::mutex> lk(s->mu);
+  return s->err.c_str();
+}
 
 void chz_server_close(chz_server *s) {
   if (!s) return;

@@ -45,6 +45,7 @@ def _lib():
             "chz_server_poll": (ctypes.c_int, [ctypes.c_void_p, P(Response), ctypes.c_int]),
             "chz_server_outstanding": (ctypes.c_longlong, [ctypes.c_void_p]),
             "chz_server_peak_bytes": (ctypes.c_longlong, [ctypes.c_void_p]),
+            "chz_server_error": (ctypes.c_char_p, [ctypes.c_void_p]),
             "chz_server_clock": (ctypes.c_double, [ctypes.c_void_p]),
             "chz_server_close": (None, [ctypes.c_void_p]),
         }
@@ -107,6 +108,10 @@ class Server:
     @property
     def peak_bytes(self):
         return _lib().chz_server_peak_bytes(self._h)
+
+    def error(self) -> str:
+        """the last device error a worker met ("" if none)"""
+        return _lib().chz_server_error(self._h).decode()
 
     def clock(self):
         return _lib().chz_server_clock(self._h)

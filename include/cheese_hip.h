@@ -213,7 +213,7 @@ typedef struct chz_server_opts {
   int device;
   int plain;             /* 1: one request at a time, in arrival order */
   int tcount_K;          /* T-count table depth (<= 0: 4) */
-  int max_batch;         /* most searches admitted to one batch (<= 0: no limit) */
+  int max_batch;         /* most live searches in one batch (<= 0: 1024 for 2 qubits, 96 for 3) */
   double mem_budget_gb;  /* admission budget of a gate-count batch's frontier (<= 0: 200) */
 } chz_server_opts;
 
@@ -245,6 +245,7 @@ int chz_server_submit(chz_server *s, const chz_request *r);
 int chz_server_poll(chz_server *s, chz_response *out, int max);   /* finished answers, up to max */
 long long chz_server_outstanding(const chz_server *s);            /* submitted and not yet answered */
 long long chz_server_peak_bytes(const chz_server *s);             /* largest frontier-memory estimate seen */
+const char *chz_server_error(chz_server *s);                       /* last device error a worker met ("" none) */
 double chz_server_clock(const chz_server *s);                      /* seconds since the server opened */
 void chz_server_close(chz_server *s);                              /* finishes outstanding work first */
 

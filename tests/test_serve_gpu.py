@@ -25,12 +25,13 @@ def stream():
         cl.close()
 
 
-def _run(reqs, plain, arrivals=None):
+def _run(reqs, plain, arrivals=None, max_batch=0):
     spec2 = W.spec_from_skeleton(W.skeleton(2), W.unitary(2, []), 1)
     spec3 = W.spec_from_skeleton(W.skeleton(3), W.unitary(3, []), 1)
-    s = serve.Server(spec2, spec3, read_tspec("toffoli"), plain=plain)
+    s = serve.Server(spec2, spec3, read_tspec("toffoli"), plain=plain, max_batch=max_batch)
     try:
         answers, _ = serve.replay(s, reqs, arrivals or [0.0] * len(reqs))
+        assert s.error() == ""
     finally:
         s.close()
     return answers
@@ -59,3 +60,14 @@ def test_served_answers_equal_plain_answers(stream):
                 assert cl.tableau(b["out"]) == tabs[i]
     finally:
         cl.close()
+
+
+def test_capped_batches_join_as_slots_free(stream):
+    """a burst offered at once to batches of at most 4 live searches (each T-count batch at most 4 targets): the
+    queued searches join as earlier ones end, over many steps, and every answer is still the plain answer"""
+    reqs, _, _ = stream
+    plain = _run(reqs, plain=True)
+    served = _run(reqs, plain=False, max_batch=4)
+    for i, r in enumerate(reqs):
+        for f in ("rc", "len", "out", "found", "split"):
+            assert plain[i][f] == served[i][f], (i, r["kind"], f)

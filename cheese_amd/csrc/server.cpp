@@ -88,7 +88,7 @@ constexpr int kAdmitPerStep = 64;
 constexpr double kFreeReserve = 24e9;
 // Searches already admitted keep growing for several levels, which no admission-time estimate sees; without a
 // caller's max_batch, a batch holds at most this many live searches (2 and 3 qubits)
-constexpr int kMaxLive[2] = {1024, 96};
+constexpr int kMaxLive[2] = {1024, 64};
 
 double per_node_bytes(int qi, int G) { return kNodeBytes[qi] * (1.0 + kGrowth) + G * kChildBytes; }
 

@@ -213,7 +213,7 @@ typedef struct chz_server_opts {
   int device;
   int plain;             /* 1: one request at a time, in arrival order */
   int tcount_K;          /* T-count table depth (<= 0: 4) */
-  int max_batch;         /* most live searches in one batch (<= 0: 1024 for 2 qubits, 96 for 3) */
+  int max_batch;         /* most live searches in one batch (<= 0: 1024 for 2 qubits, 64 for 3) */
   double mem_budget_gb;  /* admission budget of a gate-count batch's frontier (<= 0: 200) */
 } chz_server_opts;
 

@@ -120,6 +120,48 @@ __global__ __launch_bounds__(CT_THREADS) void k_compact_one(F f, long long n, ui
   if (threadIdx.x == 0) *tot = (unsigned long long)pos;
 }
 
+// run starts (a position whose key differs from the one before and equals the one after), appended in tiles:
+// each thread tests RH_ITEMS positions, a block scan places the tile's starts and one atomic per tile reserves
+// their slots at *count (the runs' order does not matter: each is deduplicated on its own)
+constexpr int RH_ITEMS = 16;
+__attribute__((unused)) __global__ __launch_bounds__(256) void k_run_starts(const uint32_t *k, long long n,
+                                                                            uint32_t *runs, unsigned long long *count,
+                                                                            const unsigned long long *nd) {
+  n = dev_n(n, nd);
+  __shared__ unsigned long long s_wave[4], s_base;
+  const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
+  const long long tile = 256ll * RH_ITEMS;
+  for (long long t0 = (long long)blockIdx.x * tile; t0 < n; t0 += (long long)gridDim.x * tile) {
+    uint32_t mask = 0;
+    for (int it = 0; it < RH_ITEMS; it++) {
+      const long long i = t0 + it * 256ll + tid;
+      if (i < n) {
+        const uint32_t ki = k[i];
+        const bool head = i == 0 || k[i - 1] != ki;
+        if (head && i + 1 < n && k[i + 1] == ki) mask |= 1u << it;
+      }
+    }
+    const uint32_t c = (uint32_t)__popc(mask);
+    uint32_t x = c;                                  // inclusive scan of the starts over the wave
+    for (int d = 1; d < 64; d <<= 1) {
+      const uint32_t y = (uint32_t)__shfl_up((int)x, d);
+      if (lane >= d) x += y;
+    }
+    if (lane == 63) s_wave[w] = x;
+    __syncthreads();
+    if (tid == 0) {
+      unsigned long long tot = 0;
+      for (int j = 0; j < 4; j++) { const unsigned long long v = s_wave[j]; s_wave[j] = tot; tot += v; }
+      s_base = tot ? atomicAdd(count, tot) : 0ull;
+    }
+    __syncthreads();
+    unsigned long long pos = s_base + s_wave[w] + (x - c);
+    for (int it = 0; it < RH_ITEMS; it++)
+      if ((mask >> it) & 1u) runs[pos++] = (uint32_t)(t0 + it * 256ll + tid);
+    __syncthreads();                                 // s_wave and s_base are reused by the next tile
+  }
+}
+
 // ------------------------------------------------------------------------------------------------------
 // small sorts in one launch: n <= SS_MAX (key prefix, rank) pairs as 64-bit composites (prefix << 32 | rank,
 // all distinct) by a bitonic network in LDS.  Ascending composites are the stable order of the prefixes, so the

@@ -453,47 +453,6 @@ __global__ void k_fill_kept(uint8_t *kept, long long n, const unsigned long long
     for (long long i = i0; i < ns; i += stride) gflag[i] = 0u;
 }
 
-// run starts (a position whose key differs from the one before and equals the one after), appended in tiles:
-// each thread tests RH_ITEMS positions, a block scan places the tile's starts and one atomic per tile reserves
-// their slots (the runs' order does not matter: each is deduplicated on its own)
-constexpr int RH_ITEMS = 16;
-__global__ __launch_bounds__(256) void k_run_heads(const uint32_t *k, long long n, uint32_t *runs,
-                                                   unsigned long long *cnt, const unsigned long long *nd) {
-  n = dev_n(n, nd);
-  __shared__ unsigned long long s_wave[4], s_base;
-  const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
-  const long long tile = 256ll * RH_ITEMS;
-  for (long long t0 = (long long)blockIdx.x * tile; t0 < n; t0 += (long long)gridDim.x * tile) {
-    uint32_t mask = 0;
-    for (int it = 0; it < RH_ITEMS; it++) {
-      const long long i = t0 + it * 256ll + tid;
-      if (i < n) {
-        const uint32_t ki = k[i];
-        const bool head = i == 0 || k[i - 1] != ki;
-        if (head && i + 1 < n && k[i + 1] == ki) mask |= 1u << it;
-      }
-    }
-    const uint32_t c = (uint32_t)__popc(mask);
-    uint32_t x = c;                                  // inclusive scan of the starts over the wave
-    for (int d = 1; d < 64; d <<= 1) {
-      const uint32_t y = (uint32_t)__shfl_up((int)x, d);
-      if (lane >= d) x += y;
-    }
-    if (lane == 63) s_wave[w] = x;
-    __syncthreads();
-    if (tid == 0) {
-      unsigned long long tot = 0;
-      for (int j = 0; j < 4; j++) { const unsigned long long v = s_wave[j]; s_wave[j] = tot; tot += v; }
-      s_base = tot ? atomicAdd(&cnt[C_RUNS], tot) : 0ull;
-    }
-    __syncthreads();
-    unsigned long long pos = s_base + s_wave[w] + (x - c);
-    for (int it = 0; it < RH_ITEMS; it++)
-      if ((mask >> it) & 1u) runs[pos++] = (uint32_t)(t0 + it * 256ll + tid);
-    __syncthreads();                                 // s_wave and s_base are reused by the next tile
-  }
-}
-
 struct DedupeArgs {
   const uint32_t *k, *v;          // sorted key prefixes and survivor ranks
   long long n;
@@ -1064,8 +1023,8 @@ static void step_graph(chz_ctx *c, int cur, long long m, const unsigned long lon
                                     dsurv);
     hipLaunchKernelGGL(k_fill_kept, dim3(grid_for(ncap / 256 + 1)), dim3(256), 0, s, c->kept.p, ncap, dsurv, nullptr,
                        0);
-    hipLaunchKernelGGL(k_run_heads, dim3(grid_for(ncap / (64 * RH_ITEMS) + 1)), dim3(256), 0, s, c->sk[0].p, ncap,
-                       c->runs.p, c->cnt, dsurv);
+    hipLaunchKernelGGL(k_run_starts, dim3(grid_for(ncap / (64 * RH_ITEMS) + 1)), dim3(256), 0, s, c->sk[0].p, ncap,
+                       c->runs.p, c->cnt + C_RUNS, dsurv);
     const DedupeArgs da{c->sk[0].p, c->sv[0].p, ncap, c->surv_o.p, c->surv_key.p, c->runs.p, c->res[cur].p,
                         c->pk[cur].p, c->kept.p, c->cnt, pinst, dsurv};
     hipLaunchKernelGGL(k_dedupe_runs<NQ>, dim3(grid_for(ncap / CPW + 1, 4096)), dim3(256), 0, s, ks, da);
@@ -1276,8 +1235,8 @@ template <int NQ> static void batch_step(chz_ctx *c) {
       hipLaunchKernelGGL(k_fill_kept, dim3(grid_for(nsurv / 256 + 1)), dim3(256), 0, c->stream, c->kept.p, nsurv,
                          nullptr, c->gflag.p, ns);
       gflag_clear = true;
-      hipLaunchKernelGGL(k_run_heads, dim3(grid_for(nsurv / (64 * RH_ITEMS) + 1)), dim3(256), 0, c->stream, c->sk[0].p, nsurv,
-                         c->runs.p, c->cnt, nullptr);
+      hipLaunchKernelGGL(k_run_starts, dim3(grid_for(nsurv / (64 * RH_ITEMS) + 1)), dim3(256), 0, c->stream,
+                         c->sk[0].p, nsurv, c->runs.p, c->cnt + C_RUNS, nullptr);
       const DedupeArgs da{c->sk[0].p, c->sv[0].p, nsurv, c->surv_o.p, c->surv_key.p, c->runs.p,
                           c->res[cur].p, c->pk[cur].p, c->kept.p, c->cnt, pinst, nullptr};
       hipLaunchKernelGGL(k_dedupe_runs<NQ>, dim3(grid_for(nsurv / CPW + 1, 4096)), dim3(256), 0, c->stream, ks, da);

@@ -316,14 +316,6 @@ template <int NQ> __global__ __launch_bounds__(64) void k_keys(KeysArgs a) {
   }
 }
 
-// run starts of sorted prefixes that hold at least two children
-__global__ void k_multi_runs(const uint32_t *k32, long long n, uint32_t *runs, unsigned long long *nruns) {
-  for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (long long)gridDim.x * blockDim.x) {
-    const bool head = (i == 0 || k32[i] != k32[i - 1]) && i + 1 < n && k32[i + 1] == k32[i];
-    if (head) runs[atomicAdd(nruns, 1ull)] = (uint32_t)i;
-  }
-}
-
 struct DedupeArgs {
   Rot R;
   Forest F;
@@ -907,7 +899,8 @@ static GrowOut grow(chz_tc *c, Levels &L, int t, const uint32_t *plist, long lon
   c->cnt.need(4);
   HIPCHK(hipMemsetAsync(c->cnt.p, 0, 32, c->stream));
   hipLaunchKernelGGL(k_fill_u8, dim3(gridt(nc)), dim3(256), 0, c->stream, c->keep.p, nc, (uint8_t)1);
-  hipLaunchKernelGGL(k_multi_runs, dim3(gridt(nc)), dim3(256), 0, c->stream, c->sk[0].p, nc, c->runs.p, c->cnt.p);
+  hipLaunchKernelGGL(k_run_starts, dim3(grid_for(nc / (64 * RH_ITEMS) + 1)), dim3(256), 0, c->stream, c->sk[0].p, nc,
+                     c->runs.p, c->cnt.p, nullptr);
   unsigned long long nruns = 0;
   HIPCHK(hipMemcpyAsync(&nruns, c->cnt.p, 8, hipMemcpyDeviceToHost, c->stream));
   HIPCHK(hipStreamSynchronize(c->stream));

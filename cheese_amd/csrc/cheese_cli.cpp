@@ -51,6 +51,7 @@ int main(int argc, char **argv) {
   if (rc == 0) {
     if (r.len == 0) std::printf("{\"found\": 0, \"circuit\": []}\n");
     else {
+      // a found count of 0 prints as 1, as the reference's print_circuit does (src/cheese.cu:1659)
       std::printf("{\"t\": %d, \"found\": %ld, \"circuit\": [", r.len, r.found ? r.found : 1L);
       for (int i = 0; i < r.len; i++) std::printf("%s\"%s\"", i ? ", " : "", chz_gate_name(ctx, r.circuit[i]));
       std::printf("], \"seconds\": %.2f}\n", r.seconds);

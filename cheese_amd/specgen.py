@@ -41,59 +41,62 @@ def candidates(n: int, gates):
     return out
 
 
-def _idle(M, bit):
-    """the matrix acts as the identity on local qubit `bit`"""
+def _acts_trivially(M, bit):
+    """M = M' (x) I on the local qubit `bit`: every entry equals the matching entry of the block where that
+    qubit's row and column bits are 0 when the two bits agree, and is zero when they differ"""
     N = len(M)
     for r in range(N):
         for c in range(N):
-            if (r ^ c) & bit:
-                if M[r][c] != W.ZERO:
-                    return False
-            elif not (r | c) & bit and M[r][c] != M[r | bit][c | bit]:
+            want = W.ZERO if (r ^ c) & bit else M[r & ~bit][c & ~bit]
+            if M[r][c] != want:
                 return False
     return True
 
 
 def product_key(cs, seq):
-    """exact key of the product of candidates seq (first applied first) on the qubits it acts on nontrivially:
-    (qubits, sqrt2 exponent, normalised entries) (spec.py:139-163)"""
-    qubits = sorted({q for g in seq for q in cs[g][2]})
-    k, M = W.unitary(len(qubits), [(cs[g][1], tuple(qubits.index(q) for q in cs[g][2])) for g in seq])
-    for i in range(len(qubits) - 1, -1, -1):       # drop qubits the product leaves alone, highest first
-        bit = 1 << i
-        if _idle(M, bit):
-            keep = [r for r in range(len(M)) if not r & bit]
-            k, M = W.normalise(k, [[M[r][c] for c in keep] for r in keep])
-            del qubits[i]
-    return tuple(qubits), k, tuple(tuple(row) for row in M)
+    """exact identity of the product of candidates seq (first applied first): the qubits it acts on
+    nontrivially, its sqrt2 exponent and its normalised entries on those qubits"""
+    support = sorted({q for g in seq for q in cs[g][2]})
+    local = [(cs[g][1], tuple(support.index(q) for q in cs[g][2])) for g in seq]
+    k, M = W.unitary(len(support), local)
+    idle = [i for i in range(len(support)) if _acts_trivially(M, 1 << i)]
+    if idle:
+        drop = sum(1 << i for i in idle)
+        rows = [r for r in range(len(M)) if not r & drop]      # the block on the remaining qubits
+        k, M = W.normalise(k, [[M[r][c] for c in rows] for r in rows])
+        support = [q for i, q in enumerate(support) if i not in idle]
+    return tuple(support), k, tuple(map(tuple, M))
+
+
+def _cut_windows(windows, keys, reducible):
+    """the windows (given in enumeration order) that W2/W3 cut: a window whose product equals a shorter one
+    (`reducible`), and every window of an equal-product class except its earliest in enumeration order"""
+    earliest = {}
+    for i, key in enumerate(keys):
+        earliest.setdefault(key, i)
+    return [w for i, (w, key) in enumerate(zip(windows, keys)) if key in reducible or earliest[key] != i]
 
 
 def window_rules(cs, use3: bool = True, max_triples: int = 300_000):
-    """W2 pairs and W3 triples (spec.py:165-185): a window is cut when its product equals a shorter product, or
-    an earlier window of the same length (pairs in lexicographic order; triples over the allowed pairs in sorted
-    order, each pair's successors in the iteration order of the allowed-pair set, as the reference enumerates
-    them: which member of an equal product survives depends on it)"""
+    """W2 pairs and W3 triples of candidate indices.  Pairs are enumerated lexicographically.  Triples extend
+    the allowed pairs (f, g), taken in sorted order, by every h such that (g, h) is allowed, the h in the
+    iteration order of the allowed-pair set built as the set difference of all pairs and the cut pairs; which
+    member of an equal-product class is kept depends on that order, so it is the writer's contract
+    (python/cheese/spec.py:165-185).  No triples are cut when there are more than max_triples of them."""
     G = len(cs)
-    shorter = {product_key(cs, ())} | {product_key(cs, (g,)) for g in range(G)}
-    seen, bad2, pair_keys = {}, [], set()
-    for pair in itertools.product(range(G), repeat=2):
-        k = product_key(cs, pair)
-        pair_keys.add(k)
-        if k in shorter or seen.setdefault(k, pair) != pair:
-            bad2.append(pair)
-    allowed = set(itertools.product(range(G), repeat=2)) - set(bad2)
-    order = list(allowed)
-    succ = {g: [h for g2, h in order if g2 == g] for g in range(G)}
-    triples = [(f, g, h) for f, g in sorted(allowed) for h in succ[g]]
-    bad3 = []
-    if use3 and len(triples) <= max_triples:
-        shorter |= pair_keys
-        seen = {}
-        for tri in triples:
-            k = product_key(cs, tri)
-            if k in shorter or seen.setdefault(k, tri) != tri:
-                bad3.append(tri)
-    return bad2, bad3
+    lower = {product_key(cs, ())} | {product_key(cs, (g,)) for g in range(G)}
+    pairs = list(itertools.product(range(G), repeat=2))
+    pair_keys = [product_key(cs, p) for p in pairs]
+    bad2 = _cut_windows(pairs, pair_keys, lower)
+    allowed = set(pairs) - set(bad2)
+    after = [[] for _ in range(G)]
+    for g, h in allowed:                       # successors of each gate, in the set's iteration order
+        after[g].append(h)
+    triples = [(f, g, h) for f, g in sorted(allowed) for h in after[g]]
+    if not use3 or len(triples) > max_triples:
+        return bad2, []
+    lower |= set(pair_keys)
+    return bad2, _cut_windows(triples, [product_key(cs, t) for t in triples], lower)
 
 
 def cut_list(n: int):
@@ -105,9 +108,11 @@ def cut_list(n: int):
 
 
 def cx_table(n: int):
-    """(cuts, radices, table): table[needs in mixed radix 2|A|+1] = fewest two-qubit gates whose crossings meet
-    every cut's need, by enumeration of per-pair counts and a superset minimum (spec.py:199-226); None when the
-    table would exceed 2e6 entries"""
+    """(cuts, radices, table): the fewest two-qubit gates whose placements cross every cut at least as often as
+    that cut's need, for every need vector (need of cut i in [0, 2|A_i|], mixed radix 2|A_i| + 1, cut 0 the
+    lowest digit).  A gate lowers the need of each cut it crosses by one, so table[v] = 1 + the least
+    table[v'] over the placements that cross a cut with a positive need, v' being v after that placement;
+    v' < v in the mixed-radix order, so one ascending pass fills the table.  None (no table) above 2e6 entries."""
     cuts = cut_list(n)
     rad = [2 * len(A) + 1 for A in cuts]
     size = 1
@@ -115,23 +120,20 @@ def cx_table(n: int):
         size *= r
     if size > 2_000_000:
         return cuts, rad, None
-    pairs = list(itertools.combinations(range(n), 2))
-    top = max(rad) - 1
-    best = [1 << 30] * size
-    for counts in itertools.product(range(top + 1), repeat=len(pairs)):
-        idx, mult = 0, 1
-        for A, r in zip(cuts, rad):
-            cross = sum(c for (a, b), c in zip(pairs, counts) if (a in A) != (b in A))
-            idx += min(cross, r - 1) * mult
-            mult *= r
-        best[idx] = min(best[idx], sum(counts))
-    mult = 1
-    for r in rad:
-        for i in range(size - 1, -1, -1):
-            if (i // mult) % r < r - 1:
-                best[i] = min(best[i], best[i + mult])
-        mult *= r
-    return cuts, rad, best
+    stride = [1] * len(cuts)
+    for i in range(1, len(cuts)):
+        stride[i] = stride[i - 1] * rad[i - 1]
+    crossed = [[i for i, A in enumerate(cuts) if (a in A) != (b in A)] for a, b in itertools.combinations(range(n), 2)]
+    table = [0] * size
+    for v in range(1, size):
+        need = [(v // s) % r for s, r in zip(stride, rad)]
+        best = None
+        for cr in crossed:
+            lowered = sum(stride[i] for i in cr if need[i] > 0)
+            if lowered and (best is None or table[v - lowered] < best):
+                best = table[v - lowered]
+        table[v] = 1 + best
+    return cuts, rad, table
 
 
 def _local_entries(g):

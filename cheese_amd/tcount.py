@@ -1,179 +1,55 @@
-"""T-count mode, host side: channel representations, Pauli rotation tables, .tspec text, exact verification of
-decompositions, and seeded T-count workloads.
+"""T-count mode, host side: the .tspec text of a target and the exact check of a decomposition (both computed by
+the library's host code, chz_tspec_write / chz_tc_verify), seeded T-count workloads, and the GPU coset table.
 
 A unitary U is U = R(P_1) ... R(P_t) C with R(P) = ((1 + w) I + (1 - w) P) / 2, C a Clifford, t the T-count,
 global phase ignored (python/cheese/tcount.py:1-12).  The search runs on channel representations
 Uc[a][b] = Tr(P_a U P_b U^dag) / 2^n: real 4^n x 4^n matrices with entries (x + y sqrt2) / sqrt2^e.
-Exact arithmetic throughout; matrices over Z[w] are (k, rows) as in workloads.py.
+Matrices over Z[w] are (k, rows) as in workloads.py.
 """
 from __future__ import annotations
 
 from . import workloads as W
 
-ZERO = W.ZERO
-ONE = W.ONE
 
-
-def _add(x, y):
-    return (x[0] + y[0], x[1] + y[1], x[2] + y[2], x[3] + y[3])
-
-
-def _conj(x):
-    return (x[0], -x[3], -x[2], -x[1])
-
-
-def matmul(A, B):
-    (ka, a), (kb, b) = A, B
-    N = len(a)
-    out = [[ZERO] * N for _ in range(N)]
-    for i in range(N):
-        for l in range(N):
-            if a[i][l] != ZERO:
-                ail = a[i][l]
-                row = b[l]
-                o = out[i]
-                for j in range(N):
-                    if row[j] != ZERO:
-                        o[j] = _add(o[j], W._zmul(ail, row[j]))
-    return W.normalise(ka + kb, out)
-
-
-def dagger(A):
-    k, a = A
-    N = len(a)
-    return k, [[_conj(a[j][i]) for j in range(N)] for i in range(N)]
-
-
-def pauli_matrix(n, p):
-    """P_p, digits per qubit (0 I, 1 X, 2 Y, 3 Z): P|x> = w^ph |x ^ flip>"""
+def _entries(n, U):
+    """(k, rows) -> (k, the N x N entries as a flat int64 array of a b c d coefficients)"""
+    import ctypes
+    k, M = U
     N = 1 << n
-    M = [[ZERO] * N for _ in range(N)]
-    for x in range(N):
-        flip = ph = 0
-        for q in range(n):
-            d, bit = (p >> (2 * q)) & 3, (x >> q) & 1
-            if d == 1:
-                flip |= 1 << q
-            elif d == 2:
-                flip |= 1 << q
-                ph += 2 + 4 * bit
-            elif d == 3:
-                ph += 4 * bit
-        M[x ^ flip][x] = W._wpow(ph)
-    return 0, M
-
-
-def channel(n, U):
-    """(e, rows of (x, y)): Uc[a][b] = (x + y sqrt2) / sqrt2^e, normalised"""
-    P = 4 ** n
-    paulis = [pauli_matrix(n, p) for p in range(P)]
-    Ud = dagger(U)
-    rows = [[None] * P for _ in range(P)]
-    for b in range(P):
-        A = matmul(matmul(U, paulis[b]), Ud)
-        for a in range(P):
-            k, M = matmul(paulis[a], A)
-            tr = ZERO
-            for i in range(len(M)):
-                tr = _add(tr, M[i][i])
-            aa, bb, cc, dd = tr
-            assert cc == 0 and dd == -bb, "trace not real"
-            rows[a][b] = (k, (aa, bb))
-    E = max(k for r in rows for k, _ in r) + 2 * n
-    out = []
-    for r in rows:
-        row = []
-        for k, (x, y) in r:
-            for _ in range(E - 2 * n - k):        # times sqrt2: (x + y s) s = 2y + x s
-                x, y = 2 * y, x
-            row.append((x, y))
-        out.append(row)
-    e = E
-    while e > 0 and all(x % 2 == 0 for r in out for x, _ in r):   # (x + y s) / s = y + (x / 2) s
-        out = [[(y, x // 2) for x, y in r] for r in out]
-        e -= 1
-    return e, out
-
-
-def pauli_tables(n):
-    """for rotation P_p (p >= 1) and Pauli P_c: (-1, 0) if they commute, else (r, s) with i P_p P_c = s P_r"""
-    P = 4 ** n
-    paulis = [pauli_matrix(n, p) for p in range(P)]
-    iw = (0, 0, 1, 0)
-    tab = []
-    for p in range(1, P):
-        row = []
-        for c in range(P):
-            _, M = matmul(paulis[p], paulis[c])
-            _, M2 = matmul(paulis[c], paulis[p])
-            if M == M2:
-                row.append((-1, 0))
-                continue
-            M = [[W._zmul(iw, x) for x in r] for r in M]
-            hit = None
-            for r in range(P):
-                Q = paulis[r][1]
-                for sign in (1, -1):
-                    if all(M[i][j] == tuple(sign * v for v in Q[i][j]) for i in range(len(M)) for j in range(len(M))):
-                        hit = (r, sign)
-                if hit:
-                    break
-            assert hit, "i P Q is not a signed Pauli"
-            row.append(hit)
-        tab.append(row)
-    return tab
-
-
-_TABLES = {}
+    if len(M) != N or any(len(r) != N for r in M):
+        raise ValueError(f"expected a {N} x {N} matrix")
+    flat = [c for r in M for z in r for c in z]
+    return k, (ctypes.c_longlong * len(flat))(*flat)
 
 
 def tspec_text(n, U):
-    """the .tspec text of U (format of python/cheese/tcount.py:157-167)"""
-    e, M = channel(n, U)
-    if n not in _TABLES:
-        _TABLES[n] = pauli_tables(n)
-    tab = _TABLES[n]
-    lines = [f"tcount-spec 1\nn {n}\nchannel {e}\n"]
-    lines += [" ".join(f"{x} {y}" for x, y in r) + "\n" for r in M]
-    lines.append(f"rotations {len(tab)}\n")
-    lines += [" ".join(f"{r} {s}" for r, s in row) + "\n" for row in tab]
-    lines.append("end\n")
-    return "".join(lines)
-
-
-def rot(n, p):
-    """R(P_p) = ((1 + w) I + (1 - w) P_p) / 2"""
-    N = 1 << n
-    _, Pm = pauli_matrix(n, p)
-    a, b = (1, 1, 0, 0), (1, -1, 0, 0)
-    return W.normalise(2, [[_add(a if i == j else ZERO, W._zmul(b, Pm[i][j])) for j in range(N)] for i in range(N)])
-
-
-def _phase_match(A, Q):
-    return any(all(A[i][j] == W._zmul(W._wpow(ph), Q[i][j]) for i in range(len(A)) for j in range(len(A)))
-               for ph in range(8))
-
-
-def is_clifford(n, C):
-    """C P C^dag is a Pauli up to phase for every non-identity Pauli P"""
-    P = 4 ** n
-    paulis = [pauli_matrix(n, p) for p in range(P)]
-    Cd = dagger(C)
-    for p in range(1, P):
-        k, A = matmul(matmul(C, paulis[p]), Cd)
-        if k != 0 or not any(_phase_match(A, paulis[r][1]) for r in range(1, P)):
-            return False
-    return True
+    """the .tspec text of the unitary U = (k, rows) (chz_tspec_write: transfer matrix and rotation tables; the
+    text `python3 -m cheese.tcount spec` writes, python/cheese/tcount.py:157-167)"""
+    import ctypes
+    L = _tc_lib()
+    k, ent = _entries(n, U)
+    size = L.chz_tspec_write(n, k, ent, None, 0)
+    if size < 0:
+        raise ValueError(f"chz_tspec_write: {size}")
+    buf = ctypes.create_string_buffer(size + 1)
+    got = L.chz_tspec_write(n, k, ent, buf, size + 1)
+    if got != size:
+        raise ValueError(f"chz_tspec_write: {got}")
+    return buf.value.decode()
 
 
 def verify(n, U, rotations):
-    """peel the listed rotations off U (U = X_1 ... X_t C, X = R(P_p) or R(P_p)^-1 for [p, inv]) and check
-    that a Clifford remains (python/cheese/tcount.py:174-183 restated)"""
-    Wm = U
-    for p, inv in rotations:
-        X = rot(n, p)
-        Wm = matmul(X if inv else dagger(X), Wm)
-    return is_clifford(n, Wm)
+    """exact check of a T-count answer (chz_tc_verify): peel the listed rotations (p, inv) off U in order,
+    U = X_1 ... X_t C with X = R(P_p) or R(P_p)^-1, and test that a Clifford remains"""
+    import ctypes
+    L = _tc_lib()
+    k, ent = _entries(n, U)
+    flat = [int(v) for pr in rotations for v in pr]
+    rots = (ctypes.c_int * max(1, len(flat)))(*flat)
+    r = L.chz_tc_verify(n, k, ent, rots, len(rotations))
+    if r < 0:
+        raise ValueError(f"chz_tc_verify: {r}")
+    return r == 1
 
 
 def random_target(seed, n=3, kmin=3, kmax=7, clifford_words=None):
@@ -248,6 +124,10 @@ def _tc_lib():
             "chz_tc_dedupe_counts": (ctypes.c_int, [ctypes.c_void_p, P(ctypes.c_double)]),
             "chz_tc_free": (None, [ctypes.c_void_p]),
             "chz_tc_last_error": (ctypes.c_char_p, []),
+            "chz_tspec_write": (ctypes.c_int, [ctypes.c_int, ctypes.c_int, P(ctypes.c_longlong), ctypes.c_char_p,
+                                               ctypes.c_size_t]),
+            "chz_tc_verify": (ctypes.c_int, [ctypes.c_int, ctypes.c_int, P(ctypes.c_longlong), P(ctypes.c_int),
+                                             ctypes.c_int]),
         }
         for name, (res, args) in sigs.items():
             fn = getattr(L, name)

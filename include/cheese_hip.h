@@ -197,6 +197,17 @@ int chz_tc_dedupe_counts(const chz_tc *t, double *out);
 void chz_tc_free(chz_tc *t);
 const char *chz_tc_last_error(void);
 
+/* Host only (no device work).  The .tspec text of the n-qubit unitary entries / sqrt2^k (entries: N x N
+ * row-major, four int64 coefficients a b c d of a + b w + c w^2 + d w^3 each): the normalised transfer matrix
+ * and the rotation tables (the writer the reference runs as `python3 -m cheese.tcount spec`,
+ * python/cheese/tcount.py:79-129, :157-167).  Writes a NUL-terminated text into out (cap bytes) and returns its
+ * length; out NULL: only the length.  -2 not unitary, -3 bad arguments, -4 out too small. */
+int chz_tspec_write(int n, int k, const long long *entries, char *out, size_t cap);
+/* Host only.  Exact check of a T-count answer (`python3 -m cheese.tcount verify`, python/cheese/tcount.py:
+ * 170-183): peels the t rotations (p, inv) pairs off U in order and tests that a Clifford remains.
+ * 1 verified, 0 not, <0 bad arguments or not unitary. */
+int chz_tc_verify(int n, int k, const long long *entries, const int *rotations, int t);
+
 
 /* ---------------- request server (no reference counterpart) ----------------
  * Gate-count, T-count and Clifford re-synthesis requests submitted from any thread and answered by worker

@@ -3,6 +3,10 @@
 the reference's, as tests/test_specgen.py checks) and runs the reference CPU build (oracle/_ref/cheese-cpu,
 built by `make -C oracle ref`) for their per-level logs.
 
+Gate sets with ccx (the rus-13 unitary, the lite bound's fixtures) had no golden spec to pin the restated writer
+on, so their .spec files are written by the reference's own stdlib-only writer (python/cheese/spec.py
+write_spec), imported here only; tests/test_specgen.py then checks the restated writer against them.
+
 Outputs (committed data): tests/golden/specs/*.spec, tests/golden/logs/*.log, tests/golden/manifest.json,
 and cheese_amd/skeletons/n{2,3}.spec (the default gate set's spec with an identity target, from which
 targets are spliced in).  Usage: python tests/golden/make_fixtures.py [--only NAME ...]
@@ -41,16 +45,37 @@ CASES = [
     ("half_adder_3-d9", "half_adder_3.qasm", 9, []), ("grover_3-d10", "grover_3.qasm", 10, []),
     ("ricotta-rus-0003-d10", "ricotta-rus-0003.qasm", 10, []),
     ("ricotta-rus-0003-d11", "ricotta-rus-0003.qasm", 11, []),
+    ("ccz-d8-nosym", "ccz.qasm", 8, ["--no-sym"]),
+    # benchmarks/rus/13/unitary-opt.qasm: two ccx, so the spec has `lite 1` and the search the lite bound
+    ("rus13-unitary-d12", "rus13-unitary-opt.qasm", 12, []),
+    ("rus13-unitary-d13", "rus13-unitary-opt.qasm", 13, []),
+    ("rus13-unitary-d14", "rus13-unitary-opt.qasm", 14, []),
 ]
+# (H T) on each of the 3 qubits, 8 times (48 gates): target sqrt2 exponent 14, so the T bound of the first
+# levels' children runs with n + k > 16 (the wide transform)
+T_LADDER = [(g, (q,)) for _ in range(8) for q in range(3) for g in ("h", "t")]
+LADDER_CASES = [("t-ladder-d42", 42), ("t-ladder-d44", 44)]
 NRANDOM = 32   # seeded random 3-qubit Clifford+T circuits of 10..13 gates, budget = length
 
 
 def spec_text(n, circ, depth):
+    if any(g == "ccx" for g, _ in circ):
+        return reference_writer_text(n, circ, depth)
     return specgen.spec_text(n, circ, depth)
 
 
+def reference_writer_text(n, circ, depth):
+    """the reference's .spec writer (this container only)"""
+    import io
+    sys.path.insert(0, os.path.join(REF, "python"))
+    from cheese import spec as ref_spec
+    buf = io.StringIO()
+    ref_spec.write_spec(buf, n, circ, depth)
+    return buf.getvalue()
+
+
 REF_TIMEOUT = 150   # seconds of reference CPU time per case; slower cases keep their spec without a log
-LONG = {"toffoli-d13": 1200, "toffoli-d14": 2400}    # the Toffoli optimum-15 proofs: run to the end
+LONG = {"toffoli-d13": 1200, "toffoli-d14": 2400, "t-ladder-d44": 300, "rus13-unitary-d14": 600}    # the Toffoli optimum-15 proofs: run to the end
 
 
 def run_ref(path, flags, timeout=REF_TIMEOUT):
@@ -74,6 +99,8 @@ def main():
         with open(os.path.join(QASM, q)) as f:
             n, circ = workloads.parse_qasm(f.read())
         jobs.append((name, n, circ, d, flags, {"qasm": q}))
+    for name, d in LADDER_CASES:
+        jobs.append((name, 3, T_LADDER, d, [], {"circuit": [[g, list(q)] for g, q in T_LADDER]}))
     for s in range(NRANDOM):
         circ = workloads.random_circuit(s, 3, 10, 13)
         jobs.append((f"random3-s{s}", 3, circ, len(circ), [], {"seed": s, "circuit": [[g, list(q)] for g, q in circ]}))

@@ -21,6 +21,35 @@ def test_oracle_matches_reference_logs(oracle_lib, name):
     assert strip_seconds(out) == strip_seconds(read_log(name))
 
 
+@pytest.mark.parametrize("name", ["toffoli-d11-nosym", "rus13-unitary-d13"])
+def test_oracle_matches_reference_logs_large(oracle_lib, name):
+    """the same on larger fixtures (5-40 s each here): the largest --no-sym proof and the lite bound (a ccx gate
+    set); the wide T transform's fixture (t-ladder-d42, target sqrt2 exponent 14) is among the fast ones above"""
+    m = manifest()[name]
+    kw = dict(depth=-1, dedupe="--no-dedupe" not in m["flags"], sym="--no-sym" not in m["flags"])
+    rc, out = oracle_run(oracle_lib, read_spec(name), **kw)
+    assert rc == m["rc"]
+    assert strip_seconds(out) == strip_seconds(read_log(name))
+
+
+def test_published_ccz_d8_frontier():
+    """docs/ALGORITHM.md:191: CCZ 8 largest frontier 4,960 exact residuals vs 501 symmetry classes; the golden
+    logs (with and without --no-sym) agree"""
+    assert max(l["distinct"] for l in strip_seconds(read_log("ccz-d8"))) == 501
+    assert max(l["distinct"] for l in strip_seconds(read_log("ccz-d8-nosym"))) == 4960
+
+
+def test_published_toffoli_d11_nosym_frontier():
+    """docs/ALGORITHM.md:191: Toffoli 11 largest frontier 172,054 exact residuals (--no-sym)"""
+    assert max(l["distinct"] for l in strip_seconds(read_log("toffoli-d11-nosym"))) == 172054
+
+
+def test_lite_fixtures_use_the_lite_bound():
+    """the rus-13 unitary's specs carry `lite 1` (a ccx gate set), so their golden logs pin the lite bound"""
+    for d in (12, 13, 14):
+        assert "\nstandard 0\nlite 1\ntdiv 2\n" in read_spec(f"rus13-unitary-d{d}")
+
+
 def test_published_toffoli_d11_frontier():
     """docs/ALGORITHM.md:191: Toffoli 11 largest frontier 49,503 classes; the golden log agrees"""
     lines = strip_seconds(read_log("toffoli-d11"))
@@ -148,14 +177,14 @@ def test_spec_writer_reproduces_reference_spec(name):
     if "circuit" in m:
         n, circ = 3, [(g, tuple(q)) for g, q in m["circuit"]]
     else:
-        qasm = m["qasm"]
-        if qasm.startswith("ricotta"):
-            pytest.skip("gate set beyond the default skeleton (cz, y)")
         import os
         from conftest import ROOT
-        path = os.path.join(ROOT, "tests", "golden", "qasm", qasm)
+        path = os.path.join(ROOT, "tests", "golden", "qasm", m["qasm"])
         with open(path) as f:
             n, circ = workloads.parse_qasm(f.read())
+        extra = {g for g, _ in circ} - set(workloads.CLIFFORD_T)
+        if extra:
+            pytest.skip(f"gate set beyond the default skeleton ({', '.join(sorted(extra))})")
     text = workloads.spec_from_skeleton(workloads.skeleton(n), workloads.unitary(n, circ), m["depth"])
     assert text == read_spec(name)
 

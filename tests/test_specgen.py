@@ -80,3 +80,28 @@ def test_loop_writes_specs_for_other_gates(tmp_path):
     name, n, ng, base = R.target_spec(str(q))
     assert (name, n, ng) == ("r", 2, 2)
     assert base == S.spec_text(2, [("y", (0,)), ("cz", (0, 1))], 2)
+
+
+@pytest.mark.parametrize("n", [2, 3, 4])
+def test_cut_table_by_brute_force(n):
+    """the cut table against a direct minimum over bounded per-placement gate counts: for every need vector,
+    the fewest gates among all count vectors (each count 0..4) whose crossings reach every need; every need
+    vector for 2 and 3 qubits, 400 seeded ones of the 10,125 for 4"""
+    import itertools
+    import random
+    cuts, rad, table = S.cx_table(n)
+    pairs = list(itertools.combinations(range(n), 2))
+    best = {}
+    for counts in itertools.product(range(5), repeat=len(pairs)):
+        cross = tuple(sum(c for (a, b), c in zip(pairs, counts) if (a in A) != (b in A)) for A in cuts)
+        tot = sum(counts)
+        if best.get(cross, 99) > tot:
+            best[cross] = tot
+    vs = range(len(table)) if n < 4 else random.Random(4).sample(range(len(table)), 400)
+    for v in vs:
+        need, x = [], v
+        for r in rad:
+            need.append(x % r)
+            x //= r
+        want = min(t for cr, t in best.items() if all(c >= q for c, q in zip(cr, need)))
+        assert table[v] == want, (n, need)

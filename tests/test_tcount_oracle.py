@@ -106,3 +106,18 @@ def test_reference_channels_satisfy_the_entry_bound(name):
             A = sum(X[a * P + b] * X[a * P + c] + 2 * Y[a * P + b] * Y[a * P + c] for a in range(P))
             B = sum(X[a * P + b] * Y[a * P + c] + Y[a * P + b] * X[a * P + c] for a in range(P))
             assert B == 0 and A == (2 ** e if b == c else 0)
+
+
+def test_clifford_check_closed_forms():
+    """the decomposition check on known answers: a Clifford word needs no rotation; T on qubit q is the rotation
+    about Z_q (p = 3 * 4^q), peeled either way round (T = R(Z) I = R(Z)^-1 S); T alone is no Clifford"""
+    from cheese_amd import tcount as T
+    cliff = [("h", (0,)), ("cx", (0, 2)), ("s", (1,)), ("h", (2,)), ("cx", (2, 1)), ("sdg", (0,)), ("x", (1,))]
+    assert T.verify(3, W.unitary(3, cliff), [])
+    for q in range(3):
+        U = W.unitary(3, [("t", (q,))])
+        assert not T.verify(3, U, [])
+        assert T.verify(3, U, [(3 << (2 * q), 0)]) and T.verify(3, U, [(3 << (2 * q), 1)])
+        assert not T.verify(3, U, [(1 << (2 * q), 0)])          # the rotation about X_q leaves a non-Clifford
+    U = W.unitary(3, cliff + [("t", (2,))] + cliff)
+    assert not T.verify(3, U, [])

@@ -5,7 +5,7 @@ at gate budget 12, an exhaustive proof that no circuit of <= 12 gates exists (ex
 checked against the reference CPU build's golden log on every step.  value = prefixes/s (sum of the
 per-level "prefixes" / wall time of a step); ms_per_step is the time to proof.
 
-  python bench.py [--gpus 1] [--steps K] [--warmup W] [--cpu-sample SPEC] [--no-cpu]
+  python bench.py [--gpus 1] [--steps K] [--warmup W] [--cpu-runs R] [--no-cpu] [--serve N] [--table K]
 """
 import argparse
 import json
@@ -31,21 +31,54 @@ def _strip(lines):
     return out
 
 
-def cpu_baseline(spec_name):
-    """the reference CPU build (oracle/_ref, compiled from the reference sources) or, without it, the
-    oracle's restatement, timed on this host on a bounded sample: one full proof of `spec_name`"""
-    spec = os.path.join(ROOT, "tests", "golden", "specs", spec_name + ".spec")
+def _cpu_model():
+    try:
+        with open("/proc/cpuinfo") as f:
+            for ln in f:
+                if ln.startswith("model name"):
+                    return ln.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
+def cpu_baseline(runs=3, threads=16):
+    """The CPU baseline timed on this host beside the GPU measurement: oracle/_ref/cheese-cpu, the reference's own
+    src/cheese.cu compiled here as its CPU configuration (Thrust's OpenMP backend; rocThrust's headers stand in for
+    CCCL's), or without it the oracle's C++ restatement ("port").  `threads` OpenMP threads (16: the host's CPU
+    share of one GPU; nproc is reported beside it), pinned (OMP_PROC_BIND=close, OMP_PLACES=cores); the median of
+    `runs` full proofs each of Toffoli budget 9 and budget 12, every log checked against the golden log.  value:
+    prefixes/s of the budget-12 proof."""
     ref = os.path.join(ROOT, "oracle", "_ref", "cheese-cpu")
     port = os.path.join(ROOT, "oracle", "cheese-oracle")
-    threads = min(os.cpu_count() or 1, 16)
-    env = dict(os.environ, OMP_NUM_THREADS=str(threads))
     kind, exe = ("reference", ref) if os.path.exists(ref) else ("port", port)
-    t = time.perf_counter()
-    p = subprocess.run([exe, spec], capture_output=True, text=True, env=env)
-    sec = time.perf_counter() - t
-    pref = sum(json.loads(l).get("prefixes", 0) for l in p.stdout.splitlines() if '"prefixes"' in l)
-    return {"value": round(pref / sec, 1), "unit": "prefixes/s", "cores": threads, "kind": kind,
-            "sample": f"{spec_name}: one full search, {pref} prefixes in {sec:.2f} s (exit {p.returncode})"}
+    env = dict(os.environ, OMP_NUM_THREADS=str(threads), OMP_PROC_BIND="close", OMP_PLACES="cores")
+    out = {}
+    for d in (9, 12):
+        spec = os.path.join(ROOT, "tests", "golden", "specs", f"toffoli-d{d}.spec")
+        with open(os.path.join(ROOT, "tests", "golden", "logs", f"toffoli-d{d}.log")) as f:
+            golden = _strip(f.read().strip().splitlines())
+        times, pref = [], 0
+        for _ in range(runs):
+            t = time.perf_counter()
+            p = subprocess.run([exe, spec], capture_output=True, text=True, env=env)
+            times.append(time.perf_counter() - t)
+            lines = [l for l in p.stdout.splitlines() if l.strip()]
+            if p.returncode != 1 or _strip(lines) != golden:
+                print(f"bench.py: CPU baseline log of Toffoli d{d} differs from the golden log", file=sys.stderr)
+                sys.exit(1)
+            pref = sum(json.loads(l).get("prefixes", 0) for l in lines)
+        med = sorted(times)[len(times) // 2]
+        out[d] = {"ms_per_proof": round(med * 1e3, 1), "runs_ms": [round(x * 1e3, 1) for x in times],
+                  "prefixes": pref}
+    d12 = out[12]
+    return {"value": round(d12["prefixes"] / (d12["ms_per_proof"] / 1e3), 1), "unit": "prefixes/s",
+            "cores": threads, "kind": kind,
+            "build": ("reference src/cheese.cu, CPU configuration (Thrust OpenMP backend, rocThrust headers), "
+                      "compiled here") if kind == "reference" else "oracle/cheese_oracle.cpp restatement",
+            "sample": f"Toffoli proofs at budgets 9 and 12, {runs} runs each, median; value from budget 12",
+            "toffoli_d9": out[9], "toffoli_d12": out[12], "nproc": os.cpu_count(), "cpu_model": _cpu_model(),
+            "pinning": "OMP_PROC_BIND=close OMP_PLACES=cores"}
 
 
 def copy_probe(nbytes):
@@ -180,6 +213,13 @@ def serving(cliff, n_req=400, seed=1):
         srv = serve.Server(spec2, spec3, tspec, plain=(mode == "plain"))
         try:
             ans, wall = serve.replay(srv, reqs, [0.0] * n_req)
+            if srv.error():
+                print(f"bench.py: {mode} server device error: {srv.error()}", file=sys.stderr)
+            if mode == "served":
+                out["served_engines"] = srv.stats()
+                out["served_peak_device_gb"] = round(srv.peak_device_bytes / 2**30, 2)
+                out["served_peak_frontier_estimate_gb"] = round(srv.peak_bytes / 2**30, 2)
+                out["hbm_gb"] = 288
         finally:
             srv.close()
         res[mode] = ans
@@ -187,11 +227,19 @@ def serving(cliff, n_req=400, seed=1):
         out[mode + "_finish_by_kind_s"] = serve.finish_by_kind(reqs, ans)
         if mode == "plain":
             out["plain_service_by_kind"] = serve.service_breakdown(reqs, ans)
+    bad = []
     for i in range(n_req):
         a, b = res["plain"][i], res["served"][i]
         if any(a[f] != b[f] for f in ("rc", "len", "out", "found", "split")):
-            print(f"bench.py: served answer {i} differs from the plain answer", file=sys.stderr)
-            sys.exit(1)
+            bad.append(i)
+    if bad:
+        for i in bad[:8]:
+            r = reqs[i]
+            print(f"bench.py: served answer {i} differs from the plain answer: kind {r['kind']}, n {r.get('n')}, "
+                  f"length {r.get('length')}; plain {[res['plain'][i][f] for f in ('rc', 'len', 'out', 'found')]}, "
+                  f"served {[res['served'][i][f] for f in ('rc', 'len', 'out', 'found')]}", file=sys.stderr)
+        print(f"bench.py: {len(bad)} of {n_req} served answers differ", file=sys.stderr)
+        sys.exit(1)
     rate = 1.5 * out["plain_capacity"]["solves_per_s"]
     arrivals = serve.poisson_arrivals(n_req, rate, seed)
     for mode in ("plain", "served"):
@@ -212,7 +260,8 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=5)
     ap.add_argument("--warmup", type=int, default=1)
-    ap.add_argument("--cpu-sample", default="toffoli-d11")
+    ap.add_argument("--cpu-runs", type=int, default=3, help="CPU baseline: runs per budget (median)")
+    ap.add_argument("--cpu-threads", type=int, default=16, help="CPU baseline: OpenMP threads (the host's share)")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--copy-probe-mb", type=int, default=0, help="also run a D2D copy of this size (profiling)")
     ap.add_argument("--random", action=argparse.BooleanOptionalAction, default=True,
@@ -225,7 +274,7 @@ def main():
                     help="also time the Toffoli budget-9 and budget-14 proofs")
     ap.add_argument("--tune", action=argparse.BooleanOptionalAction, default=True,
                     help="autotune the expansion schedule on the workload before the timed proofs")
-    ap.add_argument("--serve", type=int, default=400, metavar="N",
+    ap.add_argument("--serve", type=int, default=2000, metavar="N",
                     help="also serve a mixed stream of N requests, plain and batched (0: skip)")
     a = ap.parse_args()
     if a.gpus != 1:
@@ -316,7 +365,12 @@ def main():
             line["serving"] = serving(cliff, a.serve)
         cliff.close()
     if not a.no_cpu:
-        line["cpu_baseline"] = cpu_baseline(a.cpu_sample)
+        line["cpu_baseline"] = cpu_baseline(a.cpu_runs, a.cpu_threads)
+        cpu_ms = line["cpu_baseline"]["toffoli_d12"]["ms_per_proof"]
+        line["time_to_proof_vs_cpu"] = {"toffoli_d12": round(cpu_ms / line["ms_per_step"], 1)}
+        if "toffoli_d9" in line:
+            line["time_to_proof_vs_cpu"]["toffoli_d9"] = round(
+                line["cpu_baseline"]["toffoli_d9"]["ms_per_proof"] / line["toffoli_d9"]["ms_per_proof"], 1)
     print(json.dumps(line), flush=True)
 
 

@@ -22,17 +22,22 @@ _lib = None
 class Level(ctypes.Structure):
     _fields_ = [("t", ctypes.c_int), ("parents", ctypes.c_long), ("prefixes", ctypes.c_long),
                 ("cut_w3", ctypes.c_long), ("cut_lb", ctypes.c_long), ("survivors", ctypes.c_long),
-                ("distinct", ctypes.c_long), ("proved_unsat", ctypes.c_int), ("seconds", ctypes.c_double)]
+                ("distinct", ctypes.c_long), ("proved_unsat", ctypes.c_int), ("seconds", ctypes.c_double),
+                ("dir", ctypes.c_int)]
 
 
 class Opts(ctypes.Structure):
     _fields_ = [("depth", ctypes.c_int), ("dedupe", ctypes.c_int), ("sym", ctypes.c_int), ("chunk", ctypes.c_long),
-                ("profile", ctypes.c_int), ("device", ctypes.c_int)]
+                ("profile", ctypes.c_int), ("device", ctypes.c_int), ("table", ctypes.c_int), ("mitm", ctypes.c_int)]
 
 
 class Result(ctypes.Structure):
     _fields_ = [("rc", ctypes.c_int), ("len", ctypes.c_int), ("found", ctypes.c_long),
-                ("circuit", ctypes.c_int * DMAX), ("seconds", ctypes.c_double)]
+                ("circuit", ctypes.c_int * DMAX), ("seconds", ctypes.c_double), ("split_kind", ctypes.c_int),
+                ("split", ctypes.c_int * 2)]
+
+
+DIRS = {1: "table", 2: "fwd", 3: "bwd", 4: "join"}
 
 
 class KStat(ctypes.Structure):
@@ -87,10 +92,16 @@ def lib():
 
 
 def level_line(lv: dict) -> str:
-    """the reference's per-level stdout line (src/cheese.cu:1354-1356)"""
-    return ('{"t": %d, "parents": %d, "prefixes": %d, "cut_w3": %d, "cut_lb": %d, "survivors": %d, '
+    """the reference's per-level stdout line (src/cheese.cu:1354-1356), with its "dir" prefix in the table and
+    meet-in-the-middle modes, or the join line (:1714-1715)"""
+    d = lv.get("dir", 0)
+    if d == 4:
+        return ('{"dir": "join", "forward": %d, "backward": %d, "hash_matches": %d, "seconds": %.2f}'
+                % (lv["parents"], lv["prefixes"], lv["distinct"], lv["seconds"]))
+    pre = '"dir": "%s", ' % DIRS[d] if d else ""
+    return ('{%s"t": %d, "parents": %d, "prefixes": %d, "cut_w3": %d, "cut_lb": %d, "survivors": %d, '
             '"distinct": %d, "proved_unsat": %s, "seconds": %.2f}'
-            % (lv["t"], lv["parents"], lv["prefixes"], lv["cut_w3"], lv["cut_lb"], lv["survivors"],
+            % (pre, lv["t"], lv["parents"], lv["prefixes"], lv["cut_w3"], lv["cut_lb"], lv["survivors"],
                lv["distinct"], "true" if lv["proved_unsat"] else "false", lv["seconds"]))
 
 
@@ -101,11 +112,11 @@ class SearchError(RuntimeError):
 
 
 def search(spec_text: str, depth: int | None = None, dedupe: bool = True, sym: bool = True, chunk: int = 0,
-           device: int = 0, stats: bool = False):
-    """Runs the gate-count search on a .spec text (the reference's `cheese SPEC [flags]`).
-    Returns dict(rc, levels=[...], circuit=[names] or None, found, seconds, gates, kernel_stats)."""
+           device: int = 0, stats: bool = False, table: int = 0, mitm: bool = False):
+    """Runs the gate-count search on a .spec text (the reference's `cheese SPEC [flags]`; table=K is `--table K`,
+    mitm `--mitm`).  Returns dict(rc, levels=[...], circuit=[names] or None, found, seconds, split, kernel_stats)."""
     L = lib()
-    o = Opts(-1 if depth is None else depth, int(dedupe), int(sym), chunk, 0, device)
+    o = Opts(-1 if depth is None else depth, int(dedupe), int(sym), chunk, 0, device, table, int(mitm))
     ctx = ctypes.c_void_p()
     raw = spec_text.encode()
     rc = L.chz_open(raw, len(raw), ctypes.byref(o), ctypes.byref(ctx))
@@ -122,7 +133,8 @@ def search(spec_text: str, depth: int | None = None, dedupe: bool = True, sym: b
     try:
         rc = L.chz_run(ctx, cb, None, ctypes.byref(res))
         out = {"rc": rc, "levels": levels, "circuit": None, "found": res.found, "seconds": res.seconds,
-               "sym": L.chz_sym_size(ctx)}
+               "sym": L.chz_sym_size(ctx), "depth": L.chz_depth(ctx),
+               "split": (("table", "mitm")[res.split_kind - 1], [res.split[0], res.split[1]]) if res.split_kind else None}
         if rc == 0:
             idx = [res.circuit[i] for i in range(res.len)]
             out["circuit_idx"] = idx
@@ -152,7 +164,7 @@ class Searcher:
     def __init__(self, spec_text: str, depth: int | None = None, dedupe: bool = True, sym: bool = True,
                  chunk: int = 0, device: int = 0):
         L = lib()
-        o = Opts(-1 if depth is None else depth, int(dedupe), int(sym), chunk, 0, device)
+        o = Opts(-1 if depth is None else depth, int(dedupe), int(sym), chunk, 0, device, 0, 0)
         self._ctx = ctypes.c_void_p()
         raw = spec_text.encode()
         rc = L.chz_open(raw, len(raw), ctypes.byref(o), ctypes.byref(self._ctx))
@@ -203,14 +215,21 @@ class Searcher:
 
 
 def result_line(out: dict) -> str:
-    """the reference's result line (src/cheese.cu:1658-1662, :1800)"""
+    """the reference's result line (src/cheese.cu:1658-1662, :1800), with its "table" / "mitm" field, or the
+    proof line of those modes (:1840, :1758)"""
+    split = out.get("split")
+    if out["rc"] == 1 and split:
+        kind, (a, b) = split
+        val = b if kind == "table" else "[%d, %d]" % (a, b)
+        return '{"depth": %d, "proved_unsat": true, "%s": %s}' % (out["depth"], kind, val)
     if out["rc"] != 0:
         return ""
     if not out["circuit"]:
         return '{"found": 0, "circuit": []}'
     names = ", ".join(json.dumps(c) for c in out["circuit"])
-    return '{"t": %d, "found": %d, "circuit": [%s], "seconds": %.2f}' % (
-        len(out["circuit"]), out["found"] or 1, names, out["seconds"])
+    extra = ', "%s": [%d, %d]' % (split[0], split[1][0], split[1][1]) if split else ""
+    return '{"t": %d, "found": %d, "circuit": [%s]%s, "seconds": %.2f}' % (
+        len(out["circuit"]), out["found"] or 1, names, extra, out["seconds"])
 
 
 def verify_spec(spec_text: str, circuit_idx) -> bool:
@@ -228,7 +247,7 @@ class Batch:
         L = lib()
         self._ctx = ctypes.c_void_p()
         raw = base_spec.encode()
-        o = Opts(-1, 1, 1, chunk, 0, device)
+        o = Opts(-1, 1, 1, chunk, 0, device, 0, 0)
         rc = L.chz_open(raw, len(raw), ctypes.byref(o), ctypes.byref(self._ctx))
         if rc:
             raise SearchError(rc, L.chz_last_error().decode())

@@ -28,6 +28,8 @@ def _lib():
             "chz_cliff_readback": (ctypes.c_int, [ctypes.c_void_p, P(ctypes.c_uint64), ctypes.c_longlong, ctypes.c_int,
                                                   P(ctypes.c_uint8), P(ctypes.c_int)]),
             "chz_cliff_sample": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_uint64, ctypes.c_longlong, P(ctypes.c_uint64)]),
+            "chz_cliff_opt": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_char_p, ctypes.c_size_t, ctypes.c_int,
+                                             ctypes.c_char_p, ctypes.c_size_t]),
             "chz_cliff_free": (None, [ctypes.c_void_p]),
             "chz_cliff_last_error": (ctypes.c_char_p, []),
         }
@@ -125,6 +127,15 @@ class CliffordTable:
             raise RuntimeError(f"chz_cliff_sample exit {rc}: {_lib().chz_cliff_last_error().decode()}")
         return list(out[:count])
 
+    def optimize(self, job_text: str, beam: int = 16) -> str:
+        """cliffopt on a job text (chz_cliff_opt): the reference's stdout, first line JSON, then the gate lines"""
+        raw = job_text.encode()
+        buf = ctypes.create_string_buffer(1 << 22)
+        rc = _lib().chz_cliff_opt(self.h, raw, len(raw), beam, buf, len(buf))
+        if rc:
+            raise RuntimeError(f"chz_cliff_opt exit {rc}: {_lib().chz_cliff_last_error().decode()}")
+        return buf.value.decode()
+
     def close(self):
         if self.h:
             _lib().chz_cliff_free(self.h)
@@ -135,3 +146,47 @@ class CliffordTable:
             self.close()
         except Exception:
             pass
+
+
+JOB_GATES = ("h", "s", "sdg", "x", "z", "cx")
+
+
+def random_job(seed, n=3, t=4, lmin=3, lmax=12):
+    """a seeded cliffopt job: t T gates (uniform qubit, uniform dagger) between t + 1 random Clifford segments of
+    lmin..lmax gates over h s sdg x z cx (cx on a uniform ordered pair).  Returns (job text, circuit as
+    (gate, qubits) for the exact unitary)"""
+    r = SplitMix64(seed)
+    tq = [(r.below(n), r.below(2)) for _ in range(t)]
+    segs = []
+    for _ in range(t + 1):
+        seg = []
+        for _ in range(lmin + r.below(lmax - lmin + 1)):
+            g = JOB_GATES[r.below(len(JOB_GATES))]
+            if g == "cx":
+                a = r.below(n)
+                b = r.below(n - 1)
+                b += b >= a
+                seg.append((g, (a, b)))
+            else:
+                seg.append((g, (r.below(n),)))
+        segs.append(seg)
+    lines = [f"{n} {t}"] + [f"{q} {d}" for q, d in tq]
+    circ = []
+    for j, seg in enumerate(segs):
+        lines.append(" ".join([str(len(seg))] + [" ".join([g] + [str(x) for x in qs]) for g, qs in seg]))
+        circ += seg
+        if j < t:
+            circ.append(("tdg" if tq[j][1] else "t", (tq[j][0],)))
+    return "\n".join(lines) + "\n", circ
+
+
+def parse_opt_output(text):
+    """the (gate, qubits) circuit and the JSON summary of cliffopt's output"""
+    import json
+    lines = text.strip().splitlines()
+    head = json.loads(lines[0])
+    circ = []
+    for ln in lines[1:]:
+        parts = ln.split()
+        circ.append((parts[0], tuple(int(x) for x in parts[1:])))
+    return head, circ

@@ -9,12 +9,28 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <array>
+#include <chrono>
 #include <cstring>
+#include <mutex>
+#include <sstream>
 #include <string>
 #include <vector>
 
 #include "../../include/cheese_hip.h"
+#include "host.h"
 #include "primitives.h"
+
+// Device allocations of the Clifford table go through the process-wide capture lock: another thread (the request
+// server's gate-count engines) may be capturing a level step, and hipFree synchronises the device.
+static hipError_t cl_malloc(void **p, size_t bytes) {
+  std::lock_guard<std::recursive_mutex> g(chz::capture_mu());
+  return hipMalloc(p, bytes);
+}
+static hipError_t cl_free(void *p) {
+  std::lock_guard<std::recursive_mutex> g(chz::capture_mu());
+  return hipFree(p);
+}
 
 namespace chz_cl {
 
@@ -270,6 +286,8 @@ __global__ __launch_bounds__(256) void k_digest(Table T, long long nslots, unsig
 
 static thread_local std::string g_err;
 
+#include "cliffopt_kernels.h"
+
 }  // namespace chz_cl
 
 struct chz_cliff {
@@ -296,6 +314,7 @@ extern "C" {
 const char *chz_cliff_last_error(void) { return chz_cl::g_err.c_str(); }
 
 int chz_cliff_build(int n, int device, chz_cliff **out) {
+  chz::RelaxedCapture relaxed;
   *out = nullptr;
   if (n < 1 || n > 3) { chz_cl::g_err = "n must be 1..3"; return 3; }
   chz_cliff *c = new chz_cliff();
@@ -309,14 +328,14 @@ int chz_cliff_build(int n, int device, chz_cliff **out) {
   c->T.bmask = (uint64_t)(c->nslots / 8 - 1);
   CLCHK(hipSetDevice(device));
   CLCHK(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking));
-  CLCHK(hipMalloc((void **)&c->T.slot, c->nslots * 8));
+  CLCHK(cl_malloc((void **)&c->T.slot, c->nslots * 8));
   CLCHK(hipMemsetAsync(c->T.slot, 0, c->nslots * 8, c->stream));
   const long long cap = std::max(1024ll, order / 2);
   uint64_t *fa = nullptr, *fb = nullptr;
   unsigned long long *cnt = nullptr;
-  CLCHK(hipMalloc((void **)&fa, cap * 8));
-  CLCHK(hipMalloc((void **)&fb, cap * 8));
-  CLCHK(hipMalloc((void **)&cnt, 16));
+  CLCHK(cl_malloc((void **)&fa, cap * 8));
+  CLCHK(cl_malloc((void **)&fb, cap * 8));
+  CLCHK(cl_malloc((void **)&cnt, 16));
   hipEvent_t e0, e1;
   CLCHK(hipEventCreate(&e0));
   CLCHK(hipEventCreate(&e1));
@@ -354,9 +373,9 @@ int chz_cliff_build(int n, int device, chz_cliff **out) {
   c->build_ms = ms;
   (void)hipEventDestroy(e0);
   (void)hipEventDestroy(e1);
-  (void)hipFree(fa);
-  (void)hipFree(fb);
-  (void)hipFree(cnt);
+  (void)cl_free(fa);
+  (void)cl_free(fb);
+  (void)cl_free(cnt);
   *out = c;
   if (total != order) { chz_cl::g_err = "group order mismatch"; return 4; }   // src/clifford.cu:279-280
   return 0;
@@ -375,12 +394,12 @@ double chz_cliff_build_ms(const chz_cliff *c) { return c->build_ms; }
 int chz_cliff_digest(const chz_cliff *c, uint64_t *digest, long long *entries) {
   unsigned long long *d = nullptr, h[2] = {0, 0};
   CLCHK(hipSetDevice(c->device));
-  CLCHK(hipMalloc((void **)&d, 16));
+  CLCHK(cl_malloc((void **)&d, 16));
   CLCHK(hipMemsetAsync(d, 0, 16, c->stream));
   hipLaunchKernelGGL(chz_cl::k_digest, dim3(2048), dim3(256), 0, c->stream, c->T, c->nslots, d);
   CLCHK(hipMemcpyAsync(h, d, 16, hipMemcpyDeviceToHost, c->stream));
   CLCHK(hipStreamSynchronize(c->stream));
-  (void)hipFree(d);
+  (void)cl_free(d);
   *digest = h[0];
   *entries = (long long)h[1];
   return 0;
@@ -393,8 +412,8 @@ static int cliff_sorted(chz_cliff *c) {
   const long long nb = (ns + CT_TILE - 1) / CT_TILE;
   unsigned long long *bc = nullptr, *tmp = nullptr, *hist = nullptr;
   uint32_t *idx = nullptr, *k0 = nullptr, *v0 = nullptr, *k1 = nullptr, *v1 = nullptr;
-  CLCHK(hipMalloc((void **)&bc, (nb + 1) * 8));
-  CLCHK(hipMalloc((void **)&tmp, (ns / SC_TILE + 8) * 8));
+  CLCHK(cl_malloc((void **)&bc, (nb + 1) * 8));
+  CLCHK(cl_malloc((void **)&tmp, (ns / SC_TILE + 8) * 8));
   chz_cl::FlagSlot f{c->T.slot};
   hipLaunchKernelGGL(k_flag_count<chz_cl::FlagSlot>, dim3((unsigned)nb), dim3(CT_THREADS), 0, c->stream, f, ns, bc);
   CLCHK(scan_excl(bc, nb, tmp, c->stream));
@@ -402,13 +421,13 @@ static int cliff_sorted(chz_cliff *c) {
   CLCHK(hipMemcpyAsync(&n, bc + nb, 8, hipMemcpyDeviceToHost, c->stream));
   CLCHK(hipStreamSynchronize(c->stream));
   const long long rb = ((long long)n + RS_TILE - 1) / RS_TILE;
-  CLCHK(hipMalloc((void **)&idx, std::max<unsigned long long>(n, 1) * 4));
-  CLCHK(hipMalloc((void **)&k0, std::max<unsigned long long>(n, 1) * 4));
-  CLCHK(hipMalloc((void **)&v0, std::max<unsigned long long>(n, 1) * 4));
-  CLCHK(hipMalloc((void **)&k1, std::max<unsigned long long>(n, 1) * 4));
-  CLCHK(hipMalloc((void **)&v1, std::max<unsigned long long>(n, 1) * 4));
-  CLCHK(hipMalloc((void **)&hist, (rb * 256 + 1) * 8));
-  CLCHK(hipMalloc((void **)&c->sorted, std::max<unsigned long long>(n, 1) * 8));
+  CLCHK(cl_malloc((void **)&idx, std::max<unsigned long long>(n, 1) * 4));
+  CLCHK(cl_malloc((void **)&k0, std::max<unsigned long long>(n, 1) * 4));
+  CLCHK(cl_malloc((void **)&v0, std::max<unsigned long long>(n, 1) * 4));
+  CLCHK(cl_malloc((void **)&k1, std::max<unsigned long long>(n, 1) * 4));
+  CLCHK(cl_malloc((void **)&v1, std::max<unsigned long long>(n, 1) * 4));
+  CLCHK(cl_malloc((void **)&hist, (rb * 256 + 1) * 8));
+  CLCHK(cl_malloc((void **)&c->sorted, std::max<unsigned long long>(n, 1) * 8));
   hipLaunchKernelGGL(k_scatter_idx<chz_cl::FlagSlot>, dim3((unsigned)nb), dim3(CT_THREADS), 0, c->stream, f, ns, bc, idx);
   const unsigned g = (unsigned)std::min<long long>(((long long)n + 255) / 256 + 1, 16384);
   hipLaunchKernelGGL(chz_cl::k_key_lo, dim3(g), dim3(256), 0, c->stream, c->T.slot, idx, (long long)n, k0, v0);
@@ -419,7 +438,7 @@ static int cliff_sorted(chz_cliff *c) {
   CLCHK(hipStreamSynchronize(c->stream));
   c->order = (long long)n;
   void *fr[] = {bc, tmp, hist, idx, k0, v0, k1, v1};
-  for (void *p : fr) (void)hipFree(p);
+  for (void *p : fr) (void)cl_free(p);
   return 0;
 }
 
@@ -429,12 +448,12 @@ int chz_cliff_sample(chz_cliff *c, uint64_t seed, long long count, uint64_t *tab
   if (const int rc = cliff_sorted(c)) return rc;
   if (c->order <= 0) { chz_cl::g_err = "empty table"; return 4; }
   uint64_t *d = nullptr;
-  CLCHK(hipMalloc((void **)&d, std::max<long long>(count, 1) * 8));
+  CLCHK(cl_malloc((void **)&d, std::max<long long>(count, 1) * 8));
   hipLaunchKernelGGL(chz_cl::k_sample, dim3((unsigned)std::min<long long>((count + 255) / 256 + 1, 4096)), dim3(256), 0,
                      c->stream, c->sorted, c->order, (unsigned long long)seed, count, d);
   CLCHK(hipMemcpyAsync(tab, d, count * 8, hipMemcpyDeviceToHost, c->stream));
   CLCHK(hipStreamSynchronize(c->stream));
-  (void)hipFree(d);
+  (void)cl_free(d);
   return 0;
 }
 
@@ -442,16 +461,17 @@ int chz_cliff_sample(chz_cliff *c, uint64_t seed, long long count, uint64_t *tab
 // ms (may be NULL) receives the device time of the word and lookup kernels.
 int chz_cliff_query_words(chz_cliff *c, const uint8_t *letters, const long long *offs, long long nq, int *dist,
                           double *ms) {
+  chz::RelaxedCapture relaxed;
   CLCHK(hipSetDevice(c->device));
   const long long nl = offs[nq];
   uint8_t *dl = nullptr;
   long long *doff = nullptr;
   uint64_t *tab = nullptr;
   int *dd = nullptr;
-  CLCHK(hipMalloc((void **)&dl, std::max<long long>(nl, 1)));
-  CLCHK(hipMalloc((void **)&doff, (nq + 1) * 8));
-  CLCHK(hipMalloc((void **)&tab, std::max<long long>(nq, 1) * 8));
-  CLCHK(hipMalloc((void **)&dd, std::max<long long>(nq, 1) * 4));
+  CLCHK(cl_malloc((void **)&dl, std::max<long long>(nl, 1)));
+  CLCHK(cl_malloc((void **)&doff, (nq + 1) * 8));
+  CLCHK(cl_malloc((void **)&tab, std::max<long long>(nq, 1) * 8));
+  CLCHK(cl_malloc((void **)&dd, std::max<long long>(nq, 1) * 4));
   CLCHK(hipMemcpyAsync(dl, letters, nl, hipMemcpyHostToDevice, c->stream));
   CLCHK(hipMemcpyAsync(doff, offs, (nq + 1) * 8, hipMemcpyHostToDevice, c->stream));
   hipEvent_t e0, e1;
@@ -470,10 +490,10 @@ int chz_cliff_query_words(chz_cliff *c, const uint8_t *letters, const long long 
   if (ms) *ms = t;
   (void)hipEventDestroy(e0);
   (void)hipEventDestroy(e1);
-  (void)hipFree(dl);
-  (void)hipFree(doff);
-  (void)hipFree(tab);
-  (void)hipFree(dd);
+  (void)cl_free(dl);
+  (void)cl_free(doff);
+  (void)cl_free(tab);
+  (void)cl_free(dd);
   return 0;
 }
 
@@ -489,31 +509,34 @@ uint64_t chz_cliff_word_tableau(const chz_cliff *c, const uint8_t *letters, int 
 
 // optimal words of nq tableaux (gate-set indices, <= maxlen each); lens[i] = length or -1
 int chz_cliff_readback(chz_cliff *c, const uint64_t *tab, long long nq, int maxlen, uint8_t *words, int *lens) {
+  chz::RelaxedCapture relaxed;
   CLCHK(hipSetDevice(c->device));
   uint64_t *dt = nullptr;
   uint8_t *dw = nullptr;
   int *dlen = nullptr;
-  CLCHK(hipMalloc((void **)&dt, std::max<long long>(nq, 1) * 8));
-  CLCHK(hipMalloc((void **)&dw, std::max<long long>(nq * maxlen, 1)));
-  CLCHK(hipMalloc((void **)&dlen, std::max<long long>(nq, 1) * 4));
+  CLCHK(cl_malloc((void **)&dt, std::max<long long>(nq, 1) * 8));
+  CLCHK(cl_malloc((void **)&dw, std::max<long long>(nq * maxlen, 1)));
+  CLCHK(cl_malloc((void **)&dlen, std::max<long long>(nq, 1) * 4));
   CLCHK(hipMemcpyAsync(dt, tab, nq * 8, hipMemcpyHostToDevice, c->stream));
   hipLaunchKernelGGL(chz_cl::k_readback, dim3((unsigned)std::min<long long>(std::max<long long>(nq, 1), 8192)),
                      dim3(64), 0, c->stream, c->G, c->T, dt, nq, maxlen, dw, dlen);
   CLCHK(hipMemcpyAsync(words, dw, nq * maxlen, hipMemcpyDeviceToHost, c->stream));
   CLCHK(hipMemcpyAsync(lens, dlen, nq * 4, hipMemcpyDeviceToHost, c->stream));
   CLCHK(hipStreamSynchronize(c->stream));
-  (void)hipFree(dt);
-  (void)hipFree(dw);
-  (void)hipFree(dlen);
+  (void)cl_free(dt);
+  (void)cl_free(dw);
+  (void)cl_free(dlen);
   return 0;
 }
+
+#include "cliffopt.h"
 
 void chz_cliff_free(chz_cliff *c) {
   if (!c) return;
   (void)hipSetDevice(c->device);
   if (c->stream) (void)hipStreamSynchronize(c->stream);
-  if (c->T.slot) (void)hipFree(c->T.slot);
-  if (c->sorted) (void)hipFree(c->sorted);
+  if (c->T.slot) (void)cl_free(c->T.slot);
+  if (c->sorted) (void)cl_free(c->sorted);
   if (c->stream) (void)hipStreamDestroy(c->stream);
   delete c;
 }

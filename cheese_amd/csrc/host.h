@@ -27,6 +27,50 @@ struct DeviceError {};
     }                                                                                  \
   } while (0)
 
+// Process-wide lock between stream capture and the HIP calls that act on the whole device or allocate.  A thread
+// capturing a level step holds it from hipStreamBeginCapture to the graph's instantiation; hipMalloc, hipFree,
+// hipHostMalloc/hipHostFree, device-wide synchronisations and hipMemGetInfo take it too.  A device-wide
+// synchronisation includes the stream under capture, and hipFree synchronises the device: issued by another
+// thread (another engine of the request server) while a stream is being captured, they must wait for the capture
+// to end.
+inline std::recursive_mutex &capture_mu() { static std::recursive_mutex *m = new std::recursive_mutex(); return *m; }
+
+// The stream the calling thread's grow-only device buffers belong to, set by each context's entry points for the
+// duration of the call: a buffer's growth then waits for that stream only (its old block can be in use only by
+// kernels queued there), not for other contexts' kernels on the same device.
+inline hipStream_t &owner_stream() { static thread_local hipStream_t s = nullptr; return s; }
+
+// The calling thread's stream-capture interaction mode, relaxed for the duration of a library call.  In HIP's
+// default (global) mode a thread's synchronous or allocating call while any stream of the process is being
+// captured invalidates that capture, even one in thread-local mode on another thread; the library's own calls never
+// touch a stream under capture (device-wide synchronisations and allocations also wait on capture_mu), so its
+// threads use the relaxed mode.
+struct RelaxedCapture {
+  hipStreamCaptureMode prev = hipStreamCaptureModeRelaxed;
+  RelaxedCapture() { (void)hipThreadExchangeStreamCaptureMode(&prev); }
+  ~RelaxedCapture() { (void)hipThreadExchangeStreamCaptureMode(&prev); }
+};
+
+struct StreamScope {
+  hipStream_t prev;
+  RelaxedCapture relaxed;
+  explicit StreamScope(hipStream_t s) : prev(owner_stream()) { owner_stream() = s; }
+  ~StreamScope() { owner_stream() = prev; }
+};
+// a synchronous copy on the context's own stream (a copy on the legacy default stream would depend on every
+// blocking stream, which is not allowed while another thread captures one)
+inline void copy_sync(void *dst, const void *src, size_t n, hipMemcpyKind kind, hipStream_t s) {
+  HIPCHK(hipMemcpyAsync(dst, src, n, kind, s));
+  HIPCHK(hipStreamSynchronize(s));
+}
+
+// no kernel of the calling context still uses its blocks once this returns
+inline hipError_t sync_owner() {
+  if (owner_stream()) return hipStreamSynchronize(owner_stream());
+  std::lock_guard<std::recursive_mutex> g(capture_mu());
+  return hipDeviceSynchronize();
+}
+
 // Process-wide cache of device blocks by (device, size class): contexts opened and closed one after another
 // (one search per context, as the command line does) reuse memory instead of paying hipMalloc/hipFree each time.
 // Size classes are 2^k rounded up to a quarter step (at most 25 % slack).  A block enters the cache only when no
@@ -58,6 +102,7 @@ struct DevPool {
       }
     }
     void *p = nullptr;
+    std::lock_guard<std::recursive_mutex> cg(capture_mu());
     if (hipMalloc(&p, got) != hipSuccess) {
       (void)hipGetLastError();
       trim();                       // give the cache back to the device and try once more
@@ -68,6 +113,7 @@ struct DevPool {
   void put(void *p, size_t got) {
     if (!p) return;
     int dev = 0;
+    std::lock_guard<std::recursive_mutex> cg(capture_mu());
     if (hipGetDevice(&dev) != hipSuccess) { (void)hipFree(p); return; }
     std::lock_guard<std::mutex> g(mu);
     if (cached + got > kMaxCached) { (void)hipFree(p); return; }
@@ -75,6 +121,7 @@ struct DevPool {
     cached += got;
   }
   void trim() {
+    std::lock_guard<std::recursive_mutex> cg(capture_mu());
     std::lock_guard<std::mutex> g(mu);
     for (auto &kv : free_blocks) (void)hipFree(kv.second);
     free_blocks.clear();
@@ -103,7 +150,11 @@ inline void dev_free(void *p) {
   {
     std::lock_guard<std::mutex> g(live_blocks().mu);
     auto it = live_blocks().sizes.find(p);
-    if (it == live_blocks().sizes.end()) { (void)hipFree(p); return; }
+    if (it == live_blocks().sizes.end()) {
+      std::lock_guard<std::recursive_mutex> cg(capture_mu());
+      (void)hipFree(p);
+      return;
+    }
     got = it->second;
     live_blocks().sizes.erase(it);
   }
@@ -120,7 +171,7 @@ template <typename T> struct DBuf {     // grow-only device array on the block c
     T *np = (T *)dev_pool().get(c * sizeof(T), got);
     if (p) {
       // kernels already queued may still read the old block
-      HIPCHK(hipDeviceSynchronize());
+      HIPCHK(sync_owner());
       dev_pool().put(p, bytes);
     }
     p = np;
@@ -139,6 +190,7 @@ struct Pinned {
   size_t cap = 0;
   uint8_t *need(size_t n) {
     if (n <= cap) return p;
+    std::lock_guard<std::recursive_mutex> cg(capture_mu());
     if (p) HIPCHK(hipHostFree(p));
     p = nullptr;
     const size_t c = std::max<size_t>(n, std::max<size_t>(4096, cap * 2));
@@ -146,7 +198,12 @@ struct Pinned {
     cap = c;
     return p;
   }
-  void release() { if (p) (void)hipHostFree(p); p = nullptr; cap = 0; }
+  void release() {
+    std::lock_guard<std::recursive_mutex> cg(capture_mu());
+    if (p) (void)hipHostFree(p);
+    p = nullptr;
+    cap = 0;
+  }
 };
 
 inline double now_s() { return std::chrono::duration<double>(std::chrono::steady_clock::now().time_since_epoch()).count(); }

@@ -19,6 +19,8 @@
 #include "search_kernels.h"
 #include "spec.h"
 
+static int verify_hs(const chz::HostSpec &hs, const int *circuit, int len);
+
 namespace chz {
 
 // ------------------------------------------------------------------------------------------------------
@@ -700,6 +702,8 @@ __global__ void k_init_sort(const uint64_t *key, long long n, uint32_t *k32, uin
 enum { K_EXPAND = 0, K_COMPACT, K_SORT, K_DEDUPE, K_GATHER, K_STEP, K_NFAM };
 static const char *k_names[K_NFAM] = {"expand", "compact", "sort", "dedupe", "gather", "level_graph"};
 
+#include "table_join_kernels.h"
+
 }  // namespace chz
 
 namespace chz {
@@ -781,6 +785,15 @@ struct chz_ctx {
   double k_ms[chz::K_NFAM] = {}, k_work[chz::K_NFAM] = {};
   double k_stage[5] = {};       // expansion stage counts: residuals, structural bounds, cut ranks, T bounds, keys
   double avg_terms = 1;         // nonzero terms per inverse column, averaged over the candidates
+  bool goal_skip = false;       // every candidate has a goal pre-image: the found-level skip is exact
+  // --table K: the distance table (built on first use, kept with the context); a context that builds one has
+  // gt_filter set (its levels drop classes the table holds and are appended to it) and no goal test
+  chz::GTable *gt = nullptr;
+  chz::GTable *gt_filter = nullptr;
+  bool nogoal = false;
+  chz::DBuf<uint64_t> gt_keys, gt_hit;
+  chz::DBuf<uint8_t> gt_cost;
+  chz::DBuf<unsigned long long> gt_best;
 };
 
 namespace chz {
@@ -932,9 +945,9 @@ static void walk_circuit(chz_ctx *c, unsigned long long o, int G, std::vector<in
   for (int s = c->gstep; s >= 0; s--) {
     uint32_t par;
     uint8_t g;
-    HIPCHK(hipMemcpy(&par, c->tree_par[s] + i, 4, hipMemcpyDeviceToHost));
+    chz::copy_sync(&par, c->tree_par[s] + i, 4, hipMemcpyDeviceToHost, c->stream);
     if (par == NO_PARENT) break;
-    HIPCHK(hipMemcpy(&g, c->tree_gate[s] + i, 1, hipMemcpyDeviceToHost));
+    chz::copy_sync(&g, c->tree_gate[s] + i, 1, hipMemcpyDeviceToHost, c->stream);
     seq.push_back(g);
     i = par;
   }
@@ -958,10 +971,12 @@ static void prune_tree(chz_ctx *c) {
 // is rebuilt when a buffer address, the slot count or the schedule it was captured with changes.  The gather runs
 // after it as an ordinary launch (its tree level differs every step), then the level synchronises once.
 constexpr long long STEP_CAP = 1024;
-// Switched off: with captured steps on, the multi-threaded request-server test stopped making progress (the
-// single-threaded searches, batches and golden logs all passed through them); the cause is not determined, so the
-// level step runs uncaptured until it is.
-constexpr bool kStepGraphs = false;
+// Captures run while other threads (the request server's other engines) keep working on the device.  None of
+// them may synchronise the whole device or allocate during a capture (a device-wide synchronisation includes the
+// stream under capture; HIP's hipFree synchronises the device): buffer growth waits for the context's own stream
+// only, and allocations, frees, device-wide synchronisations and memory queries take capture_mu, which the capture
+// holds.
+constexpr bool kStepGraphs = true;
 constexpr int STEP_BUCKET = 10;    // the schedule of the 1,024-parent bucket drives the captured expansion
 
 template <int NQ>
@@ -1001,10 +1016,12 @@ static void step_graph(chz_ctx *c, int cur, long long m, const unsigned long lon
     const unsigned long long *dm = c->cnt + C_M, *dncc = c->cnt + C_NCC, *dsurv = c->cnt + C_SURV;
     hipStream_t s = c->stream;
     hipGraph_t graph = nullptr;
+    // no device-wide or allocating HIP call from another thread while this stream is captured (capture_mu)
+    std::lock_guard<std::recursive_mutex> capture_lock(capture_mu());
     HIPCHK(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
     ExpandArgs ea{c->res[cur].p, c->pk[cur].p, c->g1[cur].p, c->g2[cur].p, 0, ncap, 0, 0,
                   c->status.p, c->keys.p, c->info.p, c->pb[cur].p, c->cnt, pinst, c->d_rem, c->d_keepi,
-                  c->icnt, 1, dm, sc.tw, c->gflag.p};
+                  c->icnt, 1, dm, sc.tw, c->goal_skip ? c->gflag.p : nullptr};
     const dim3 grid(grid_for((cap * G + CPW - 1) / CPW, sc.cap));
     switch (sc.var) {
       case 0: hipLaunchKernelGGL((k_level<NQ, 4>), grid, dim3(256), 0, s, ks, ea, cap); break;
@@ -1052,7 +1069,7 @@ static void step_graph(chz_ctx *c, int cur, long long m, const unsigned long lon
   c->ev.end(c->stream);
   const GatherArgs ga{c->kidx.p, ncap, c->surv_o.p, c->res[cur].p, c->pk[cur].p, c->g1[cur].p,
                       c->res[nxt].p, c->pk[nxt].p, c->g1[nxt].p, c->g2[nxt].p, tp, tg, pinst, c->finst[nxt].p,
-                      c->surv_bw.p, c->pb[nxt].p, c->bcount.p + nbA, c->gflag.p, CPW};
+                      c->surv_bw.p, c->pb[nxt].p, c->bcount.p + nbA, c->goal_skip ? c->gflag.p : nullptr, CPW};
   HIPCHK(hipMemsetAsync(c->gflag.p, 0, (size_t)ns * 4, c->stream));   // the graph's expansion has read them
   c->ev.begin(K_GATHER, c->stream);
   hipLaunchKernelGGL(k_gather<NQ>, dim3(grid_for((ncap + CPW - 1) / CPW)), dim3(256), 0, c->stream, ks, ga);
@@ -1127,7 +1144,8 @@ template <int NQ> static void batch_step(chz_ctx *c) {
   uint8_t *tg = nullptr;
   // small levels run as one captured graph (expansion through kept compaction) plus the gather: one
   // synchronisation per level instead of two, and one launch instead of about twenty
-  const bool graphed = kStepGraphs && !c->tuning && c->opts.dedupe && m > 0 && m <= STEP_CAP && m <= chunk;
+  const bool graphed = kStepGraphs && !c->tuning && !c->gt_filter && c->opts.dedupe && m > 0 && m <= STEP_CAP &&
+                       m <= chunk;
   if (graphed) {
     for (int i = 0; i < C_N; i++) hc_in[i] = 0;
     hc_in[C_M] = (unsigned long long)m;
@@ -1162,7 +1180,7 @@ template <int NQ> static void batch_step(chz_ctx *c) {
     const int gsplit = (int)std::max<long long>(1, std::min<long long>(G, ((long long)sc.tw * CPW + cm - 1) / cm));
     ExpandArgs ea{c->res[cur].p, c->pk[cur].p, c->g1[cur].p, c->g2[cur].p, off, ncc, 0, 0,
                   c->status.p, c->keys.p, c->info.p, c->pb[cur].p, c->cnt, pinst, c->d_rem, c->d_keepi,
-                  c->icnt, gsplit, nullptr, 0, c->gflag.p};
+                  c->icnt, gsplit, nullptr, 0, c->goal_skip ? c->gflag.p : nullptr};
     const dim3 grid(grid_for((cm * gsplit + CPW - 1) / CPW, sc.cap));
     hipEvent_t te0 = nullptr, te1 = nullptr;
     if (c->tuning) {
@@ -1245,6 +1263,10 @@ template <int NQ> static void batch_step(chz_ctx *c) {
     } else {
       HIPCHK(hipMemsetAsync(c->kept.p, 1, nsurv, c->stream));
     }
+    if (c->gt_filter)      // a table build: classes already in the table (lower levels) are dropped
+      hipLaunchKernelGGL(k_gt_filter<NQ>, dim3(grid_for(nsurv / CPW + 1)), dim3(256), 0, c->stream, ks,
+                         c->gt_filter->dev(), c->surv_o.p, c->surv_key.p, nsurv, c->res[cur].p, c->pk[cur].p,
+                         c->kept.p);
     const long long nb = (nsurv + CT_TILE - 1) / CT_TILE;
     c->bcount.need(nb + 1);
     const FlagKept fk{c->kept.p, c->surv_o.p, pinst, c->d_active.p, G};
@@ -1282,7 +1304,8 @@ template <int NQ> static void batch_step(chz_ctx *c) {
     if (distinct && !graphed) {
       const GatherArgs ga{c->kidx.p, distinct, c->surv_o.p, c->res[cur].p, c->pk[cur].p, c->g1[cur].p,
                           c->res[nxt].p, c->pk[nxt].p, c->g1[nxt].p, c->g2[nxt].p, tp, tg, pinst, c->finst[nxt].p,
-                          c->surv_bw.p, c->pb[nxt].p, nullptr, c->gflag.p, gather_tile<NQ>(distinct)};
+                          c->surv_bw.p, c->pb[nxt].p, nullptr, c->goal_skip ? c->gflag.p : nullptr,
+                          gather_tile<NQ>(distinct)};
       c->ev.begin(K_GATHER, c->stream);
       hipLaunchKernelGGL(k_gather<NQ>, dim3(grid_for((distinct + ga.tn - 1) / ga.tn)), dim3(256), 0, c->stream, ks,
                          ga);
@@ -1290,6 +1313,8 @@ template <int NQ> static void batch_step(chz_ctx *c) {
       c->k_work[K_GATHER] += (double)distinct * (E * 16.0 * 2.0 + 16.0);
     }
     const bool single = c->pending.empty() && chz_batch_live(c) == 1;
+    if (c->gt_filter && distinct)   // a table build keeps every level, resident, with its classes' keys
+      gt_add_level<NQ>(ks, *c->gt_filter, c->res[nxt].p, c->pk[nxt].p, tp, tg, distinct, c->stream);
     append_roots<NQ>(c, nxt, distinct, tp, tg);
     c->tree_par.push_back(tp);
     c->tree_gate.push_back(tg);
@@ -1431,10 +1456,13 @@ static void upload_spec(chz_ctx *c) {
   std::vector<uint64_t> xgm(G, 0);
   std::vector<int> xgk(G, -1);
   uint64_t gmax = 1;
+  std::vector<HostMat> pre;
+  std::vector<char> have;
+  c->goal_skip = goal_preimages(s, pre, have);
   for (int g = 0; g < G; g++) {
     const HostGate &h = s.gates[g];
-    HostMat X;
-    if (N * N <= 64 && host_goal_preimage(s, g, X)) {
+    const HostMat &X = pre[g];
+    if (have[g]) {
       xgk[g] = X.k;
       for (int i = 0; i < N * N; i++) {
         const Zq &z = X.e[i];
@@ -1456,7 +1484,7 @@ static void upload_spec(chz_ctx *c) {
   add((void **)&c->d_xgm, xgm.data(), xgm.size() * 8);
   add((void **)&c->d_xgk, xgk.data(), xgk.size() * 4);
   c->spec_blob.need(blob.size());
-  HIPCHK(hipMemcpy(c->spec_blob.p, blob.data(), blob.size(), hipMemcpyHostToDevice));
+  chz::copy_sync(c->spec_blob.p, blob.data(), blob.size(), hipMemcpyHostToDevice, c->stream);
   for (auto &d : at) *d.first = c->spec_blob.p + d.second;
   KSpec &k = c->ks;
   std::memset(&k, 0, sizeof k);
@@ -1511,7 +1539,7 @@ static int batch_add_parsed(chz_ctx *c, const HostSpec &hs, int depth, int tag) 
   HostMat U;
   U.e = hs.target;
   U.k = hs.tk;
-  if (host_equal(U, c->hs.goal, c->hs.gk)) { ev.result.rc = 0; c->events.push_back(ev); return 0; }
+  if (!c->nogoal && host_equal(U, c->hs.goal, c->hs.gk)) { ev.result.rc = 0; c->events.push_back(ev); return 0; }
   if (depth <= 0) { ev.result.rc = 1; c->events.push_back(ev); return 0; }
   // a free instance slot: not held by a live search
   std::vector<bool> used(c->nslots + c->binst.size() + 1, false);
@@ -1546,6 +1574,8 @@ static int batch_add_parsed(chz_ctx *c, const HostSpec &hs, int depth, int tag) 
   return 0;
 }
 
+#include "table_join.h"
+
 }  // namespace chz
 
 // ------------------------------------------------------------------------------------------------------
@@ -1569,7 +1599,9 @@ int chz_open(const char *spec_text, size_t len, const chz_opts *opts, chz_ctx **
   if (c->hs.n < 1 || c->hs.n > 3) { chz::set_err("n = " + std::to_string(c->hs.n) + " not built (1..3)"); return 3; }
   if (c->hs.truncated) { chz::set_err("malformed spec"); return 2; }
   if ((int)c->hs.target.size() != c->hs.N() * c->hs.N()) { chz::set_err("spec has no target"); return 2; }
-  c->grp = chz::build_symmetry(c->hs, o.sym != 0);
+  // symmetry classes: not with --mitm, whose join compares exact residuals (src/cheese.cu:1786)
+  c->grp = chz::build_symmetry(c->hs, o.sym != 0 && !o.mitm);
+  if (o.table < 0 || o.table > CHZ_DMAX) { chz::set_err("--table K outside 0..48"); return 3; }
   // default occupancy: 5 waves per SIMD fits the n = 3 kernel's registers; smaller groups spill there, so 4
   if (c->hs.n < 3)
     for (auto &sc : c->sched) sc.var = 0;
@@ -1577,6 +1609,7 @@ int chz_open(const char *spec_text, size_t len, const chz_opts *opts, chz_ctx **
     c->device = o.device;
     HIPCHK(hipSetDevice(o.device));
     HIPCHK(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking));
+    chz::StreamScope scope(c->stream);
     chz::upload_spec(c.get());
   } catch (chz::DeviceError &) {
     return 5;
@@ -1593,6 +1626,21 @@ int chz_run(chz_ctx *c, chz_level_cb cb, void *user, chz_result *out) {
   for (int f = 0; f < chz::K_NFAM; f++) { c->k_launch[f] = 0; c->k_ms[f] = 0; c->k_work[f] = 0; }
   for (double &v : c->k_stage) v = 0;
   c->events.clear();
+  res->split_kind = 0;
+  res->split[0] = res->split[1] = -1;
+  if (c->opts.mitm || c->opts.table > 0) {   // --mitm takes precedence, as in the reference (src/cheese.cu:1803)
+    try {
+      HIPCHK(hipSetDevice(c->device));
+      chz::StreamScope scope(c->stream);
+      switch (c->hs.n) {
+        case 1: return c->opts.mitm ? chz::run_mitm<1>(c, cb, user, res) : chz::run_table<1>(c, cb, user, res);
+        case 2: return c->opts.mitm ? chz::run_mitm<2>(c, cb, user, res) : chz::run_table<2>(c, cb, user, res);
+        default: return c->opts.mitm ? chz::run_mitm<3>(c, cb, user, res) : chz::run_table<3>(c, cb, user, res);
+      }
+    } catch (chz::DeviceError &) {
+      return res->rc = 5;
+    }
+  }
   const int rc0 = chz::batch_add_parsed(c, c->hs, c->depth, 0);
   if (rc0) return res->rc = rc0;
   while (true) {
@@ -1680,6 +1728,7 @@ int chz_batch_add(chz_ctx *c, const char *spec_text, size_t len, int depth, int 
 int chz_batch_step(chz_ctx *c) {
   try {
     HIPCHK(hipSetDevice(c->device));
+    chz::StreamScope scope(c->stream);
     switch (c->hs.n) {
       case 1: chz::batch_step<1>(c); break;
       case 2: chz::batch_step<2>(c); break;
@@ -1818,6 +1867,8 @@ void chz_close(chz_ctx *c) {
   c->surv_bw.release();
   c->info.release(); c->bcount.release(); c->hist.release(); c->runs.release(); c->kidx.release();
   c->spec_blob.release();
+  if (c->gt) { c->gt->release(); delete c->gt; c->gt = nullptr; }
+  c->gt_keys.release(); c->gt_hit.release(); c->gt_cost.release(); c->gt_best.release();
   c->ev.destroy();
   if (c->stream) (void)hipStreamDestroy(c->stream);
   delete c;

@@ -57,7 +57,12 @@ struct chz_server {
   chz_cliff *cl = nullptr;
   std::string err;
   int plain = 0;
-  long long mem_peak = 0;
+  bool dead[2] = {false, false};   // a gate engine that could not reopen after a device error (answers rc 5)
+  std::atomic<long long> mem_peak{0};       // largest frontier-memory estimate of a gate batch
+  std::atomic<long long> used_peak{0};      // largest device memory in use, sampled after every gate step
+  // per-engine statistics (under mu): gate batches [2], T-count, Clifford
+  double steps[2] = {0, 0}, step_s[2] = {0, 0}, kernel_ms[2] = {0, 0}, live_peak[2] = {0, 0};
+  double tc_batches = 0, tc_s = 0, cl_batches = 0, cl_s = 0;
 };
 
 namespace {
@@ -66,6 +71,27 @@ void finish(chz_server *s, const chz_response &resp) {
   std::lock_guard<std::mutex> lk(s->mu);
   s->done.push_back(resp);
   s->outstanding--;
+}
+
+void raise_to(std::atomic<long long> &a, long long v) {
+  long long cur = a.load();
+  while (v > cur && !a.compare_exchange_weak(cur, v)) {}
+}
+
+// device memory in use now (every process's), 0 if unknown; taken under the capture lock (a device-wide query)
+long long device_used_bytes() {
+  size_t free_b = 0, total_b = 0;
+  std::lock_guard<std::recursive_mutex> g(chz::capture_mu());
+  if (hipMemGetInfo(&free_b, &total_b) != hipSuccess) return 0;
+  return (long long)(total_b - free_b);
+}
+
+double ctx_kernel_ms(const chz_ctx *c) {
+  chz_kstat ks[16];
+  const int n = chz_kernel_stats(c, ks, 16);
+  double t = 0;
+  for (int i = 0; i < n; i++) t += ks[i].ms;
+  return t;
 }
 
 chz_response base_resp(chz_server *s, const Req &q) {
@@ -86,6 +112,7 @@ constexpr double kChildBytes = 48.0;
 constexpr double kGrowth = 4.0;
 constexpr int kAdmitPerStep = 64;
 constexpr double kFreeReserve = 24e9;
+constexpr long long kHbmBytes = 288ll << 30;
 // Searches already admitted keep growing for several levels, which no admission-time estimate sees; without a
 // caller's max_batch, a batch holds at most this many live searches (2 and 3 qubits)
 constexpr int kMaxLive[2] = {1024, 64};
@@ -94,6 +121,7 @@ double per_node_bytes(int qi, int G) { return kNodeBytes[qi] * (1.0 + kGrowth) +
 
 // ---- gate-count searches: one batch per qubit count, stepped level by level
 void gate_worker(chz_server *s, int qi) {
+  chz::RelaxedCapture relaxed;    // this thread's calls never invalidate another engine's capture
   chz_ctx *ctx = s->gctx[qi];
   const int G = chz_num_gates(ctx);
   std::map<int, Req> live;      // tag -> request
@@ -112,8 +140,8 @@ void gate_worker(chz_server *s, int qi) {
       const double per_node = per_node_bytes(qi, G);
       double held = (double)chz_batch_nodes(ctx) * per_node;
       const double want = (double)G * G * per_node;
-      size_t free_b = 0, total_b = 0;
-      const bool low_mem = hipMemGetInfo(&free_b, &total_b) == hipSuccess && (double)free_b < kFreeReserve;
+      const long long used = device_used_bytes();
+      const bool low_mem = used > 0 && (double)(kHbmBytes - used) < kFreeReserve;
       int admitted = 0;
       while (!s->q_gates[qi].empty()) {
         if (!live.empty() && (held + want > budget || admitted >= kAdmitPerStep || low_mem)) break;
@@ -137,6 +165,8 @@ void gate_worker(chz_server *s, int qi) {
       }
     }
     if (chz_batch_live(ctx)) {
+      const double ts = clock_s(s->t0);
+      const int nlive = chz_batch_live(ctx);
       if (chz_batch_step(ctx)) {   // a device error: every live search of this batch ends with it
         {
           std::lock_guard<std::mutex> lk(s->mu);
@@ -158,7 +188,9 @@ void gate_worker(chz_server *s, int qi) {
           ctx = s->gctx[qi] = fresh;
           continue;
         }
+        // no engine left for this qubit count: queued and later requests are answered with the error at once
         std::lock_guard<std::mutex> lk(s->mu);
+        s->dead[qi] = true;
         while (!s->q_gates[qi].empty()) {
           chz_response r = base_resp(s, s->q_gates[qi].front());
           r.rc = 5;
@@ -169,8 +201,14 @@ void gate_worker(chz_server *s, int qi) {
         }
         return;
       }
-      const long long mem = (long long)((double)chz_batch_nodes(ctx) * per_node_bytes(qi, G));
-      if (mem > s->mem_peak) s->mem_peak = mem;
+      raise_to(s->mem_peak, (long long)((double)chz_batch_nodes(ctx) * per_node_bytes(qi, G)));
+      raise_to(s->used_peak, device_used_bytes());
+      const double kms = ctx_kernel_ms(ctx);
+      std::lock_guard<std::mutex> lk(s->mu);
+      s->steps[qi] += 1;
+      s->step_s[qi] += clock_s(s->t0) - ts;
+      s->kernel_ms[qi] = kms;
+      s->live_peak[qi] = std::max(s->live_peak[qi], (double)nlive);
     }
     int n;
     while ((n = chz_batch_events(ctx, ev.data(), (int)ev.size())) > 0)
@@ -200,6 +238,7 @@ void tc_fill(const chz_tc_result &t, chz_response &r) {
 
 // ---- T-count: every queued target in one batched search
 void tc_worker(chz_server *s) {
+  chz::RelaxedCapture relaxed;    // this thread's calls never invalidate another engine's capture
   while (true) {
     std::vector<Req> batch;
     {
@@ -210,6 +249,7 @@ void tc_worker(chz_server *s) {
       while (!s->q_tc.empty() && (int)batch.size() < cap) { batch.push_back(std::move(s->q_tc.front())); s->q_tc.pop_front(); }
     }
     if (batch.empty()) continue;
+    const double ts = clock_s(s->t0);
     // one search per maximum (the maximum bounds how deep a batch peels)
     std::map<int, std::vector<int>> bymax;
     for (int i = 0; i < (int)batch.size(); i++) bymax[batch[i].r.budget >= 0 ? batch[i].r.budget : 16].push_back(i);
@@ -229,6 +269,9 @@ void tc_worker(chz_server *s) {
         finish(s, r);
       }
     }
+    std::lock_guard<std::mutex> lk(s->mu);
+    s->tc_batches += 1;
+    s->tc_s += clock_s(s->t0) - ts;
   }
 }
 
@@ -254,6 +297,7 @@ void cl_fill(chz_server *s, std::vector<Req> &batch) {
 
 // ---- Clifford re-synthesis: every queued word in one read-back launch
 void cl_worker(chz_server *s) {
+  chz::RelaxedCapture relaxed;    // this thread's calls never invalidate another engine's capture
   while (true) {
     std::vector<Req> batch;
     {
@@ -262,12 +306,18 @@ void cl_worker(chz_server *s) {
       if (s->stop && s->q_cl.empty()) return;
       while (!s->q_cl.empty()) { batch.push_back(std::move(s->q_cl.front())); s->q_cl.pop_front(); }
     }
-    if (!batch.empty()) cl_fill(s, batch);
+    if (batch.empty()) continue;
+    const double ts = clock_s(s->t0);
+    cl_fill(s, batch);
+    std::lock_guard<std::mutex> lk(s->mu);
+    s->cl_batches += 1;
+    s->cl_s += clock_s(s->t0) - ts;
   }
 }
 
 // ---- plain: one request at a time, in arrival order
 void plain_worker(chz_server *s) {
+  chz::RelaxedCapture relaxed;    // this thread's calls never invalidate another engine's capture
   std::vector<chz_event> ev(64);
   while (true) {
     Req q;
@@ -300,6 +350,8 @@ void plain_worker(chz_server *s) {
             }
         if (!got && chz_batch_step(ctx)) { r.rc = 5; got = true; }
       }
+    } else if (q.r.kind == CHZ_REQ_TCOUNT && !s->tc) {
+      r.rc = 3;
     } else if (q.r.kind == CHZ_REQ_TCOUNT) {
       const char *sp = q.text.data();
       const size_t ln = q.text.size();
@@ -364,18 +416,36 @@ int chz_server_submit(chz_server *s, const chz_request *r) {
   q.r.text = nullptr;
   q.r.word = nullptr;
   q.t_submit = clock_s(s->t0);
+  if (r->kind != CHZ_REQ_GATES && r->kind != CHZ_REQ_TCOUNT && r->kind != CHZ_REQ_CLIFFORD) return 3;
   {
     std::lock_guard<std::mutex> lk(s->mu);
-    if (s->plain) s->q_plain.push_back(std::move(q));
-    else if (r->kind == CHZ_REQ_GATES) {
+    // a request no engine can take is answered at once: rc 3 when its kind or qubit count is not served, rc 5
+    // when its engine ended after a device error
+    int now_rc = 0;
+    int qi = -1;
+    if (r->kind == CHZ_REQ_GATES) {
       // route by qubit count: the "n" line of the spec
       const size_t p = q.text.find("\nn ");
       const int n = p == std::string::npos ? 0 : std::atoi(q.text.c_str() + p + 3);
-      s->q_gates[n == 2 ? 0 : 1].push_back(std::move(q));
-    } else if (r->kind == CHZ_REQ_TCOUNT) s->q_tc.push_back(std::move(q));
-    else if (r->kind == CHZ_REQ_CLIFFORD) s->q_cl.push_back(std::move(q));
-    else return 3;
+      qi = n == 2 ? 0 : n == 3 ? 1 : -1;
+      if (qi < 0 || !s->gctx[qi]) now_rc = 3;
+      else if (s->dead[qi]) now_rc = 5;
+    } else if (r->kind == CHZ_REQ_TCOUNT && !s->tc) {
+      now_rc = 3;
+    }
     s->outstanding++;
+    if (now_rc) {
+      chz_response a = base_resp(s, q);
+      a.rc = now_rc;
+      a.t_done = q.t_submit;
+      s->done.push_back(a);
+      s->outstanding--;
+      return 0;
+    }
+    if (s->plain) s->q_plain.push_back(std::move(q));
+    else if (r->kind == CHZ_REQ_GATES) s->q_gates[qi].push_back(std::move(q));
+    else if (r->kind == CHZ_REQ_TCOUNT) s->q_tc.push_back(std::move(q));
+    else s->q_cl.push_back(std::move(q));
   }
   s->cv.notify_all();
   return 0;
@@ -390,7 +460,18 @@ int chz_server_poll(chz_server *s, chz_response *out, int max) {
 
 long long chz_server_outstanding(const chz_server *s) { return s->outstanding.load(); }
 
-long long chz_server_peak_bytes(const chz_server *s) { return s->mem_peak; }
+long long chz_server_peak_bytes(const chz_server *s) { return s->mem_peak.load(); }
+
+long long chz_server_peak_device_bytes(const chz_server *s) { return s->used_peak.load(); }
+
+int chz_server_stats(chz_server *s, double *out, int n) {
+  std::lock_guard<std::mutex> lk(s->mu);
+  const double v[] = {s->steps[0], s->steps[1], s->step_s[0], s->step_s[1], s->kernel_ms[0], s->kernel_ms[1],
+                      s->live_peak[0], s->live_peak[1], s->tc_batches, s->tc_s, s->cl_batches, s->cl_s};
+  const int k = std::min(n, (int)(sizeof v / sizeof v[0]));
+  for (int i = 0; i < k; i++) out[i] = v[i];
+  return k;
+}
 
 const char *chz_server_error(chz_server *s) {
   std::lock_guard<std::mutex> lk(s->mu);

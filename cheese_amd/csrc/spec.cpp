@@ -235,6 +235,17 @@ bool host_goal_preimage(const HostSpec &s, int gi, HostMat &X) {
   return !Y.ovf && host_equal(Y, s.goal, s.gk);
 }
 
+bool goal_preimages(const HostSpec &s, std::vector<HostMat> &X, std::vector<char> &have) {
+  X.assign(s.G(), HostMat{});
+  have.assign(s.G(), 0);
+  bool all = true;
+  for (int g = 0; g < s.G(); g++) {
+    have[g] = s.N() * s.N() <= 64 && host_goal_preimage(s, g, X[g]);
+    all = all && have[g];
+  }
+  return all;
+}
+
 static bool zq_vec_eq(const std::vector<Zq> &a, const std::vector<Zq> &b) {
   if (a.size() != b.size()) return false;
   for (size_t i = 0; i < a.size(); i++)

@@ -53,6 +53,10 @@ bool host_equal(const HostMat &S, const std::vector<Zq> &M, int k);
 // X with X g^-1 = goal, in normal form: the one residual from which candidate gi reaches the goal.  False when
 // the product check fails or a coefficient leaves int32.
 bool host_goal_preimage(const HostSpec &s, int gi, HostMat &X);
+// the pre-images of every candidate (X[g], with have[g] = 0 where host_goal_preimage fails); returns true when
+// every candidate has one, which the search's found-level skip requires (a parent without a goal child is passed
+// over only if no candidate can reach the goal from it, which is decided by comparing with every pre-image)
+bool goal_preimages(const HostSpec &s, std::vector<HostMat> &X, std::vector<char> &have);
 
 // two specs share everything the search kernels read except the target and the depth: qubit count, mode,
 // candidates and their matrices, windows, cuts, cx table, bound flags and goal (searches of such specs can run

@@ -748,7 +748,7 @@ struct Levels {
   std::vector<uint64_t *> keys;
   std::vector<long long> count;
   void release() {
-    (void)hipDeviceSynchronize();      // the blocks return to the cache: no kernel may still use them
+    (void)chz::sync_owner();           // the blocks return to the cache: no kernel may still use them
     for (auto p : par) chz::dev_free(p);
     for (auto p : inst) chz::dev_free(p);
     for (auto p : rot) chz::dev_free(p);
@@ -827,7 +827,6 @@ template <int NQ> static void materialise(chz_tc *c, Levels &L, int t) {
   }
   if (t >= 2 && L.mat[t - 1] && L.mat[t]) {
     HIPCHK(hipStreamSynchronize(c->stream));
-    HIPCHK(hipDeviceSynchronize());
     chz::dev_free(L.mat[t - 1]);
     L.mat[t - 1] = nullptr;
   }
@@ -977,7 +976,7 @@ static void table_reserve(chz_tc *c, long long need) {
   unsigned long long slots = 1ull << 12;
   while ((double)need > 0.45 * (double)slots) slots <<= 1;
   if (c->d_hkey) {
-    (void)hipDeviceSynchronize();
+    (void)hipStreamSynchronize(c->stream);
     chz::dev_free(c->d_hkey);
     chz::dev_free(c->d_hval);
     c->d_hkey = c->d_hval = nullptr;
@@ -1043,11 +1042,11 @@ static void upload_rotations(chz_tc *c) {
   c->d_coef = dalloc<uint32_t>(G * P);
   c->d_rv = dalloc<uint32_t>(P);
   c->d_rw = dalloc<uint32_t>(P);
-  HIPCHK(hipMemcpy(c->d_anti, anti.data(), G * 8, hipMemcpyHostToDevice));
-  HIPCHK(hipMemcpy(c->d_plus, plus.data(), G * 8, hipMemcpyHostToDevice));
-  HIPCHK(hipMemcpy(c->d_coef, coef.data(), G * P * 4, hipMemcpyHostToDevice));
-  HIPCHK(hipMemcpy(c->d_rv, rv.data(), P * 4, hipMemcpyHostToDevice));
-  HIPCHK(hipMemcpy(c->d_rw, rw.data(), P * 4, hipMemcpyHostToDevice));
+  chz::copy_sync(c->d_anti, anti.data(), G * 8, hipMemcpyHostToDevice, c->stream);
+  chz::copy_sync(c->d_plus, plus.data(), G * 8, hipMemcpyHostToDevice, c->stream);
+  chz::copy_sync(c->d_coef, coef.data(), G * P * 4, hipMemcpyHostToDevice, c->stream);
+  chz::copy_sync(c->d_rv, rv.data(), P * 4, hipMemcpyHostToDevice, c->stream);
+  chz::copy_sync(c->d_rw, rw.data(), P * 4, hipMemcpyHostToDevice, c->stream);
 }
 
 // roots as residues
@@ -1092,7 +1091,7 @@ template <int NQ> static void build_table(chz_tc *c) {
   std::vector<uint32_t> hf(P * P);
   root_residues(id, hf.data());
   c->d_troot_fp = dalloc<uint32_t>(P * P);
-  HIPCHK(hipMemcpy(c->d_troot_fp, hf.data(), P * P * 4, hipMemcpyHostToDevice));
+  chz::copy_sync(c->d_troot_fp, hf.data(), P * P * 4, hipMemcpyHostToDevice, c->stream);
   Levels &T = c->tab;
   T.par.push_back(nullptr);
   T.rot.push_back(nullptr);
@@ -1119,7 +1118,7 @@ template <int NQ> static void build_table(chz_tc *c) {
   // the table's nodes are replayed from the identity when a lookup needs them; drop materialised levels
   HIPCHK(hipStreamSynchronize(c->stream));
   for (size_t l = 1; l < T.mat.size(); l++)
-    if (T.mat[l]) { HIPCHK(hipDeviceSynchronize()); chz::dev_free(T.mat[l]); T.mat[l] = nullptr; }
+    if (T.mat[l]) { chz::dev_free(T.mat[l]); T.mat[l] = nullptr; }
 }
 
 // one batch of searches against the table (src/tcount.cu:497-531 per instance)
@@ -1154,14 +1153,14 @@ static void search_batch(chz_tc *c, std::vector<Inst> &ins, int maxt, chz_tc_res
   long long *d_lo = keep_ptr(dalloc<long long>(ni));
   long long *d_hi = keep_ptr(dalloc<long long>(ni));
   uint32_t *d_hitx = nullptr;
-  HIPCHK(hipMemcpy(rf, hf.data(), hf.size() * 4, hipMemcpyHostToDevice));
+  chz::copy_sync(rf, hf.data(), hf.size() * 4, hipMemcpyHostToDevice, c->stream);
   S.par.push_back(nullptr);
   S.rot.push_back(nullptr);
   S.mat.push_back(rf);
   S.inst.push_back(dalloc<uint32_t>(ni));
   S.keys.push_back(dalloc<uint64_t>(ni));
   S.count.push_back(ni);
-  HIPCHK(hipMemcpy(S.inst[0], hinst.data(), ni * 4, hipMemcpyHostToDevice));
+  chz::copy_sync(S.inst[0], hinst.data(), ni * 4, hipMemcpyHostToDevice, c->stream);
   root_keys<NQ>(c, rf, ni, S.keys[0]);
 
   std::vector<int> state(ni, -1);             // -1 searching, else the result code
@@ -1314,7 +1313,8 @@ int chz_tc_table(const char *spec_text, size_t len, int K, int device, chz_tc **
   c->spec = s;
   try {
     HIPCHK(hipSetDevice(device));
-    HIPCHK(hipStreamCreate(&c->stream));
+    HIPCHK(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking));
+    chz::StreamScope scope(c->stream);
     upload_rotations(c);
     switch (c->n) {
       case 1: build_table<1>(c); break;
@@ -1352,6 +1352,7 @@ int chz_tc_search(chz_tc *c, const char *const *specs, const size_t *lens, int n
   using namespace tcn;
   if (!c || ninst < 0 || (ninst > 0 && (!specs || !lens || !res))) { set_err("null argument"); return 3; }
   if (maxt < 0 || maxt > MAXT) { set_err("maxt outside 0..32"); return 3; }
+  chz::StreamScope scope(c->stream);
   const double t0 = chz::now_s();
   // parse and check every target (exact orthogonality is O(P^3) each), spread over host threads
   std::vector<Inst> all(ninst);
@@ -1421,7 +1422,8 @@ int chz_tc_dedupe_counts(const chz_tc *c, double *out) {
 void chz_tc_free(chz_tc *c) {
   if (!c) return;
   (void)hipSetDevice(c->device);
-  (void)hipDeviceSynchronize();      // cached device blocks must be idle before they are reused
+  chz::StreamScope scope(c->stream);
+  (void)chz::sync_owner();           // cached device blocks must be idle before they are reused
   c->tab.release();
   void *ps[] = {c->d_anti, c->d_plus, c->d_coef, c->d_rv, c->d_rw, c->d_troot_fp, c->d_hkey, c->d_hval};
   for (void *p : ps) chz::dev_free(p);

@@ -45,6 +45,8 @@ def _lib():
             "chz_server_poll": (ctypes.c_int, [ctypes.c_void_p, P(Response), ctypes.c_int]),
             "chz_server_outstanding": (ctypes.c_longlong, [ctypes.c_void_p]),
             "chz_server_peak_bytes": (ctypes.c_longlong, [ctypes.c_void_p]),
+            "chz_server_peak_device_bytes": (ctypes.c_longlong, [ctypes.c_void_p]),
+            "chz_server_stats": (ctypes.c_int, [ctypes.c_void_p, P(ctypes.c_double), ctypes.c_int]),
             "chz_server_error": (ctypes.c_char_p, [ctypes.c_void_p]),
             "chz_server_clock": (ctypes.c_double, [ctypes.c_void_p]),
             "chz_server_close": (None, [ctypes.c_void_p]),
@@ -109,6 +111,24 @@ class Server:
     def peak_bytes(self):
         return _lib().chz_server_peak_bytes(self._h)
 
+    @property
+    def peak_device_bytes(self):
+        """largest device memory in use seen after a gate-count step (measured)"""
+        return _lib().chz_server_peak_device_bytes(self._h)
+
+    def stats(self):
+        """per-engine work so far: level steps, seconds and kernel milliseconds of the 2- and 3-qubit gate batches,
+        T-count and Clifford batches and seconds"""
+        buf = (ctypes.c_double * 16)()
+        n = _lib().chz_server_stats(self._h, buf, 16)
+        v = list(buf[:n])
+        return {"gates2": {"steps": int(v[0]), "seconds": round(v[2], 4), "kernel_ms": round(v[4], 2),
+                           "live_peak": int(v[6])},
+                "gates3": {"steps": int(v[1]), "seconds": round(v[3], 4), "kernel_ms": round(v[5], 2),
+                           "live_peak": int(v[7])},
+                "tcount": {"batches": int(v[8]), "seconds": round(v[9], 4)},
+                "clifford": {"batches": int(v[10]), "seconds": round(v[11], 4)}}
+
     def error(self) -> str:
         """the last device error a worker met ("" if none)"""
         return _lib().chz_server_error(self._h).decode()
@@ -168,8 +188,9 @@ def poisson_arrivals(n, rate, seed):
     return out
 
 
-def replay(server: Server, requests, arrivals):
-    """submit request i at arrivals[i] seconds after the start; returns (answers by id, wall seconds)"""
+def replay(server: Server, requests, arrivals, deadline_s: float = 600.0):
+    """submit request i at arrivals[i] seconds after the start; returns (answers by id, wall seconds).  Raises
+    when answers are still missing deadline_s seconds after the last arrival, naming them."""
     t0 = time.perf_counter()
     answers = {}
     i = 0
@@ -187,6 +208,10 @@ def replay(server: Server, requests, arrivals):
         if i < n:
             time.sleep(max(0.0, min(0.002, arrivals[i] - (time.perf_counter() - t0))))
         else:
+            if time.perf_counter() - t0 > arrivals[-1] + deadline_s:
+                missing = [j for j in range(n) if j not in answers]
+                raise RuntimeError(f"{len(missing)} requests unanswered {deadline_s:.0f} s after the last arrival "
+                                   f"(ids {missing[:20]}); server error: {server.error()!r}")
             time.sleep(0.001)
     wall = time.perf_counter() - t0
     return answers, wall

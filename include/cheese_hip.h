@@ -32,6 +32,9 @@ typedef struct chz_opts {
   long chunk;     /* parents per expansion pass, `--chunk N` (src/cheese.cu:1863); <= 0: default */
   int profile;    /* nonzero: per-pass seconds kept for chz_profile (`--profile`, src/cheese.cu:1230-1247) */
   int device;     /* HIP device ordinal */
+  int table;      /* K > 0: `--table K` (src/cheese.cu:1804-1842), the distance table of the last K gates (built on
+                     the context's first run, kept with it) */
+  int mitm;       /* nonzero: `--mitm` (src/cheese.cu:1664-1760), the meet-in-the-middle join; symmetry classes off */
 } chz_opts;
 
 /* one level of the search: the fields of the per-level JSON line (src/cheese.cu:1354-1356) */
@@ -40,6 +43,9 @@ typedef struct chz_level {
   long parents, prefixes, cut_w3, cut_lb, survivors, distinct;
   int proved_unsat;
   double seconds;
+  int dir;        /* 0 the search; 1 "dir": "table" (the table build), 2 "fwd", 3 "bwd" (--mitm sides); 4 the join
+                     line {"dir": "join", "forward": parents, "backward": prefixes, "hash_matches": distinct,
+                     "seconds"} (src/cheese.cu:1714-1715) */
 } chz_level;
 
 typedef void (*chz_level_cb)(const chz_level *lvl, void *user);
@@ -51,6 +57,10 @@ typedef struct chz_result {
   long found;              /* children equal to the goal at that level ("found" field) */
   int circuit[CHZ_DMAX];   /* candidate indices, in circuit order */
   double seconds;          /* since the search started */
+  int split_kind;          /* 1 --table: the "table": [split[0], split[1]] field of a circuit (prefix levels, table
+                              gates), or with rc 1 the proof line {"depth": D, "proved_unsat": true, "table": K}
+                              (split[1] = K); 2 --mitm: "mitm": [a, b] of a circuit or of the proof line; 0 none */
+  int split[2];
 } chz_result;
 
 /* Parses a .spec (read_spec, src/cheese.cu:1149-1226), builds the symmetry group (sym_group, :1379-1429) and
@@ -148,6 +158,15 @@ int chz_cliff_readback(chz_cliff *c, const uint64_t *tab, long long nq, int maxl
 /* count uniformly random group elements (uniform over the table's entries), seeded (no reference counterpart:
  * generates the Clifford factors of random T-count targets); 0 or 4 */
 int chz_cliff_sample(chz_cliff *c, uint64_t seed, long long count, uint64_t *tab);
+/* `cliffopt JOB [--beam B]` (src/clifford.cu:168-206, :285-385): the job text (n, the T gates in time order as
+ * "qubit dagger" lines, the t + 1 Clifford segments as "m gate q [q2] ..." lines, gates h s sdg x z y cx) is
+ * rewritten with the same T gates, each moved through a Clifford K (K Z_q' K^-1 = +-Z_q) chosen by a beam search of
+ * width beam (1..255) so that the summed optimal segment lengths are small, and every segment replaced by an
+ * optimal gate list from the table.  out receives the reference's stdout: {"clifford_gates": ...,
+ * "clifford_gates_before": ..., "t_gates": ..., "seconds": ...} and the gate lines.  The placements are taken in
+ * the table's sorted element order, so equal-cost choices are deterministic.  0, 2 (bad job), 3 (qubit count
+ * differs from the table's), 4 (out too small), 5. */
+int chz_cliff_opt(chz_cliff *c, const char *job, size_t len, int beam, char *out, size_t cap);
 void chz_cliff_free(chz_cliff *c);
 const char *chz_cliff_last_error(void);
 
@@ -256,6 +275,11 @@ int chz_server_submit(chz_server *s, const chz_request *r);
 int chz_server_poll(chz_server *s, chz_response *out, int max);   /* finished answers, up to max */
 long long chz_server_outstanding(const chz_server *s);            /* submitted and not yet answered */
 long long chz_server_peak_bytes(const chz_server *s);             /* largest frontier-memory estimate seen */
+long long chz_server_peak_device_bytes(const chz_server *s);      /* largest device memory in use, measured */
+/* per-engine work so far (call between bursts): out[0..1] level steps of the 2- and 3-qubit gate batches,
+ * [2..3] their seconds, [4..5] their kernel milliseconds, [6..7] their most live searches, [8] T-count batches,
+ * [9] their seconds, [10] Clifford batches, [11] their seconds; returns the number written */
+int chz_server_stats(chz_server *s, double *out, int n);
 const char *chz_server_error(chz_server *s);                       /* last device error a worker met ("" none) */
 double chz_server_clock(const chz_server *s);                      /* seconds since the server opened */
 void chz_server_close(chz_server *s);                              /* finishes outstanding work first */

@@ -93,6 +93,11 @@ def flags_to_kwargs(flags):
         elif f == "--depth":
             kw["depth"] = int(flags[i + 1])
             i += 1
+        elif f == "--table":
+            kw["table"] = int(flags[i + 1])
+            i += 1
+        elif f == "--mitm":
+            kw["mitm"] = True
         i += 1
     return kw
 

@@ -50,6 +50,23 @@ CASES = [
     ("rus13-unitary-d12", "rus13-unitary-opt.qasm", 12, []),
     ("rus13-unitary-d13", "rus13-unitary-opt.qasm", 13, []),
     ("rus13-unitary-d14", "rus13-unitary-opt.qasm", 14, []),
+    # the distance table of the last K gates (--table K): table build lines, the search's levels 1..D-K, then a
+    # circuit from the table or the table's proof line
+    ("toffoli-d9-table4", "toffoli.qasm", 9, ["--table", "4"]),
+    ("toffoli-d10-table5", "toffoli.qasm", 10, ["--table", "5"]),
+    ("toffoli-d12-table6", "toffoli.qasm", 12, ["--table", "6"]),
+    ("cs-d5-table3", "cs.qasm", 5, ["--table", "3"]),
+    ("qft_2-d7-table3", "qft_2.qasm", 7, ["--table", "3"]),
+    ("relative-phase-toffoli-d9-table4", "relative-phase-toffoli.qasm", 9, ["--table", "4"]),
+    ("peres-d9-table5", "peres.qasm", 9, ["--table", "5"]),
+    # the meet-in-the-middle join (--mitm): forward and backward levels, the join line, a circuit or the proof line
+    ("toffoli-d9-mitm", "toffoli.qasm", 9, ["--mitm"]),
+    ("toffoli-d10-mitm", "toffoli.qasm", 10, ["--mitm"]),
+    ("cs-d5-mitm", "cs.qasm", 5, ["--mitm"]),
+    ("qft_2-d7-mitm", "qft_2.qasm", 7, ["--mitm"]),
+    ("relative-phase-toffoli-d9-mitm", "relative-phase-toffoli.qasm", 9, ["--mitm"]),
+    ("ccz-d7-mitm", "ccz.qasm", 7, ["--mitm"]),
+    ("fredkin-d9-mitm", "fredkin.qasm", 9, ["--mitm"]),
 ]
 # (H T) on each of the 3 qubits, 8 times (48 gates): target sqrt2 exponent 14, so the T bound of the first
 # levels' children runs with n + k > 16 (the wide transform)
@@ -79,9 +96,13 @@ LONG = {"toffoli-d13": 1200, "toffoli-d14": 2400, "t-ladder-d44": 300, "rus13-un
 
 
 def run_ref(path, flags, timeout=REF_TIMEOUT):
+    import tempfile
     t = time.time()
     try:
-        p = subprocess.run([BIN, path] + flags, capture_output=True, text=True, timeout=timeout)
+        # a fresh working directory: no table cached by an earlier run (build/table-*.bin), so the table is built
+        # and its level lines printed
+        with tempfile.TemporaryDirectory() as cwd:
+            p = subprocess.run([BIN, path] + flags, capture_output=True, text=True, timeout=timeout, cwd=cwd)
     except subprocess.TimeoutExpired:
         return None, None, time.time() - t
     return p.returncode, p.stdout, time.time() - t

@@ -1,6 +1,7 @@
 // Test-only C entry points over the product's host spec code (cheese_amd/csrc/spec.cpp), built by
 // tests/test_host_spec.py with g++ so that the host logic is checked without a GPU.
 #include <string>
+#include <vector>
 
 #include "../cheese_amd/csrc/spec.h"
 
@@ -28,4 +29,16 @@ extern "C" int shim_goal_preimage(const char *spec, long len, int g, long long *
     out[4 * i] = X.e[i].a; out[4 * i + 1] = X.e[i].b; out[4 * i + 2] = X.e[i].c; out[4 * i + 3] = X.e[i].d;
   }
   return X.k;
+}
+
+// whether every candidate has a goal pre-image (the condition of the found-level skip), after multiplying the
+// goal's entries by `scale` (a large scale makes pre-images leave int32, so some candidates have none)
+extern "C" int shim_goal_skip(const char *spec, long len, long long scale) {
+  chz::HostSpec s;
+  std::string err;
+  if (chz::parse_spec(std::string(spec, (size_t)len), s, err)) return -2;
+  for (chz::Zq &z : s.goal) { z.a *= scale; z.b *= scale; z.c *= scale; z.d *= scale; }
+  std::vector<chz::HostMat> X;
+  std::vector<char> have;
+  return chz::goal_preimages(s, X, have) ? 1 : 0;
 }

@@ -11,10 +11,11 @@ import cheese_amd
 
 pytestmark = pytest.mark.gpu
 
-# proofs, found circuits (result line included), flag variants, the lite bound and the wide T transform
+# proofs, found circuits (result line included), flag variants, the lite bound, the wide T transform, the table
+# and join modes
 CASES = ["cs-d4", "cs-d5", "qft_2-d6", "qft_2-d7", "toffoli-d9", "toffoli-d9-nosym", "toffoli-d8-nodedupe",
          "toffoli-d10-chunk1000", "ccz-d8", "relative-phase-toffoli-d9", "random3-s0", "random3-s5",
-         "rus13-unitary-d12", "t-ladder-d42"]
+         "rus13-unitary-d12", "t-ladder-d42", "toffoli-d10-table5", "cs-d5-table3", "cs-d5-mitm", "toffoli-d9-mitm"]
 
 
 @pytest.mark.parametrize("name", CASES)

@@ -9,6 +9,8 @@ import subprocess
 
 import pytest
 
+from conftest import read_spec
+
 from cheese_amd import workloads as W
 
 HERE = os.path.dirname(os.path.abspath(__file__))
@@ -32,6 +34,8 @@ def shim(tmp_path_factory):
     L.shim_goal_preimage.restype = ctypes.c_int
     L.shim_goal_preimage.argtypes = [ctypes.c_char_p, ctypes.c_long, ctypes.c_int, ctypes.POINTER(ctypes.c_longlong),
                                      ctypes.c_int]
+    L.shim_goal_skip.restype = ctypes.c_int
+    L.shim_goal_skip.argtypes = [ctypes.c_char_p, ctypes.c_long, ctypes.c_longlong]
     L.shim_gate_info.restype = ctypes.c_int
     L.shim_gate_info.argtypes = [ctypes.c_char_p, ctypes.c_long, ctypes.c_int, ctypes.POINTER(ctypes.c_int),
                                  ctypes.c_int]
@@ -121,3 +125,12 @@ def test_goal_preimage_is_the_candidate_itself(shim, n):
         assert k == want_k, name
         got = [tuple(buf[4 * i:4 * i + 4]) for i in range(N * N)]
         assert got == [x for row in want for x in row], name
+
+
+@pytest.mark.parametrize("name", ["cs-d4", "toffoli-d9", "rus13-unitary-d12"])
+def test_found_level_skip_needs_every_preimage(shim, name):
+    """the search skips goal-free parents at a found level only when every candidate has a goal pre-image; goal
+    entries scaled beyond int32 leave candidates without one, which turns the skip off"""
+    spec = read_spec(name).encode()
+    assert shim.shim_goal_skip(spec, len(spec), 1) == 1
+    assert shim.shim_goal_skip(spec, len(spec), 1 << 40) == 0

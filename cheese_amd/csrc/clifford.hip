@@ -290,8 +290,37 @@ static thread_local std::string g_err;
 
 }  // namespace chz_cl
 
+namespace chz_cl {
+
+// Fixed staging for chz_cliff_query_words_noalloc: the tableaux of up to kQueryScratch words, written by k_words
+// and read by k_find.  Each GPU the library runs on holds its own copy until the process ends.  Host code never
+// uses this name as an address: scratch_address() resolves each device's copy once, with that device current.
+constexpr long long kQueryScratch = 1ll << 20;
+__device__ __align__(256) uint64_t scratch[kQueryScratch];
+
+constexpr int kMaxDevices = 64;
+static std::mutex g_scratch_mu;
+static uint64_t *g_scratch_dev[kMaxDevices];
+
+// the device address of this device's scratch (the device must be current); 0 or 5 with g_err set
+static int scratch_address(int device, uint64_t **out) {
+  std::lock_guard<std::mutex> lk(g_scratch_mu);
+  if (device < 0 || device >= kMaxDevices) { g_err = "device ordinal out of range"; return 3; }
+  if (!g_scratch_dev[device]) {
+    void *p = nullptr;
+    const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    if (e != hipSuccess || !p) { g_err = std::string("hipGetSymbolAddress: ") + hipGetErrorString(e); return 5; }
+    g_scratch_dev[device] = (uint64_t *)p;
+  }
+  *out = g_scratch_dev[device];
+  return 0;
+}
+
+}  // namespace chz_cl
+
 struct chz_cliff {
   int n = 0, device = 0;
+  uint64_t *scratch = nullptr;         // this device's chz_cl::scratch (its device address)
   chz_cl::Gates G{};
   chz_cl::Table T{};
   long long nslots = 0;
@@ -327,6 +356,7 @@ int chz_cliff_build(int n, int device, chz_cliff **out) {
   c->nslots = 1ll << lg;
   c->T.bmask = (uint64_t)(c->nslots / 8 - 1);
   CLCHK(hipSetDevice(device));
+  if (const int rc = chz_cl::scratch_address(device, &c->scratch)) { delete c; return rc; }
   CLCHK(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking));
   CLCHK(cl_malloc((void **)&c->T.slot, c->nslots * 8));
   CLCHK(hipMemsetAsync(c->T.slot, 0, c->nslots * 8, c->stream));
@@ -494,6 +524,30 @@ int chz_cliff_query_words(chz_cliff *c, const uint8_t *letters, const long long 
   (void)cl_free(doff);
   (void)cl_free(tab);
   (void)cl_free(dd);
+  return 0;
+}
+
+// chz_cliff_query_words with device buffers and no allocation: the tableaux are staged in this device's fixed
+// scratch (at most kQueryScratch words per call); the kernels go on `stream` (NULL: the table's own stream,
+// synchronised before returning).  Calls that overlap on one GPU share the scratch and race.
+int chz_cliff_query_words_noalloc(chz_cliff *c, const uint8_t *d_letters, const long long *d_offs, long long nq,
+                                  int *d_dist, void *stream) {
+  if (nq < 0 || (nq > 0 && (!d_letters || !d_offs || !d_dist))) { chz_cl::g_err = "bad arguments"; return 3; }
+  if (nq > chz_cl::kQueryScratch) {
+    chz_cl::g_err = "chz_cliff_query_words_noalloc: " + std::to_string(nq) + " words exceed the scratch capacity of " +
+                    std::to_string(chz_cl::kQueryScratch);
+    return 4;
+  }
+  if (nq == 0) return 0;
+  chz::RelaxedCapture relaxed;
+  CLCHK(hipSetDevice(c->device));
+  hipStream_t st = stream ? (hipStream_t)stream : c->stream;
+  hipLaunchKernelGGL(chz_cl::k_words, dim3((unsigned)std::min<long long>((nq + 255) / 256 + 1, 4096)), dim3(256), 0,
+                     st, c->G, d_letters, d_offs, nq, c->scratch);
+  hipLaunchKernelGGL(chz_cl::k_find, dim3((unsigned)std::min<long long>((nq + 31) / 32 + 1, 8192)), dim3(256), 0,
+                     st, c->T, c->scratch, nq, d_dist);
+  CLCHK(hipGetLastError());
+  if (!stream) CLCHK(hipStreamSynchronize(st));
   return 0;
 }
 

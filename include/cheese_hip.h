@@ -152,6 +152,14 @@ int chz_cliff_digest(const chz_cliff *c, uint64_t *digest, long long *entries);
 int chz_cliff_query_words(chz_cliff *c, const uint8_t *letters, const long long *offs, long long nq, int *dist,
                           double *ms);
 uint64_t chz_cliff_word_tableau(const chz_cliff *c, const uint8_t *letters, int len);
+/* No counterpart in the reference: for callers that cannot allocate device memory (code under HIP graph capture,
+ * tight serving loops, harnesses that forbid hipMalloc).  chz_cliff_query_words with DEVICE buffers d_letters,
+ * d_offs (nq + 1 offsets) and d_dist, calling no hipMalloc/hipFree: the words' tableaux are staged in a fixed
+ * per-device scratch array of 2^20 entries (8 MiB, held until the process ends), so nq is at most 2^20 (else
+ * 4).  Kernels are enqueued on stream (a hipStream_t; NULL: the table's own stream, synchronised before the
+ * call returns).  Calls that overlap on one GPU share the scratch and race.  0, 3, 4, 5. */
+int chz_cliff_query_words_noalloc(chz_cliff *c, const uint8_t *d_letters, const long long *d_offs, long long nq,
+                                  int *d_dist, void *stream);
 /* optimal words of tableaux by walking distances down, first gate in gate order (src/clifford.cu:354-372);
  * words: nq x maxlen letters, lens[i] = length or -1 */
 int chz_cliff_readback(chz_cliff *c, const uint64_t *tab, long long nq, int maxlen, uint8_t *words, int *lens);

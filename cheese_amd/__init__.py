@@ -73,6 +73,8 @@ SIGNATURES = {
     "chz_batch_events": (ctypes.c_int, [ctypes.c_void_p, ctypes.POINTER(Event), ctypes.c_int]),
     "chz_batch_live": (ctypes.c_int, [ctypes.c_void_p]),
     "chz_batch_nodes": (ctypes.c_longlong, [ctypes.c_void_p]),
+    "chz_batch_set_budget": (None, [ctypes.c_void_p, ctypes.c_double]),
+    "chz_batch_waits": (ctypes.c_longlong, [ctypes.c_void_p]),
 }
 
 
@@ -162,9 +164,9 @@ class Searcher:
     the stream and the tree pool persist between runs, as in a long-running caller"""
 
     def __init__(self, spec_text: str, depth: int | None = None, dedupe: bool = True, sym: bool = True,
-                 chunk: int = 0, device: int = 0):
+                 chunk: int = 0, device: int = 0, table: int = 0, mitm: bool = False):
         L = lib()
-        o = Opts(-1 if depth is None else depth, int(dedupe), int(sym), chunk, 0, device, 0, 0)
+        o = Opts(-1 if depth is None else depth, int(dedupe), int(sym), chunk, 0, device, table, int(mitm))
         self._ctx = ctypes.c_void_p()
         raw = spec_text.encode()
         rc = L.chz_open(raw, len(raw), ctypes.byref(o), ctypes.byref(self._ctx))
@@ -191,7 +193,13 @@ class Searcher:
         res = Result()
         rc = L.chz_run(self._ctx, cb, None, ctypes.byref(res))
         out = {"rc": rc, "levels": levels, "found": res.found, "seconds": res.seconds,
-               "circuit_idx": [res.circuit[i] for i in range(res.len)] if rc == 0 else None}
+               "circuit_idx": [res.circuit[i] for i in range(res.len)] if rc == 0 else None,
+               "split": (("table", "mitm")[res.split_kind - 1], [res.split[0], res.split[1]]) if res.split_kind else None}
+        if rc == 0:
+            idx = out["circuit_idx"]
+            out["circuit"] = [L.chz_gate_name(self._ctx, g).decode() for g in idx]
+            arr = (ctypes.c_int * max(1, len(idx)))(*idx)
+            out["verified"] = L.chz_verify(self._ctx, arr, len(idx)) == 1
         if stats:
             ks = (KStat * 16)()
             nk = L.chz_kernel_stats(self._ctx, ks, 16)
@@ -287,6 +295,14 @@ class Batch:
     def nodes(self) -> int:
         return lib().chz_batch_nodes(self._ctx)
 
+    def set_budget(self, nbytes: float):
+        """per-level memory planning: searches that would not fit wait a step (chz_batch_set_budget)"""
+        lib().chz_batch_set_budget(self._ctx, float(nbytes))
+
+    @property
+    def waits(self) -> int:
+        return lib().chz_batch_waits(self._ctx)
+
     def close(self):
         if getattr(self, "_ctx", None):
             lib().chz_close(self._ctx)
@@ -299,9 +315,12 @@ class Batch:
             pass
 
 
-def search_many(specs, device: int = 0):
-    """all searches in one batch (they must share a gate set); returns per spec the dict search() returns"""
+def search_many(specs, device: int = 0, budget: float = 0, stats: dict | None = None):
+    """all searches in one batch (they must share a gate set); returns per spec the dict search() returns.  budget:
+    per-level memory planning in bytes (0: none); stats (a dict) receives the number of search-levels that waited"""
     b = Batch(specs[0], device)
+    if budget:
+        b.set_budget(budget)
     outs = [{"rc": None, "levels": [], "circuit": None} for _ in specs]
     try:
         for i, sp in enumerate(specs):
@@ -322,6 +341,8 @@ def search_many(specs, device: int = 0):
             rc = b.step()
             if rc:
                 raise SearchError(rc, lib().chz_last_error().decode())
+        if stats is not None:
+            stats["waits"] = b.waits
         return outs
     finally:
         b.close()

@@ -712,6 +712,7 @@ struct BInst {
   int tag = 0, depth = 0, t = 0, slot = 0, root_step = -1;
   bool live = false;
   long long nodes = 0;
+  double growth = 0;           // the last level's distinct / parents (0: not known yet)
   double t0 = 0;
 };
 struct Pending {               // a search waiting to join the frontier as a root
@@ -786,6 +787,11 @@ struct chz_ctx {
   double k_stage[5] = {};       // expansion stage counts: residuals, structural bounds, cut ranks, T bounds, keys
   double avg_terms = 1;         // nonzero terms per inverse column, averaged over the candidates
   bool goal_skip = false;       // every candidate has a goal pre-image: the found-level skip is exact
+  // per-level memory planning (chz_batch_set_budget): searches in frontier order; with a budget, a level expands
+  // the longest prefix of them whose planned bytes fit, and the rest wait, carried to the next frontier unexpanded
+  std::vector<int> forder;
+  double mem_budget = 0;
+  long long carried_levels = 0;   // search-levels that waited (measurement)
   // --table K: the distance table (built on first use, kept with the context); a context that builds one has
   // gt_filter set (its levels drop classes the table holds and are appended to it) and no goal test
   chz::GTable *gt = nullptr;
@@ -853,6 +859,15 @@ static double expand_ops(const chz_ctx *c, int nq, const unsigned long long *hc)
 
 
 constexpr uint32_t NO_PARENT = 0xffffffffu;
+constexpr uint8_t CARRY_GATE = 0xfe;      // tree entry of a node carried unexpanded to the next level
+
+// tree entries of n carried nodes: parent = their index in the level before (first_old + j)
+__global__ void k_carry_tree(uint32_t *tp, uint8_t *tg, long long n, long long first_old) {
+  for (long long j = (long long)blockIdx.x * blockDim.x + threadIdx.x; j < n; j += (long long)gridDim.x * blockDim.x) {
+    tp[j] = (uint32_t)(first_old + j);
+    tg[j] = CARRY_GATE;
+  }
+}
 
 // stable sort of (key prefix, rank) pairs of n survivors; the result lands in sk[0]/sv[0]
 static void sort_pairs(chz_ctx *c, long long n) {
@@ -933,7 +948,10 @@ template <int NQ> static void append_roots(chz_ctx *c, int nxt, long long distin
   HIPCHK(hipMemcpyAsync(tp + distinct, np.data(), nr * 4, hipMemcpyHostToDevice, c->stream));
   HIPCHK(hipMemcpyAsync(tg + distinct, none.data(), nr, hipMemcpyHostToDevice, c->stream));
   HIPCHK(hipStreamSynchronize(c->stream));
-  for (long long i = 0; i < nr; i++) c->binst[c->pending[i].inst].root_step = c->gstep + 1;
+  for (long long i = 0; i < nr; i++) {
+    c->binst[c->pending[i].inst].root_step = c->gstep + 1;
+    c->forder.push_back(c->pending[i].inst);
+  }
   c->pending.clear();
 }
 
@@ -948,7 +966,7 @@ static void walk_circuit(chz_ctx *c, unsigned long long o, int G, std::vector<in
     chz::copy_sync(&par, c->tree_par[s] + i, 4, hipMemcpyDeviceToHost, c->stream);
     if (par == NO_PARENT) break;
     chz::copy_sync(&g, c->tree_gate[s] + i, 1, hipMemcpyDeviceToHost, c->stream);
-    seq.push_back(g);
+    if (g != CARRY_GATE) seq.push_back(g);     // a level the search waited through
     i = par;
   }
   std::reverse(seq.begin(), seq.end());
@@ -1114,6 +1132,29 @@ template <int NQ> static void batch_step(chz_ctx *c) {
   const double tl = now_s();
   const int cur = c->cur;
   const long long m = c->m;
+  // per-level memory planning: the searches expanded this step are the longest prefix of the frontier (whole
+  // searches, in frontier order, at least one) whose planned bytes fit the budget beside the frontier held: per
+  // parent its children's records and its share of the next frontier, by the search's last growth (a prior of
+  // kGrowthPrior before its first level); the others wait a step, carried unexpanded
+  long long ma = m;
+  std::vector<char> carried(c->binst.size(), 0);
+  if (c->mem_budget > 0 && m > 0) {
+    constexpr double node_b = E * 16.0 + 24.0, child_b = 48.0, kGrowthPrior = 8.0;
+    const double held = (double)m * node_b;
+    double plan = 0;
+    long long acc = 0;
+    bool stop = false;
+    for (int bi : c->forder) {
+      const BInst &b = c->binst[bi];
+      if (!b.live) continue;
+      const double g = std::min<double>(G, b.growth > 0 ? b.growth : kGrowthPrior);
+      const double need = (double)b.nodes * (G * child_b + g * node_b);
+      if (acc > 0 && (stop || held + plan + need > c->mem_budget)) { stop = true; carried[bi] = 1; continue; }
+      plan += need;
+      acc += b.nodes;
+    }
+    ma = acc;
+  }
   // the level's control block staged in pinned memory with the device block's layout, uploaded in one copy:
   // [counters in | per-slot counters in | rem | keepi], then [counters out | per-slot counters out] (read back
   // in one copy), the kept total and the per-slot active flags
@@ -1145,7 +1186,7 @@ template <int NQ> static void batch_step(chz_ctx *c) {
   // small levels run as one captured graph (expansion through kept compaction) plus the gather: one
   // synchronisation per level instead of two, and one launch instead of about twenty
   const bool graphed = kStepGraphs && !c->tuning && !c->gt_filter && c->opts.dedupe && m > 0 && m <= STEP_CAP &&
-                       m <= chunk;
+                       m <= chunk && ma == m;
   if (graphed) {
     for (int i = 0; i < C_N; i++) hc_in[i] = 0;
     hc_in[C_M] = (unsigned long long)m;
@@ -1162,17 +1203,17 @@ template <int NQ> static void batch_step(chz_ctx *c) {
         active[b.slot] = !x[IC_FOUND] && !x[IC_OVF] && keepi[b.slot];
       }
   } else {
-  const long long maxcc = std::min<long long>(chunk, m) * G;
+  const long long maxcc = std::min<long long>(chunk, std::max<long long>(ma, 1)) * G;
   c->status.need(maxcc);
   c->info.need(maxcc);
   c->keys.need(maxcc);
   c->bcount.need((maxcc + CT_TILE - 1) / CT_TILE + 1);
   // survivors of every chunk append at the device-side count C_SURV: the buffers hold every child of the level
-  c->surv_o.need((size_t)m * G);
-  c->surv_key.need((size_t)m * G);
-  c->surv_bw.need((size_t)m * G);
-  for (long long off = 0; off < m; off += chunk) {
-    const long long cm = std::min(chunk, m - off), ncc = cm * G;
+  c->surv_o.need((size_t)std::max<long long>(ma, 1) * G);
+  c->surv_key.need((size_t)std::max<long long>(ma, 1) * G);
+  c->surv_bw.need((size_t)std::max<long long>(ma, 1) * G);
+  for (long long off = 0; off < ma; off += chunk) {
+    const long long cm = std::min(chunk, ma - off), ncc = cm * G;
     // the schedule of this chunk's size bucket: few parents split each parent's candidates over several lane
     // groups so the level fills the GPU
     const int bucket = std::min(31, 63 - __builtin_clzll((unsigned long long)cm));
@@ -1292,7 +1333,8 @@ template <int NQ> static void batch_step(chz_ctx *c) {
   // next frontier: kept children, then the searches admitted since the last step.  The goal flags describe the
   // frontier being expanded until here; the gather sets them for the next one.
   if (!graphed && !gflag_clear) HIPCHK(hipMemsetAsync(c->gflag.p, 0, (size_t)ns * 4, c->stream));
-  const long long mn = distinct + (long long)c->pending.size();
+  const long long ncarry = m - ma;            // nodes of the searches that waited
+  const long long mn = distinct + ncarry + (long long)c->pending.size();
   std::vector<long long> cnts(c->nslots, 0);
   if (mn) {
     if (!graphed) {
@@ -1315,7 +1357,24 @@ template <int NQ> static void batch_step(chz_ctx *c) {
     const bool single = c->pending.empty() && chz_batch_live(c) == 1;
     if (c->gt_filter && distinct)   // a table build keeps every level, resident, with its classes' keys
       gt_add_level<NQ>(ks, *c->gt_filter, c->res[nxt].p, c->pk[nxt].p, tp, tg, distinct, c->stream);
-    append_roots<NQ>(c, nxt, distinct, tp, tg);
+    if (ncarry) {                   // the waiting searches' nodes follow the gathered ones, unchanged
+      HIPCHK(hipMemcpyAsync(c->res[nxt].p + (size_t)distinct * E, c->res[cur].p + (size_t)ma * E,
+                            (size_t)ncarry * E * sizeof(int4), hipMemcpyDeviceToDevice, c->stream));
+      HIPCHK(hipMemcpyAsync(c->pk[nxt].p + distinct, c->pk[cur].p + ma, (size_t)ncarry * 4, hipMemcpyDeviceToDevice,
+                            c->stream));
+      HIPCHK(hipMemcpyAsync(c->g1[nxt].p + distinct, c->g1[cur].p + ma, (size_t)ncarry, hipMemcpyDeviceToDevice,
+                            c->stream));
+      HIPCHK(hipMemcpyAsync(c->g2[nxt].p + distinct, c->g2[cur].p + ma, (size_t)ncarry, hipMemcpyDeviceToDevice,
+                            c->stream));
+      HIPCHK(hipMemcpyAsync(c->pb[nxt].p + distinct, c->pb[cur].p + ma, (size_t)ncarry * 4, hipMemcpyDeviceToDevice,
+                            c->stream));
+      HIPCHK(hipMemcpyAsync(c->finst[nxt].p + distinct, c->finst[cur].p + ma, (size_t)ncarry * 4,
+                            hipMemcpyDeviceToDevice, c->stream));
+      hipLaunchKernelGGL(k_carry_tree, dim3(grid_for(ncarry / 64 + 1)), dim3(256), 0, c->stream, tp + distinct,
+                         tg + distinct, ncarry, ma);
+      for (size_t bi = 0; bi < carried.size(); bi++) c->carried_levels += carried[bi];
+    }
+    append_roots<NQ>(c, nxt, distinct + ncarry, tp, tg);
     c->tree_par.push_back(tp);
     c->tree_gate.push_back(tg);
     if (single) {   // one search: its nodes are the whole frontier (its kernel times are read next step)
@@ -1334,6 +1393,7 @@ template <int NQ> static void batch_step(chz_ctx *c) {
   for (size_t bi = 0; bi < c->binst.size(); bi++) {
     BInst &b = c->binst[bi];
     if (!b.live || b.root_step > c->gstep) continue;      // admitted in this step: it has not expanded yet
+    if (bi < carried.size() && carried[bi]) continue;      // waited this step: no level
     const unsigned long long *x = &ic[(size_t)b.slot * IC_N];
     chz_event evn{};
     evn.tag = b.tag;
@@ -1367,6 +1427,7 @@ template <int NQ> static void batch_step(chz_ctx *c) {
       lv.level.seconds = now - tl;
       c->events.push_back(lv);
       b.t++;
+      if (b.nodes > 0) b.growth = (double)dist / (double)b.nodes;
       b.nodes = dist;
       if (dist) continue;
       evn.kind = 1;
@@ -1382,6 +1443,8 @@ template <int NQ> static void batch_step(chz_ctx *c) {
   c->cur = nxt;
   c->m = mn;
   c->gstep++;
+  c->forder.erase(std::remove_if(c->forder.begin(), c->forder.end(), [&](int bi) { return !c->binst[bi].live; }),
+                  c->forder.end());
   prune_tree(c);
 }
 
@@ -1739,6 +1802,10 @@ int chz_batch_step(chz_ctx *c) {
     return 5;
   }
 }
+
+void chz_batch_set_budget(chz_ctx *c, double bytes) { c->mem_budget = bytes > 0 ? bytes : 0; }
+
+long long chz_batch_waits(const chz_ctx *c) { return c->carried_levels; }
 
 int chz_batch_events(chz_ctx *c, chz_event *out, int max) {
   int k = 0;

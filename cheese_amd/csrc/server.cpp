@@ -103,27 +103,27 @@ chz_response base_resp(chz_server *s, const Req &q) {
   return r;
 }
 
-// bytes a gate-count search holds per frontier node and per child while a level is expanded
-constexpr double kNodeBytes[2] = {16.0 * 16 + 16, 16.0 * 64 + 16};
+// Memory: each gate-count batch gets a share of the budget (kShare), enforced level by level by the batch's own
+// planning (chz_batch_set_budget: searches that would not fit wait a step, carried unexpanded).  Admission keeps
+// the frontier a new search joins within half of that share: per frontier node its residual and the children a
+// level makes (kNodeBytes, kChildBytes), a root charged for its first level of G children, at most kAdmitPerStep
+// joining one step.  kMaxLive is a throughput setting, not a memory bound: the 2,000-request burst is served
+// fastest with about 64 live 3-qubit searches per batch.
+constexpr double kNodeBytes[2] = {16.0 * 16 + 24, 16.0 * 64 + 24};
 constexpr double kChildBytes = 48.0;
-// admission plans for a next frontier up to kGrowth times the current one (the level's gather holds both), lets
-// at most kAdmitPerStep searches join one step (so growth is seen before more join), and admits nothing while
-// the device has less than kFreeReserve bytes free
-constexpr double kGrowth = 4.0;
+constexpr double kShare[2] = {0.2, 0.8};
 constexpr int kAdmitPerStep = 64;
-constexpr double kFreeReserve = 24e9;
-constexpr long long kHbmBytes = 288ll << 30;
-// Searches already admitted keep growing for several levels, which no admission-time estimate sees; without a
-// caller's max_batch, a batch holds at most this many live searches (2 and 3 qubits)
 constexpr int kMaxLive[2] = {1024, 64};
 
-double per_node_bytes(int qi, int G) { return kNodeBytes[qi] * (1.0 + kGrowth) + G * kChildBytes; }
+double per_node_bytes(int qi, int G) { return kNodeBytes[qi] + G * kChildBytes; }
 
 // ---- gate-count searches: one batch per qubit count, stepped level by level
 void gate_worker(chz_server *s, int qi) {
   chz::RelaxedCapture relaxed;    // this thread's calls never invalidate another engine's capture
   chz_ctx *ctx = s->gctx[qi];
   const int G = chz_num_gates(ctx);
+  const double share = s->o.mem_budget_gb * 1e9 * kShare[qi];
+  chz_batch_set_budget(ctx, share);
   std::map<int, Req> live;      // tag -> request
   int next_tag = 0;
   std::vector<chz_event> ev(256);
@@ -133,18 +133,14 @@ void gate_worker(chz_server *s, int qi) {
       if (live.empty())
         s->cv.wait_for(lk, std::chrono::milliseconds(50), [&] { return s->stop || !s->q_gates[qi].empty(); });
       if (s->stop && live.empty() && s->q_gates[qi].empty()) return;
-      // admission: queued searches join while the batch's frontier memory estimate stays in budget; a search
-      // joins as one root and is charged for its first two levels of children, and every admitted search counts
-      // against the budget at once
-      const double budget = s->o.mem_budget_gb * 1e9;
+      // admission: queued searches join while the frontier the batch holds, with every search admitted in this
+      // round counted at once, stays within half the batch's share
       const double per_node = per_node_bytes(qi, G);
       double held = (double)chz_batch_nodes(ctx) * per_node;
-      const double want = (double)G * G * per_node;
-      const long long used = device_used_bytes();
-      const bool low_mem = used > 0 && (double)(kHbmBytes - used) < kFreeReserve;
+      const double want = (double)G * per_node;
       int admitted = 0;
       while (!s->q_gates[qi].empty()) {
-        if (!live.empty() && (held + want > budget || admitted >= kAdmitPerStep || low_mem)) break;
+        if (!live.empty() && (held + want > 0.5 * share || admitted >= kAdmitPerStep)) break;
         if ((int)live.size() >= (s->o.max_batch > 0 ? s->o.max_batch : kMaxLive[qi])) break;
         held += want;
         admitted++;
@@ -186,6 +182,7 @@ void gate_worker(chz_server *s, int qi) {
         chz_ctx *fresh = nullptr;
         if (chz_open(s->base[qi].data(), s->base[qi].size(), &co, &fresh) == 0) {
           ctx = s->gctx[qi] = fresh;
+          chz_batch_set_budget(ctx, share);
           continue;
         }
         // no engine left for this qubit count: queued and later requests are answered with the error at once

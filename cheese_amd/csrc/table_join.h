@@ -23,7 +23,7 @@ static std::vector<int> walk_node(chz_ctx *c, uint32_t j) {
     copy_sync(&par, c->tree_par[s] + i, 4, hipMemcpyDeviceToHost, c->stream);
     if (par == NO_PARENT) break;
     copy_sync(&g, c->tree_gate[s] + i, 1, hipMemcpyDeviceToHost, c->stream);
-    seq.push_back(g);
+    if (g != CARRY_GATE) seq.push_back(g);
     i = par;
   }
   std::reverse(seq.begin(), seq.end());
@@ -52,6 +52,7 @@ static void batch_clear(chz_ctx *c) {
   for (BInst &b : c->binst) b.live = false;
   c->pending.clear();
   c->events.clear();
+  c->forder.clear();
   c->m = 0;
   prune_tree(c);
 }

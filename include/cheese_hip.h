@@ -131,6 +131,12 @@ int chz_batch_step(chz_ctx *ctx);
 int chz_batch_events(chz_ctx *ctx, chz_event *out, int max);
 int chz_batch_live(const chz_ctx *ctx);         /* searches not finished */
 long long chz_batch_nodes(const chz_ctx *ctx);  /* frontier nodes held on the device, queued roots included */
+/* per-level memory planning (no reference counterpart): with bytes > 0, each step expands the longest prefix of
+ * the batch's searches (in admission order, at least one) whose planned bytes - children's records and the next
+ * frontier by each search's last growth - fit beside the frontier held; the other searches wait, their nodes
+ * carried to the next level unexpanded (their level lines and results are unchanged, only later).  0: no limit. */
+void chz_batch_set_budget(chz_ctx *ctx, double bytes);
+long long chz_batch_waits(const chz_ctx *ctx);  /* search-levels that waited so far */
 
 /* ---------------- Clifford-group table (src/clifford.cu) ----------------
  * The whole n-qubit Clifford group (n <= 3, up to global phase) tabulated by breadth-first search over the gate

@@ -126,3 +126,25 @@ def test_autotuned_schedule_keeps_the_log():
         assert strip_seconds(_levels_text(out["levels"])) == ref
     finally:
         s.close()
+
+
+@pytest.mark.parametrize("budget", [2e6, 4e7])
+def test_memory_planning_makes_searches_wait_without_changing_them(budget):
+    """a batch whose per-level memory budget is far below what its levels would take: searches wait (their nodes
+    carried unexpanded) and every log and circuit is still the reference's"""
+    names = [n for n in golden_cases(max_seconds=10)
+             if manifest()[n]["n"] == 3 and not manifest()[n]["flags"] and not n.startswith(("rus13", "t-ladder"))]
+    specs = [read_spec(n) for n in names]
+    st = {}
+    outs = cheese_amd.search_many(specs, budget=budget, stats=st)
+    print(f"budget {budget:.0e}: {st['waits']} search-levels waited", flush=True)
+    assert st["waits"] > 0
+    for name, o in zip(names, outs):
+        if o["rc"] == 3:
+            continue
+        m = manifest()[name]
+        assert o["rc"] == m["rc"], name
+        lines = [cheese_amd.level_line(l) for l in o["levels"]]
+        if o["rc"] == 0:
+            lines.append(cheese_amd.result_line(o))
+        assert strip_seconds("\n".join(lines)) == strip_seconds(read_log(name)), name

@@ -5,7 +5,7 @@ at gate budget 12, an exhaustive proof that no circuit of <= 12 gates exists (ex
 checked against the reference CPU build's golden log on every step.  value = prefixes/s (sum of the
 per-level "prefixes" / wall time of a step); ms_per_step is the time to proof.
 
-  python bench.py [--gpus 1] [--steps K] [--warmup W] [--cpu-runs R] [--no-cpu] [--serve N] [--table K]
+  python bench.py [--gpus 1] [--steps K] [--warmup W] [--cpu-runs R] [--no-cpu] [--serve N] [--no-table]
 """
 import argparse
 import json
@@ -120,6 +120,55 @@ def toffoli_proof(depth, runs=5):
         s.close()
     ms = sorted(times)[len(times) // 2] * 1e3
     return {"ms_per_proof": round(ms, 3), "prefixes": prefixes, "prefixes_per_s": round(prefixes / (ms / 1e3), 1)}
+
+
+def gate_table(ks=(7, 8), runs=3):
+    """--table K on the Toffoli target (src/cheese.cu:1431-1593, :1804-1842): per K one context whose first run
+    builds the distance table (every symmetry class of <= K gates) and proves budget 14 impossible, then the proof
+    again with the table kept (median of runs); with the largest K the budget-15 search finds the optimal 15-gate
+    circuit, verified exactly.  The reference quotes K=7 2.65M classes, K=8 17.9M, and with --table 8 and its table
+    cached budget 14 in 1.5 s and budget 15 in 11.1 s (docs/PLAN.md:87-92)."""
+    import cheese_amd
+    with open(os.path.join(ROOT, "tests", "golden", "specs", "toffoli-d14.spec")) as f:
+        spec14 = f.read()
+    spec15 = spec14.replace("\ndepth 14\n", "\ndepth 15\n")
+    assert spec15 != spec14
+    out = {}
+    for K in ks:
+        s = cheese_amd.Searcher(spec14, table=K)
+        try:
+            t = time.perf_counter()
+            first = s.run()
+            first_s = time.perf_counter() - t
+            tlv = [l for l in first["levels"] if l["dir"] == 1]
+            times = []
+            for _ in range(runs):
+                t = time.perf_counter()
+                r = s.run()
+                times.append(time.perf_counter() - t)
+                if r["rc"] != 1 or first["rc"] != 1:
+                    print(f"bench.py: Toffoli budget 14 with --table {K} did not prove impossibility", file=sys.stderr)
+                    sys.exit(1)
+        finally:
+            s.close()
+        out[f"k{K}"] = {"classes": 1 + sum(l["distinct"] for l in tlv), "first_run_s": round(first_s, 3),
+                        "budget14_proof_ms": round(sorted(times)[len(times) // 2] * 1e3, 2)}
+    K = max(ks)
+    s = cheese_amd.Searcher(spec15, table=K)
+    try:
+        s.run()
+        t = time.perf_counter()
+        r = s.run()
+        ms = (time.perf_counter() - t) * 1e3
+    finally:
+        s.close()
+    if r["rc"] != 0 or not r["verified"] or len(r["circuit"]) != 15:
+        print(f"bench.py: Toffoli budget 15 with --table {K}: no verified 15-gate circuit", file=sys.stderr)
+        sys.exit(1)
+    out[f"k{K}"]["budget15_found_ms"] = round(ms, 2)
+    out[f"k{K}"]["budget15_circuit"] = r["circuit"]
+    out["reference_quoted_s"] = {"k7_build": 3.6, "k8_build": 30, "budget14_k8": 1.5, "budget15_k8": 11.1}
+    return out
 
 
 def random_circuits(n=32):
@@ -272,6 +321,8 @@ def main():
                     help="also build the 3-qubit T-count table and search 256 batched targets")
     ap.add_argument("--d9", action=argparse.BooleanOptionalAction, default=True,
                     help="also time the Toffoli budget-9 and budget-14 proofs")
+    ap.add_argument("--table", action=argparse.BooleanOptionalAction, default=True,
+                    help="also time --table 7 and --table 8 on the Toffoli budget-14 proof and the budget-15 circuit")
     ap.add_argument("--tune", action=argparse.BooleanOptionalAction, default=True,
                     help="autotune the expansion schedule on the workload before the timed proofs")
     ap.add_argument("--serve", type=int, default=2000, metavar="N",
@@ -350,6 +401,8 @@ def main():
         line["toffoli_d9"] = toffoli_proof(9)
         if os.path.exists(os.path.join(ROOT, "tests", "golden", "logs", "toffoli-d14.log")):
             line["toffoli_d14"] = toffoli_proof(14, runs=3)
+    if a.table:
+        line["gate_table"] = gate_table()
     if a.random:
         line["random32"] = random_circuits()
     if a.clifford or a.tcount or a.serve:

@@ -105,6 +105,19 @@ def test_table_size_within_seven_gates():
     assert out["rc"] == 1
 
 
+def test_toffoli_optimum_fifteen_with_table_eight():
+    """docs/PLAN.md:87-91: K = 8 holds 17.9 M classes; with --table 8 budget 14 is proved impossible and budget 15
+    finds a circuit, so Toffoli's optimal gate count is 15; the circuit is checked exactly against the target"""
+    spec14 = read_spec("toffoli-d14")
+    out = cheese_amd.search(spec14, table=8)
+    total = 1 + sum(l["distinct"] for l in out["levels"] if l["dir"] == 1)
+    print(f"table K=8: {total} classes, budget 14 rc {out['rc']}", flush=True)
+    assert abs(total - 17.9e6) < 0.05e6
+    assert out["rc"] == 1 and out["split"] == ("table", [-1, 8])
+    out = cheese_amd.search(spec14.replace("\ndepth 14\n", "\ndepth 15\n"), table=8)
+    assert out["rc"] == 0 and out["verified"] and len(out["circuit"]) == 15
+
+
 @pytest.mark.parametrize("seed", [0, 3, 5, 9, 17, 21])
 @pytest.mark.parametrize("mode", ["table", "mitm"])
 def test_table_and_join_find_circuits_of_the_plain_length(seed, mode):

@@ -299,6 +299,12 @@ class Batch:
         """per-level memory planning: searches that would not fit wait a step (chz_batch_set_budget)"""
         lib().chz_batch_set_budget(self._ctx, float(nbytes))
 
+    def kernel_stats(self) -> dict:
+        """chz_kernel_stats of the batch's context: launches, milliseconds and work per kernel family"""
+        ks = (KStat * 16)()
+        nk = lib().chz_kernel_stats(self._ctx, ks, 16)
+        return {ks[i].name.decode(): {"launches": ks[i].launches, "ms": ks[i].ms, "work": ks[i].work} for i in range(nk)}
+
     @property
     def waits(self) -> int:
         return lib().chz_batch_waits(self._ctx)
@@ -317,7 +323,8 @@ class Batch:
 
 def search_many(specs, device: int = 0, budget: float = 0, stats: dict | None = None):
     """all searches in one batch (they must share a gate set); returns per spec the dict search() returns.  budget:
-    per-level memory planning in bytes (0: none); stats (a dict) receives the number of search-levels that waited"""
+    per-level memory planning in bytes (0: none); stats (a dict) receives the number of search-levels that waited
+    and the batch's kernel statistics"""
     b = Batch(specs[0], device)
     if budget:
         b.set_budget(budget)
@@ -342,6 +349,7 @@ def search_many(specs, device: int = 0, budget: float = 0, stats: dict | None = 
             if rc:
                 raise SearchError(rc, lib().chz_last_error().decode())
         if stats is not None:
+            stats["kernel_stats"] = b.kernel_stats()
             stats["waits"] = b.waits
         return outs
     finally:

@@ -24,15 +24,20 @@ struct DevGate {                               // S <- S g^-1 data of one candid
   int8_t ph[64];                               // unit gates: w-power p of inv[m][lc], -1 for zero
 };
 
+constexpr int CXL_MAX = 128;                   // largest cx table held in the expansion kernel's LDS
+
 struct KSpec {                                 // device view of a spec (kernel argument)
   int n, G, depth, standard, lite, tdiv, gk, nsym, ncuts, have_table;
   int cx_mono;                                 // cx table non-decreasing in every cut need
+  int cxlds;                                   // the cx table fits the expansion kernel's LDS copy (CXL_MAX bytes)
+  int ncx;                                     // cx table entries
   int cutmask[16], cutsize[16], rad[16];
   uint32_t wm[4];                              // Montgomery constants w^i * 2^32 mod p, i = 0..3
   const DevGate *gates;
   const uint8_t *bad2;                         // G x G
   const uint32_t *bad3;                        // bits over G^3
   const int4 *goal;                            // N x N
+  uint64_t goalnz;                             // nonzero entries of the goal (bit r * N + c)
   const int *cxtable;
   const uint8_t *symperm;                      // nsym x N basis maps
   const uint8_t *symconj;                      // nsym

@@ -80,6 +80,9 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(WPE))) void
   __shared__ uint16_t sdesc[MAXG * 64];
   __shared__ GateHdr shdr[MAXG];
   __shared__ uint8_t spos[NQ == 3 ? 16 * 64 : 1];   // n = 3: position of entry le under symmetry y (y * 64 + le)
+  __shared__ uint8_t scx[CXL_MAX];                   // the cx table when it fits (sp.cxlds)
+  for (int i = threadIdx.x; i < CXL_MAX; i += blockDim.x)
+    scx[i] = sp.cxlds && sp.have_table && i < (int)sp.ncx ? (uint8_t)sp.cxtable[i] : 0;
   for (int i = threadIdx.x; i < sp.G * GS; i += blockDim.x) {
     const int g = i / GS, e = i % GS, ec = e & (N - 1);
     const DevGate &d = sp.gates[g];
@@ -256,7 +259,8 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(WPE))) void
         if (gany<GS>(of)) stt = ST_OVF;
         else {
           while (k > 0 && gall<GS>(zhalf(s))) { s = zdown(s); k--; }
-          if (k == sp.gk && gall<GS>(zeq(s, sp.goal[le]))) stt = ST_FOUND;
+          // the goal's entries are read only for a child with the goal's exponent and nonzero pattern
+          if (k == sp.gk && gballot<GS>(!zzero(s)) == sp.goalnz && gall<GS>(zeq(s, sp.goal[le]))) stt = ST_FOUND;
           else {
             bool alive = true;
             if (sp.standard || sp.lite) {
@@ -278,14 +282,14 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(WPE))) void
               const bool tknown = tpar && h.tdel == 0;
               int T = tpar ? (Tp > h.tdel ? Tp - h.tdel : 0) : 0;   // exact if tknown, else a lower bound
               bool texact = tknown;
-              const int c2lo = !sp.lite && sp.cx_mono && (pbw & BW_CUTS) ? cx_lower(sp, (pbw & BW_NEEDS) >> 8, (uint32_t)h.cdel) : 0;
+              const int c2lo = !sp.lite && sp.cx_mono && (pbw & BW_CUTS) ? cx_lower(sp, scx, (pbw & BW_NEEDS) >> 8, (uint32_t)h.cdel) : 0;
               if (combine_bound(hh, (T + sp.tdiv - 1) / sp.tdiv, c2lo, supp) > rem) alive = false;
               else {
                 if (!sp.lite) {
                   uint32_t needs = (pbw & BW_NEEDS) >> 8;
                   const uint32_t keepc = (pbw & BW_CUTS) ? (uint32_t)h.loc : 0u;
                   n_cuts += sp.ncuts - __popc(keepc & ((1u << sp.ncuts) - 1));
-                  c2 = cx_bound<NQ>(s, le, sp, keepc, needs);
+                  c2 = cx_bound<NQ>(s, le, sp, scx, keepc, needs);
                   bw = (needs << 8 & ~BW_CUTS) | (sp.ncuts <= BW_MAXCUTS ? BW_CUTS : 0u) | BW_TUNK;
                 }
                 if (combine_bound(hh, (T + sp.tdiv - 1) / sp.tdiv, c2, supp) > rem) alive = false;
@@ -757,6 +761,7 @@ struct chz_ctx {
   chz::DBuf<uint32_t> finst[2];
   long long m = 0;             // nodes in the current frontier (buffer cur)
   int cur = 0, gstep = -1, nslots = 0;
+  bool shared_steps = false;      // the batch has held more than one search at once (captured steps are used)
   // level control block, uploaded in one copy at the start of every level and read back in one copy:
   // [cnt: C_N counters | icnt: per-slot IC_N counters | d_rem: per-slot budgets | d_keepi: per-slot keep flags]
   chz::DBuf<unsigned long long> ctl;
@@ -994,6 +999,9 @@ constexpr long long STEP_CAP = 1024;
 // stream under capture; HIP's hipFree synchronises the device): buffer growth waits for the context's own stream
 // only, and allocations, frees, device-wide synchronisations and memory queries take capture_mu, which the capture
 // holds.
+// Captured steps are used by batches that have held more than one search at once (the request server's engines):
+// measured on MI355X, a single search runs its small levels faster as direct launches (Toffoli budget 12 on a
+// reused context: 11.5 ms against 11.7 ms with captured steps), while in the server they lower the latency tail.
 constexpr bool kStepGraphs = true;
 constexpr int STEP_BUCKET = 10;    // the schedule of the 1,024-parent bucket drives the captured expansion
 
@@ -1185,8 +1193,8 @@ template <int NQ> static void batch_step(chz_ctx *c) {
   uint8_t *tg = nullptr;
   // small levels run as one captured graph (expansion through kept compaction) plus the gather: one
   // synchronisation per level instead of two, and one launch instead of about twenty
-  const bool graphed = kStepGraphs && !c->tuning && !c->gt_filter && c->opts.dedupe && m > 0 && m <= STEP_CAP &&
-                       m <= chunk && ma == m;
+  const bool graphed = kStepGraphs && c->shared_steps && !c->tuning && !c->gt_filter && c->opts.dedupe && m > 0 &&
+                       m <= STEP_CAP && m <= chunk && ma == m;
   if (graphed) {
     for (int i = 0; i < C_N; i++) hc_in[i] = 0;
     hc_in[C_M] = (unsigned long long)m;
@@ -1556,6 +1564,9 @@ static void upload_spec(chz_ctx *c) {
   for (int i = 0; i < k.ncuts; i++) { k.cutmask[i] = s.cutmask[i]; k.cutsize[i] = s.cutsize[i]; k.rad[i] = s.rad[i]; }
   // a cut bound from lowered needs is a lower bound only if the table never decreases when a need grows
   k.cx_mono = 1;
+  k.ncx = (int)s.cxtable.size();
+  k.cxlds = s.cxtable.size() <= (size_t)CXL_MAX;
+  for (int v : s.cxtable) if (v < 0 || v > 255) k.cxlds = 0;
   if (k.have_table) {
     size_t total = 1;
     for (int i = 0; i < k.ncuts; i++) total *= (size_t)s.rad[i];
@@ -1578,6 +1589,11 @@ static void upload_spec(chz_ctx *c) {
   uint64_t wp = 1;
   for (int i = 0; i < 4; i++) { k.wm[i] = (uint32_t)mulm(wp, R); wp = mulm(wp, w); }
   k.gates = c->d_gates; k.bad2 = c->d_bad2; k.bad3 = c->d_bad3; k.goal = c->d_goal; k.cxtable = c->d_cx;
+  k.goalnz = 0;
+  for (int i = 0; i < N * N; i++) {
+    const Zq &z = s.goal[i];
+    if (z.a || z.b || z.c || z.d) k.goalnz |= 1ull << i;
+  }
   k.symperm = c->d_symperm; k.symconj = c->d_symconj;
   k.xg = c->d_xg; k.xgm = c->d_xgm; k.xgk = c->d_xgk;
   k.maglim = (uint32_t)(0x7fffffffull / gmax);
@@ -1606,7 +1622,11 @@ static int batch_add_parsed(chz_ctx *c, const HostSpec &hs, int depth, int tag) 
   if (depth <= 0) { ev.result.rc = 1; c->events.push_back(ev); return 0; }
   // a free instance slot: not held by a live search
   std::vector<bool> used(c->nslots + c->binst.size() + 1, false);
-  for (const BInst &b : c->binst) if (b.live) used[b.slot] = true;
+  for (const BInst &b : c->binst)
+    if (b.live) {
+      used[b.slot] = true;
+      c->shared_steps = true;      // the batch now steps more than one search
+    }
   int slot = 0;
   while (used[slot]) slot++;
   try {

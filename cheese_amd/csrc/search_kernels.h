@@ -183,8 +183,14 @@ constexpr int BW_MAXCUTS = 7;
 // CX/CZ lower bound from the cut ranks and the ILP table (src/cheese.cu:585-597).  Cuts in `keepc` take the
 // need packed in `needs` (the parent's: the gate acts on one side of them, so the rank is unchanged); the
 // others are ranked here.  On return `needs` holds this matrix's packed needs.
-template <int NQ> __device__ __forceinline__ int cx_bound(int4 s, int le, const KSpec &sp, uint32_t keepc,
-                                                          uint32_t &needs) {
+// The cx table is read from the expansion kernel's LDS copy `cxl` when it fits (sp.cxlds): the global table sits
+// behind a vector load on every child's dependency chain.
+__device__ __forceinline__ int cx_at(const KSpec &sp, const uint8_t *cxl, int idx) {
+  return sp.cxlds ? (int)cxl[idx] : sp.cxtable[idx];
+}
+
+template <int NQ> __device__ __forceinline__ int cx_bound(int4 s, int le, const KSpec &sp, const uint8_t *cxl,
+                                                          uint32_t keepc, uint32_t &needs) {
   uint32_t fp = 0;
   bool have_fp = false;
   int idx = 0, mult = 1, mx = 0;
@@ -209,12 +215,12 @@ template <int NQ> __device__ __forceinline__ int cx_bound(int4 s, int le, const 
     if (need > mx) mx = need;
   }
   needs = out;
-  return sp.have_table ? sp.cxtable[idx] : mx;
+  return sp.have_table ? cx_at(sp, cxl, idx) : mx;
 }
 
 // Lower bound on a child's cut bound from its parent's packed needs: each need drops by at most the gate's
 // per-cut delta (operator-Schmidt ranks multiply), and the table is non-decreasing in every need.
-__device__ __forceinline__ int cx_lower(const KSpec &sp, uint32_t pneeds, uint32_t cdel) {
+__device__ __forceinline__ int cx_lower(const KSpec &sp, const uint8_t *cxl, uint32_t pneeds, uint32_t cdel) {
   int idx = 0, mult = 1, mx = 0;
   for (int i = 0; i < sp.ncuts; i++) {
     const int dl = (int)((cdel >> (3 * i)) & 7u);
@@ -224,7 +230,7 @@ __device__ __forceinline__ int cx_lower(const KSpec &sp, uint32_t pneeds, uint32
     mult *= sp.rad[i];
     if (nd > mx) mx = nd;
   }
-  return sp.have_table ? sp.cxtable[idx] : mx;
+  return sp.have_table ? cx_at(sp, cxl, idx) : mx;
 }
 
 // T lower bound: max over Pauli pairs of 2(n+k) - v(X_ab), all X_ab by one 2-D Walsh-Hadamard transform
@@ -308,8 +314,9 @@ template <int NQ> __device__ __forceinline__ int t_bound16(TStage st, int4 s, in
   asm volatile("" : "+v"(f));
   const int j = 31 - __clz(f);
   const int lowm = (1 << j) - 1;
-  // pass 0 transforms (Qa | Qc), pass 1 (Qb - Qd | Qb + Qd); one pass at a time keeps H packed values live
-  uint32_t acc[2];
+  // pass 0 transforms (Qa | Qc), pass 1 (Qb - Qd | Qb + Qd); one pass at a time keeps H packed values live.
+  // The two results are kept in named registers: an array indexed by the pass would live in scratch memory.
+  uint32_t ev = 0, od = 0;
 #pragma unroll 1
   for (int pass = 0; pass < 2; pass++) {
     const uint4 *pm = st.p + pass * TS_STRIDE;
@@ -338,9 +345,10 @@ template <int NQ> __device__ __forceinline__ int t_bound16(TStage st, int4 s, in
     uint32_t o = 0;
 #pragma unroll
     for (int i = 0; i < H; i++) o |= bits(q[i]);
-    acc[pass] = (o & 0xffffu) | (o >> 16);
+    const uint32_t folded = (o & 0xffffu) | (o >> 16);
+    if (pass == 0) ev = folded;                // a, c (even part)
+    else od = folded;                          // b - d, b + d (odd part)
   }
-  const uint32_t ev = acc[0], od = acc[1];     // ev: a, c (even part); od: b - d, b + d (odd part)
   // f = 0: |S[e]|^2 = r + y sqrt2 with r = a^2 + b^2 + c^2 + d^2, y = ab + bc + cd - da
   uint32_t r = (uint32_t)(s.x * s.x + s.y * s.y + s.z * s.z + s.w * s.w);
   uint32_t y = (uint32_t)(s.x * s.y + s.y * s.z + s.z * s.w - s.w * s.x);
@@ -365,10 +373,33 @@ template <int NQ> __device__ __forceinline__ int t_bound_any(int4 s, int k, int 
   return 2 * (NQ + k) <= 64 ? t_bound<NQ, uint32_t>(s, k, le) : t_bound<NQ, unsigned long long>(s, k, le);
 }
 
-__device__ __forceinline__ uint64_t entry_hash(int4 v) {
-  const uint64_t lo = (uint64_t)(uint32_t)v.x | ((uint64_t)(uint32_t)v.y << 32);
-  const uint64_t hi = (uint64_t)(uint32_t)v.z | ((uint64_t)(uint32_t)v.w << 32);
-  return mix64(lo ^ mix64(hi + 0x632be59bd9b4e019ull));
+// 32-bit hash of one entry's four coefficients.  A multiply moves information only towards the high bits, so
+// each round folds the high half back down before the next coefficient enters; without that, entries with small
+// coefficients collide far more often than a random function would.
+__device__ __forceinline__ uint32_t entry_hash32(int4 v) {
+  uint32_t h = (uint32_t)v.x * 0x9e3779b1u + (uint32_t)v.y;
+  h ^= h >> 15;
+  h = h * 0x85ebca77u + (uint32_t)v.z;
+  h ^= h >> 13;
+  h = h * 0xc2b2ae3du + (uint32_t)v.w;
+  h ^= h >> 16;
+  h *= 0x27d4eb2fu;
+  return h ^ (h >> 15);
+}
+// the 64-bit summand of an entry hash at position pos (1..64): two independent one-multiply mixes of the pair.
+// 32-bit multiplies run at a quarter of the VALU rate and 64-bit ones take four of them, so the summand costs
+// two where a 64-bit mix costs eight; the sum over the entries still differs between two matrices unless their
+// differing entries' hashes collide (2^-32 per entry) or the 64-bit sums do.
+__device__ __forceinline__ uint64_t slot64(uint32_t h, uint32_t pos) {
+  const uint32_t x = h ^ __umul24(pos, 0x9e3779u);
+  uint32_t lo = x ^ (x >> 16);
+  lo *= 0x7feb352du;
+  lo ^= lo >> 15;
+  uint32_t hi = x ^ 0x5bd1e995u;
+  hi ^= hi >> 15;
+  hi *= 0x2c1b3c6du;
+  hi ^= hi >> 12;
+  return ((uint64_t)hi << 32) | lo;
 }
 
 // class key: the smallest over the symmetry group of an entry-sum hash of sigma(S); equal classes give
@@ -376,12 +407,12 @@ __device__ __forceinline__ uint64_t entry_hash(int4 v) {
 template <int NQ> __device__ __forceinline__ uint64_t class_key(int4 s, int k, int le, const KSpec &sp) {
   constexpr int N = 1 << NQ, GS = N * N;
   const int r = le >> NQ, c = le & (N - 1);
-  const uint64_t hv = entry_hash(s), hc = entry_hash(zconj(s));
+  const uint32_t hv = entry_hash32(s), hc = entry_hash32(zconj(s));
   uint64_t best = ~0ull;
   for (int y = 0; y < sp.nsym; y++) {
     const uint8_t *pm = sp.symperm + y * N;
-    const uint64_t pos = (uint64_t)(pm[r] * N + pm[c]) + 1;
-    const uint64_t h = mix64((sp.symconj[y] ? hc : hv) ^ (pos * 0x9e3779b97f4a7c15ull));
+    const uint32_t pos = (uint32_t)(pm[r] * N + pm[c]) + 1;
+    const uint64_t h = slot64(sp.symconj[y] ? hc : hv, pos);
     const uint64_t key = mix64(gsum<GS>(h) ^ ((uint64_t)(k + 1) * 0xc2b2ae3d27d4eb4full));
     best = key < best ? key : best;
   }
@@ -429,11 +460,10 @@ template <int CTRL> __device__ __forceinline__ uint64_t dpp64(uint64_t v) {
 // lane bits 2-5 give the same key as class_key.  spos[y * 64 + le]: position of entry le under symmetry y;
 // cjm bit y: symmetry y conjugates.
 __device__ __forceinline__ uint64_t class_key3(int4 s, int k, int lane, const uint8_t *spos, uint32_t cjm, int nsym) {
-  const uint64_t hv = entry_hash(s), hc = entry_hash(zconj(s));
+  const uint32_t hv = entry_hash32(s), hc = entry_hash32(zconj(s));
   auto slot = [&](int y) -> uint64_t {
     if (y >= nsym) return 0;
-    const uint64_t pos = (uint64_t)spos[y * 64 + lane] + 1;
-    return mix64((((cjm >> y) & 1) ? hc : hv) ^ (pos * 0x9e3779b97f4a7c15ull));
+    return slot64(((cjm >> y) & 1) ? hc : hv, (uint32_t)spos[y * 64 + lane] + 1);
   };
   uint64_t h[8];      // slots y and y + 8 are combined as soon as they are hashed: 8 live values, not 16
 #pragma unroll

@@ -45,11 +45,16 @@ def test_golden_logs_in_one_batch():
 
 
 def test_random_circuits_alone_and_batched():
-    """the 32 seeded random 3-qubit circuits (10..13 gates, budget = length): identical alone and in one batch"""
+    """the 32 seeded random 3-qubit circuits (10..13 gates, budget = length): identical alone and in one batch.  The
+    batch runs its small levels as captured steps and a search alone as direct launches, so the two paths agree"""
     specs = [W.random_spec(seed)[1] for seed in range(32)]
-    batched = cheese_amd.search_many(specs)
+    st = {}
+    batched = cheese_amd.search_many(specs, stats=st)
+    assert st["kernel_stats"].get("level_graph", {"launches": 0})["launches"] > 0
     for seed in (0, 5, 9, 17, 21, 31):
-        _same(cheese_amd.search(specs[seed]), batched[seed])
+        alone = cheese_amd.search(specs[seed], stats=True)
+        assert alone["kernel_stats"].get("level_graph", {"launches": 0})["launches"] == 0
+        _same(alone, batched[seed])
     for o in batched:
         assert o["rc"] == 0 and o["verified"]
 

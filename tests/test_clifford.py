@@ -80,8 +80,9 @@ def test_hip_table_identical_to_oracle(orc, n):
     dig, entries = T.digest()
     assert entries == ORDERS[n]
     assert dig == orc.orc_cliff_digest(h)
-    # batched random queries (words of 5..40 letters) against the oracle's table
-    words = clifford.random_words(1234 + n, 20000 if n == 3 else 2000, T.ng)
+    # batched random queries (words of 5..40 letters) against the oracle's table: at n = 3 the 1e5 queries of the
+    # Clifford workload
+    words = clifford.random_words(1234 + n, 100000 if n == 3 else 2000, T.ng)
     got, _ = T.query_words(words)
     want = [orc.orc_cliff_find(h, _word(orc, h, w)) for w in words]
     assert got == want

@@ -136,15 +136,16 @@ def test_random_three_qubit_batch_verifies(table3):
 
 
 def test_batched_targets_from_clifford_samples(table3):
-    """64 targets of the batched workload (Cliffords sampled uniformly from the Clifford table, realised by
-    their optimal words): seeded sampling is reproducible, every decomposition is exact and within k"""
+    """the batched workload at its full size, 256 targets (Cliffords sampled uniformly from the Clifford table,
+    realised by their optimal words) in one call: seeded sampling is reproducible, every decomposition is exact and
+    within k"""
     from cheese_amd import clifford
     cl = clifford.CliffordTable(3)
     try:
         a, b = cl.sample(5, 100), cl.sample(5, 100)
         assert a == b and len(set(a)) > 90
         assert all(w is not None for w in cl.readback(a[:20]))
-        cases = T.c3_targets(cl, 64, seed0=77)
+        cases = T.c3_targets(cl, 256, seed0=77)
     finally:
         cl.close()
     got = table3.search([T.tspec_text(3, W.unitary(3, c)) for _, c in cases], maxt=16)

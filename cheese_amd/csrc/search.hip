@@ -61,10 +61,11 @@ struct ExpandArgs {
 // (mod 2^16, one lane per flip pair, when n + k <= 16; else mod 2^32 / 2^64, one lane per entry) and the
 // class key of a kept child.  Writes the status byte (and key) of every child of the chunk.
 // kind: 0 general, 1 unit entries, 2 unit permutation; tdel, loc, cdel: DevGate::tdel, ::cutloc, ::cdel
-struct GateHdr { int nq, L, mask, ik, q0, q1, q2, kind, tdel, loc, cdel; };
+struct GateHdr { int nq, L, mask, ik, q0, q1, q2, kind, tdel, loc, cdel, cdid; };
 template <int NQ> __device__ __forceinline__ GateHdr wuni(const GateHdr &h) {
   return GateHdr{wuni<NQ>(h.nq), wuni<NQ>(h.L), wuni<NQ>(h.mask), wuni<NQ>(h.ik), wuni<NQ>(h.q0), wuni<NQ>(h.q1),
-                 wuni<NQ>(h.q2), wuni<NQ>(h.kind), wuni<NQ>(h.tdel), wuni<NQ>(h.loc), wuni<NQ>(h.cdel)};
+                 wuni<NQ>(h.q2), wuni<NQ>(h.kind), wuni<NQ>(h.tdel), wuni<NQ>(h.loc), wuni<NQ>(h.cdel),
+                 wuni<NQ>(h.cdid)};
 }
 constexpr int MAXG = 64;
 
@@ -83,6 +84,9 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(WPE))) void
   __shared__ uint8_t scx[CXL_MAX];                   // the cx table when it fits (sp.cxlds)
   for (int i = threadIdx.x; i < CXL_MAX; i += blockDim.x)
     scx[i] = sp.cxlds && sp.have_table && i < (int)sp.ncx ? (uint8_t)sp.cxtable[i] : 0;
+  __shared__ uint8_t scsrc[CSRC_MAX * GS];            // cut i's operator-Schmidt matrix: entry source of lane le
+  for (int i = threadIdx.x; i < CSRC_MAX * GS; i += blockDim.x)
+    scsrc[i] = i / GS < sp.ncuts ? (uint8_t)cut_source<NQ>(i % GS, sp.cutmask[i / GS], sp.cutsize[i / GS]) : 0;
   for (int i = threadIdx.x; i < sp.G * GS; i += blockDim.x) {
     const int g = i / GS, e = i % GS, ec = e & (N - 1);
     const DevGate &d = sp.gates[g];
@@ -122,7 +126,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(WPE))) void
         if (nz != 1) kind = 1;
       }
     }
-    shdr[g] = GateHdr{d.nq, d.L, d.mask, d.ik, d.q[0], d.q[1], d.q[2], kind, d.tdel, d.cutloc, d.cdel};
+    shdr[g] = GateHdr{d.nq, d.L, d.mask, d.ik, d.q[0], d.q[1], d.q[2], kind, d.tdel, d.cutloc, d.cdel, d.cdid};
   }
   __syncthreads();
   const int lane = threadIdx.x & 63, le = lane & (GS - 1), slot = lane / GS;
@@ -185,6 +189,14 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(WPE))) void
     __builtin_amdgcn_wave_barrier();
     // window cuts of every candidate of this parent, one candidate per lane, as two bit masks: the candidate
     // loop then tests bits instead of waiting on a table load per child
+    // lower cut bound of each cut-delta class (one per parent instead of one per child)
+    const bool cdcache = !sp.lite && sp.cx_mono && (pbw & BW_CUTS) && sp.ncd > 0;
+    uint64_t c2c = 0;
+    if (cdcache)
+      for (int j = 0; j < sp.ncd; j++) {
+        const int v = cx_lower(sp, scx, (pbw & BW_NEEDS) >> 8, sp.cdv[j]);
+        c2c |= (uint64_t)(v < 255 ? v : 255) << (8 * j);
+      }
     uint64_t w2m = 0, w3m = 0;
     for (int b0 = 0; b0 < G; b0 += GS) {
       const int g = b0 + le;
@@ -282,14 +294,16 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(WPE))) void
               const bool tknown = tpar && h.tdel == 0;
               int T = tpar ? (Tp > h.tdel ? Tp - h.tdel : 0) : 0;   // exact if tknown, else a lower bound
               bool texact = tknown;
-              const int c2lo = !sp.lite && sp.cx_mono && (pbw & BW_CUTS) ? cx_lower(sp, scx, (pbw & BW_NEEDS) >> 8, (uint32_t)h.cdel) : 0;
+              const int c2lo = cdcache ? (int)((c2c >> (8 * h.cdid)) & 0xff)
+                               : !sp.lite && sp.cx_mono && (pbw & BW_CUTS) ? cx_lower(sp, scx, (pbw & BW_NEEDS) >> 8, (uint32_t)h.cdel)
+                               : 0;
               if (combine_bound(hh, (T + sp.tdiv - 1) / sp.tdiv, c2lo, supp) > rem) alive = false;
               else {
                 if (!sp.lite) {
                   uint32_t needs = (pbw & BW_NEEDS) >> 8;
                   const uint32_t keepc = (pbw & BW_CUTS) ? (uint32_t)h.loc : 0u;
                   n_cuts += sp.ncuts - __popc(keepc & ((1u << sp.ncuts) - 1));
-                  c2 = cx_bound<NQ>(s, le, sp, scx, keepc, needs);
+                  c2 = cx_bound<NQ>(s, le, sp, scx, scsrc, keepc, needs);
                   bw = (needs << 8 & ~BW_CUTS) | (sp.ncuts <= BW_MAXCUTS ? BW_CUTS : 0u) | BW_TUNK;
                 }
                 if (combine_bound(hh, (T + sp.tdiv - 1) / sp.tdiv, c2, supp) > rem) alive = false;
@@ -762,6 +776,7 @@ struct chz_ctx {
   long long m = 0;             // nodes in the current frontier (buffer cur)
   int cur = 0, gstep = -1, nslots = 0;
   bool shared_steps = false;      // the batch has held more than one search at once (captured steps are used)
+  std::vector<uint32_t> cdv;      // distinct cut-delta values of the spec's gates (DevGate::cdid indexes them)
   // level control block, uploaded in one copy at the start of every level and read back in one copy:
   // [cnt: C_N counters | icnt: per-slot IC_N counters | d_rem: per-slot budgets | d_keepi: per-slot keep flags]
   chz::DBuf<unsigned long long> ctl;
@@ -857,7 +872,9 @@ static double t_ops16(int nq) {
 static double expand_ops(const chz_ctx *c, int nq, const unsigned long long *hc) {
   const double E = (double)(1 << (2 * nq));
   const double child = E * (8.0 * c->avg_terms + 10.0), cheap = E * (4.0 + 5.0 * nq), cut = E * (30.0 + 3 * 14.0);
-  const double key = E * (2 * 20.0 + 22.0 * c->ks.nsym);
+  // class key per lane: two 32-bit entry hashes (about 16 ops each), then per symmetry one slot summand (a 24-bit
+  // and two 32-bit products with their shifts and xors, 13 ops) and its two-word share of the 64-bit sums
+  const double key = E * (2 * 16.0 + 15.0 * c->ks.nsym);
   return (double)hc[C_EVAL] * child + (double)hc[C_CHEAP] * cheap + (double)hc[C_CUTS] * cut +
          (double)hc[C_NEEDT] * t_ops16(nq) / 2 + (double)hc[C_KEYS] * key;
 }
@@ -1502,6 +1519,16 @@ static void upload_spec(chz_ctx *c) {
       }
   }
   c->avg_terms = columns > 0 ? terms / columns : 1.0;
+  // cut-delta classes: the lower cut bound of a child depends on its parent and on its gate's cdel only, so the
+  // expansion computes it once per parent and class
+  std::vector<uint32_t> cdv;
+  for (DevGate &d : dg) {
+    size_t j = 0;
+    while (j < cdv.size() && cdv[j] != (uint32_t)d.cdel) j++;
+    if (j == cdv.size()) cdv.push_back((uint32_t)d.cdel);
+    d.cdid = (int)j;
+  }
+  c->cdv = cdv;
   std::vector<int4> goal(N * N);
   for (int i = 0; i < N * N; i++) goal[i] = make_int4((int)s.goal[i].a, (int)s.goal[i].b, (int)s.goal[i].c, (int)s.goal[i].d);
   std::vector<int> cx = s.cxtable.empty() ? std::vector<int>{0} : s.cxtable;
@@ -1565,6 +1592,8 @@ static void upload_spec(chz_ctx *c) {
   // a cut bound from lowered needs is a lower bound only if the table never decreases when a need grows
   k.cx_mono = 1;
   k.ncx = (int)s.cxtable.size();
+  k.ncd = c->cdv.size() <= (size_t)CD_MAX ? (int)c->cdv.size() : 0;
+  for (int j = 0; j < k.ncd; j++) k.cdv[j] = c->cdv[j];
   k.cxlds = s.cxtable.size() <= (size_t)CXL_MAX;
   for (int v : s.cxtable) if (v < 0 || v > 255) k.cxlds = 0;
   if (k.have_table) {

@@ -134,7 +134,7 @@ template <int NQ> __device__ __forceinline__ int support(int4 s, int le, uint64_
   return cnt;
 }
 
-__device__ __forceinline__ int spread_bits(int v, int mask, int n) {   // bit j of v -> j-th set bit of mask
+__host__ __device__ inline int spread_bits(int v, int mask, int n) {   // bit j of v -> j-th set bit of mask
   int out = 0, j = 0;
   for (int q = 0; q < n; q++)
     if ((mask >> q) & 1) { out |= ((v >> j) & 1) << q; j++; }
@@ -146,26 +146,37 @@ __device__ __forceinline__ int spread_bits(int v, int mask, int n) {   // bit j 
 // of a row not yet used becomes the pivot and its column is cleared from the other unused rows by
 // fraction-free updates (each a nonzero multiple of an F_p row combination), so after r steps the rank
 // is r plus that of the unused rows; the rank over F_p does not depend on the pivot order.
-template <int NQ> __device__ __forceinline__ int cut_rank(uint32_t fp, int le, int A, int sa, int stop) {
-  constexpr int N = 1 << NQ, GS = N * N;
-  const int R = 1 << (2 * sa), C = GS / R;
-  const int mrow = le / C, mcol = le % C;
+// Lane le of the matrix across cut A (|A| = sa) holds entry (le >> lgC, le & (C - 1)), C = 2^lgC = 4^(n - sa):
+// row (output bits, input bits) on A, column the same on the other qubits.  The residual entry it reads:
+template <int NQ> __host__ __device__ inline int cut_source(int le, int A, int sa) {
+  constexpr int N = 1 << NQ;
+  const int lgC = 2 * (NQ - sa);
+  const int mrow = le >> lgC, mcol = le & ((1 << lgC) - 1);
   const int nb = NQ - sa;
   const int ia = mrow & ((1 << sa) - 1), oa = mrow >> sa, ib = mcol & ((1 << nb) - 1), ob = mcol >> nb;
   const int notA = ((1 << NQ) - 1) & ~A;
   const int o = spread_bits(oa, A, NQ) | spread_bits(ob, notA, NQ);
   const int i = spread_bits(ia, A, NQ) | spread_bits(ib, notA, NQ);
-  uint32_t v = __shfl(fp, o * N + i, GS);
+  return o * N + i;
+}
+
+// src: cut_source of this lane, staged once per kernel; rows and columns are split by shifts (C is a power of 2)
+template <int NQ> __device__ __forceinline__ int cut_rank(uint32_t fp, int le, int src, int sa, int stop) {
+  constexpr int N = 1 << NQ, GS = N * N;
+  (void)N;
+  const int lgC = 2 * (NQ - sa), cm = (1 << lgC) - 1;
+  const int mrow = le >> lgC, mcol = le & cm;
+  uint32_t v = __shfl(fp, src, GS);
   int used = 0, rank = 0;                        // rows already pivots
   while (rank < stop) {
     const bool mine = !((used >> mrow) & 1);
     const uint64_t bb = gballot<GS>(mine && v != 0);
     if (!bb) break;
     const int p = __ffsll((long long)bb) - 1;
-    const int prow = p / C, pcol = p % C;
+    const int prow = p >> lgC, pcol = p & cm;
     const uint32_t pv = __shfl(v, p, GS);
-    const uint32_t fr = __shfl(v, mrow * C + pcol, GS);
-    const uint32_t pr = __shfl(v, prow * C + mcol, GS);
+    const uint32_t fr = __shfl(v, (mrow << lgC) + pcol, GS);
+    const uint32_t pr = __shfl(v, (prow << lgC) + mcol, GS);
     used |= 1 << prow;
     if (mine && mrow != prow) v = fp_sub(mont_mul(v, pv), mont_mul(fr, pr));
     rank++;
@@ -190,7 +201,8 @@ __device__ __forceinline__ int cx_at(const KSpec &sp, const uint8_t *cxl, int id
 }
 
 template <int NQ> __device__ __forceinline__ int cx_bound(int4 s, int le, const KSpec &sp, const uint8_t *cxl,
-                                                          uint32_t keepc, uint32_t &needs) {
+                                                          const uint8_t *csrc, uint32_t keepc, uint32_t &needs) {
+  constexpr int GS = 1 << (2 * NQ);
   uint32_t fp = 0;
   bool have_fp = false;
   int idx = 0, mult = 1, mx = 0;
@@ -204,7 +216,8 @@ template <int NQ> __device__ __forceinline__ int cx_bound(int4 s, int le, const 
       const int R = 1 << (2 * sa);
       int stop = R;
       if (rad >= 2 && rad - 2 < 30 && (1 << (rad - 2)) + 1 < stop) stop = (1 << (rad - 2)) + 1;
-      const int rk = cut_rank<NQ>(fp, le, sp.cutmask[i], sa, stop);
+      const int src = i < CSRC_MAX ? (int)csrc[i * GS + le] : cut_source<NQ>(le, sp.cutmask[i], sa);
+      const int rk = cut_rank<NQ>(fp, le, src, sa, stop);
       need = 0;
       while ((1 << need) < rk) need++;
       if (need > rad - 1) need = rad - 1;

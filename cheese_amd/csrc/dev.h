@@ -46,6 +46,8 @@ struct KSpec {                                 // device view of a spec (kernel 
   const int *cxtable;
   const uint8_t *symperm;                      // nsym x N basis maps
   const uint8_t *symconj;                      // nsym
+  uint32_t cjm;                                // bit y: symmetry y conjugates (y < 16)
+  const uint4 *ktab;                           // the expansion kernel's LDS tables (LevelTabs), built on the host
   // goal pre-images: xg[g] (N x N, sqrt2 exponent xgk[g], nonzero-entry mask xgm[g]) is the one normalised
   // residual that candidate g maps to the goal (xgk -1: none); a parent whose coefficients are all at most
   // maglim has no child outside int32

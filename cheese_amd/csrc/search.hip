@@ -69,6 +69,20 @@ template <int NQ> __device__ __forceinline__ GateHdr wuni(const GateHdr &h) {
 }
 constexpr int MAXG = 64;
 
+// The expansion kernel's LDS tables, built once per context on the host (build_level_tabs) and copied into LDS by
+// every block with coalesced 16-byte loads; built inside the kernel they cost each block scattered global loads,
+// which small levels (one parent per wave) cannot amortise.
+struct LevelTabs {
+  // per candidate and entry (g * 64 + e), the child entry's terms for unit gates with at most two per column:
+  // bits 0-2 source column x0, 3-5 power p0 of w, 6-8 x1, 9-11 p1, bit 12 two terms; bit 15: use the table loop
+  uint16_t sdesc[MAXG * 64];
+  GateHdr shdr[MAXG];
+  uint8_t spos[16 * 64];           // n = 3: position of entry e under symmetry y (y * 64 + e)
+  uint8_t scx[CXL_MAX];            // the cx table when it fits (KSpec::cxlds)
+  uint8_t scsrc[CSRC_MAX * 64];    // cut i's operator-Schmidt matrix: entry source of lane le (i * GS + le)
+};
+static_assert(sizeof(LevelTabs) % 16 == 0, "LevelTabs is copied as uint4");
+
 template <int NQ, int WPE>
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(WPE))) void k_level(KSpec sp, ExpandArgs a,
                                                                                          long long np) {
@@ -76,58 +90,12 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(WPE))) void
   __shared__ uint2 stage_x[TS_STRIDE];
   __shared__ uint4 stage_p[2 * TS_STRIDE];
   __shared__ int4 spar[256];                   // each group's parent residual
-  // per candidate and entry (g * 64 + e), the child entry's terms for unit gates with at most two per column:
-  // bits 0-2 source column x0, 3-5 power p0 of w, 6-8 x1, 9-11 p1, bit 12 two terms; bit 15: use the table loop
-  __shared__ uint16_t sdesc[MAXG * 64];
-  __shared__ GateHdr shdr[MAXG];
-  __shared__ uint8_t spos[NQ == 3 ? 16 * 64 : 1];   // n = 3: position of entry le under symmetry y (y * 64 + le)
-  __shared__ uint8_t scx[CXL_MAX];                   // the cx table when it fits (sp.cxlds)
-  for (int i = threadIdx.x; i < CXL_MAX; i += blockDim.x)
-    scx[i] = sp.cxlds && sp.have_table && i < (int)sp.ncx ? (uint8_t)sp.cxtable[i] : 0;
-  __shared__ uint8_t scsrc[CSRC_MAX * GS];            // cut i's operator-Schmidt matrix: entry source of lane le
-  for (int i = threadIdx.x; i < CSRC_MAX * GS; i += blockDim.x)
-    scsrc[i] = i / GS < sp.ncuts ? (uint8_t)cut_source<NQ>(i % GS, sp.cutmask[i / GS], sp.cutsize[i / GS]) : 0;
-  for (int i = threadIdx.x; i < sp.G * GS; i += blockDim.x) {
-    const int g = i / GS, e = i % GS, ec = e & (N - 1);
-    const DevGate &d = sp.gates[g];
-    int lc = (ec >> d.q[0]) & 1;
-    if (d.nq > 1) lc |= ((ec >> d.q[1]) & 1) << 1;
-    if (d.nq > 2) lc |= ((ec >> d.q[2]) & 1) << 2;
-    const int cb = ec & ~d.mask;
-    uint32_t desc = 0;
-    int nt = 0;
-    for (int m = 0; d.unit && m < d.L; m++) {
-      const int p = d.ph[m * d.L + lc];
-      if (p < 0) continue;
-      int x = cb | ((m & 1) << d.q[0]);
-      if (d.nq > 1) x |= ((m >> 1) & 1) << d.q[1];
-      if (d.nq > 2) x |= ((m >> 2) & 1) << d.q[2];
-      if (nt < 2) desc |= (uint32_t)(x | (p << 3)) << (6 * nt);
-      nt++;
-    }
-    sdesc[g * 64 + e] = (uint16_t)(!d.unit || nt == 0 || nt > 2 ? 0x8000u : nt == 2 ? desc | 0x1000u : desc);
-  }
-  uint32_t cjm = 0;
-  if constexpr (NQ == 3) {
-    for (int i = threadIdx.x; i < 16 * 64; i += blockDim.x) {
-      const int y = i >> 6, e = i & 63;
-      spos[i] = y < sp.nsym ? (uint8_t)(sp.symperm[y * 8 + (e >> 3)] * 8 + sp.symperm[y * 8 + (e & 7)]) : 0;
-    }
-    for (int y = 0; y < sp.nsym && y < 16; y++) cjm |= (uint32_t)(sp.symconj[y] != 0) << y;
-  }
-  for (int g = threadIdx.x; g < sp.G; g += blockDim.x) {
-    const DevGate &d = sp.gates[g];
-    int kind = 0;
-    if (d.unit) {
-      kind = 2;                                // permutation: one nonzero per column
-      for (int lc = 0; lc < d.L; lc++) {
-        int nz = 0;
-        for (int m = 0; m < d.L; m++) nz += d.ph[m * d.L + lc] >= 0;
-        if (nz != 1) kind = 1;
-      }
-    }
-    shdr[g] = GateHdr{d.nq, d.L, d.mask, d.ik, d.q[0], d.q[1], d.q[2], kind, d.tdel, d.cutloc, d.cdel, d.cdid};
-  }
+  __shared__ __align__(16) LevelTabs tb;
+  for (int i = threadIdx.x; i < (int)(sizeof(LevelTabs) / 16); i += blockDim.x) ((uint4 *)&tb)[i] = sp.ktab[i];
+  const uint16_t *const sdesc = tb.sdesc;
+  const GateHdr *const shdr = tb.shdr;
+  const uint8_t *const spos = tb.spos, *const scx = tb.scx, *const scsrc = tb.scsrc;
+  const uint32_t cjm = NQ == 3 ? sp.cjm : 0u;
   __syncthreads();
   const int lane = threadIdx.x & 63, le = lane & (GS - 1), slot = lane / GS;
   const int r = le >> NQ, c = le & (N - 1);
@@ -749,6 +717,7 @@ struct chz_ctx {
   chz::KSpec ks{};
   // spec on the device
   chz::DBuf<uint8_t> spec_blob;     // the device tables below, one block
+  chz::DBuf<uint8_t> ktab;          // the expansion kernel's LDS tables (LevelTabs)
   chz::DevGate *d_gates = nullptr;
   uint8_t *d_bad2 = nullptr, *d_symperm = nullptr, *d_symconj = nullptr;
   uint32_t *d_bad3 = nullptr;
@@ -1473,6 +1442,56 @@ template <int NQ> static void batch_step(chz_ctx *c) {
   prune_tree(c);
 }
 
+// LevelTabs for a spec: the term descriptors and headers of its candidates, its symmetries' entry positions (n = 3),
+// its cx table (when it fits) and the entry sources of its cut matrices
+static void build_level_tabs(LevelTabs &T, const std::vector<DevGate> &dg, const KSpec &k, const chz::SymGroup &grp,
+                             const HostSpec &s) {
+  std::memset(&T, 0, sizeof T);
+  const int n = k.n, N = 1 << n, GS = N * N;
+  for (int g = 0; g < k.G; g++) {
+    const DevGate &d = dg[g];
+    for (int e = 0; e < GS; e++) {
+      const int ec = e & (N - 1);
+      int lc = (ec >> d.q[0]) & 1;
+      if (d.nq > 1) lc |= ((ec >> d.q[1]) & 1) << 1;
+      if (d.nq > 2) lc |= ((ec >> d.q[2]) & 1) << 2;
+      const int cb = ec & ~d.mask;
+      uint32_t desc = 0;
+      int nt = 0;
+      for (int m = 0; d.unit && m < d.L; m++) {
+        const int p = d.ph[m * d.L + lc];
+        if (p < 0) continue;
+        int x = cb | ((m & 1) << d.q[0]);
+        if (d.nq > 1) x |= ((m >> 1) & 1) << d.q[1];
+        if (d.nq > 2) x |= ((m >> 2) & 1) << d.q[2];
+        if (nt < 2) desc |= (uint32_t)(x | (p << 3)) << (6 * nt);
+        nt++;
+      }
+      T.sdesc[g * 64 + e] = (uint16_t)(!d.unit || nt == 0 || nt > 2 ? 0x8000u : nt == 2 ? desc | 0x1000u : desc);
+    }
+    int kind = 0;
+    if (d.unit) {
+      kind = 2;                                // permutation: one nonzero per column
+      for (int lc = 0; lc < d.L; lc++) {
+        int nz = 0;
+        for (int m = 0; m < d.L; m++) nz += d.ph[m * d.L + lc] >= 0;
+        if (nz != 1) kind = 1;
+      }
+    }
+    T.shdr[g] = GateHdr{d.nq, d.L, d.mask, d.ik, d.q[0], d.q[1], d.q[2], kind, d.tdel, d.cutloc, d.cdel, d.cdid};
+  }
+  if (n == 3)
+    for (int y = 0; y < 16 && y < k.nsym; y++)
+      for (int e = 0; e < 64; e++) T.spos[y * 64 + e] = (uint8_t)(grp.bperm[y * 8 + (e >> 3)] * 8 + grp.bperm[y * 8 + (e & 7)]);
+  if (k.cxlds && k.have_table)
+    for (int i = 0; i < CXL_MAX && i < k.ncx; i++) T.scx[i] = (uint8_t)s.cxtable[i];
+  for (int i = 0; i < CSRC_MAX * GS; i++) {
+    if (i / GS >= k.ncuts) continue;
+    const int le = i % GS, A = k.cutmask[i / GS], sa = k.cutsize[i / GS];
+    T.scsrc[i] = (uint8_t)(n == 3 ? cut_source<3>(le, A, sa) : n == 2 ? cut_source<2>(le, A, sa) : cut_source<1>(le, A, sa));
+  }
+}
+
 static void upload_spec(chz_ctx *c) {
   const HostSpec &s = c->hs;
   const int N = s.N(), G = s.G();
@@ -1626,6 +1645,13 @@ static void upload_spec(chz_ctx *c) {
   k.symperm = c->d_symperm; k.symconj = c->d_symconj;
   k.xg = c->d_xg; k.xgm = c->d_xgm; k.xgk = c->d_xgk;
   k.maglim = (uint32_t)(0x7fffffffull / gmax);
+  k.cjm = 0;
+  for (int y = 0; y < k.nsym && y < 16; y++) k.cjm |= (uint32_t)(c->grp.conj[y] != 0) << y;
+  std::vector<LevelTabs> tabs(1);
+  build_level_tabs(tabs[0], dg, k, c->grp, s);
+  c->ktab.need(sizeof(LevelTabs));
+  chz::copy_sync(c->ktab.p, tabs.data(), sizeof(LevelTabs), hipMemcpyHostToDevice, c->stream);
+  k.ktab = (const uint4 *)c->ktab.p;
 }
 
 // a new search of the batch: the depth-0 cases end at once (src/cheese.cu:1794-1801), the rest wait to join
@@ -1983,6 +2009,7 @@ void chz_close(chz_ctx *c) {
   c->surv_bw.release();
   c->info.release(); c->bcount.release(); c->hist.release(); c->runs.release(); c->kidx.release();
   c->spec_blob.release();
+  c->ktab.release();
   if (c->gt) { c->gt->release(); delete c->gt; c->gt = nullptr; }
   c->gt_keys.release(); c->gt_hit.release(); c->gt_cost.release(); c->gt_best.release();
   c->ev.destroy();

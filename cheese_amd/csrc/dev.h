@@ -20,7 +20,9 @@ struct DevGate {                               // S <- S g^-1 data of one candid
   int cutloc;                                  // bit i: the gate acts on one side of cut i only
   int cdel;                                    // 3 bits per cut: a child's cut need is at least its parent's
                                                // minus this (ceil log2 of the gate's Schmidt rank); 7: none
-  int cdid;                                    // index of cdel among the spec's distinct values (KSpec::cdv)
+  int cls;                                     // bits 0-7: index of cdel among the spec's distinct values
+                                               // (KSpec::cdv); bits 8-15: 1 + the gate's T-bound class (0:
+                                               // none): candidates of one class give children of equal T bound
   int4 inv[64];                                // L x L inverse local matrix, row m, column lc
   int8_t ph[64];                               // unit gates: w-power p of inv[m][lc], -1 for zero
 };
@@ -28,6 +30,7 @@ struct DevGate {                               // S <- S g^-1 data of one candid
 constexpr int CXL_MAX = 128;                   // largest cx table held in the expansion kernel's LDS
 constexpr int CD_MAX = 8;                      // distinct cut-delta classes cached per parent
 constexpr int CSRC_MAX = 4;                    // cuts whose matrix-entry sources the expansion kernel stages
+constexpr int TC_MAX = 8;                      // T-bound classes cached per parent
 
 struct KSpec {                                 // device view of a spec (kernel argument)
   int n, G, depth, standard, lite, tdiv, gk, nsym, ncuts, have_table;
@@ -35,7 +38,7 @@ struct KSpec {                                 // device view of a spec (kernel 
   int cxlds;                                   // the cx table fits the expansion kernel's LDS copy (CXL_MAX bytes)
   int ncx;                                     // cx table entries
   int ncd;                                     // distinct gate cdel values (0: more than CD_MAX)
-  uint32_t cdv[8];                             // those values; gate g's is cdv[gates[g].cdid]
+  uint32_t cdv[8];                             // those values; gate g's is cdv[gates[g].cls & 0xff]
   int cutmask[16], cutsize[16], rad[16];
   uint32_t wm[4];                              // Montgomery constants w^i * 2^32 mod p, i = 0..3
   const DevGate *gates;

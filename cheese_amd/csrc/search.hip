@@ -61,11 +61,11 @@ struct ExpandArgs {
 // (mod 2^16, one lane per flip pair, when n + k <= 16; else mod 2^32 / 2^64, one lane per entry) and the
 // class key of a kept child.  Writes the status byte (and key) of every child of the chunk.
 // kind: 0 general, 1 unit entries, 2 unit permutation; tdel, loc, cdel: DevGate::tdel, ::cutloc, ::cdel
-struct GateHdr { int nq, L, mask, ik, q0, q1, q2, kind, tdel, loc, cdel, cdid; };
+struct GateHdr { int nq, L, mask, ik, q0, q1, q2, kind, tdel, loc, cdel, cls; };
 template <int NQ> __device__ __forceinline__ GateHdr wuni(const GateHdr &h) {
   return GateHdr{wuni<NQ>(h.nq), wuni<NQ>(h.L), wuni<NQ>(h.mask), wuni<NQ>(h.ik), wuni<NQ>(h.q0), wuni<NQ>(h.q1),
                  wuni<NQ>(h.q2), wuni<NQ>(h.kind), wuni<NQ>(h.tdel), wuni<NQ>(h.loc), wuni<NQ>(h.cdel),
-                 wuni<NQ>(h.cdid)};
+                 wuni<NQ>(h.cls)};
 }
 constexpr int MAXG = 64;
 
@@ -165,6 +165,8 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(WPE))) void
         const int v = cx_lower(sp, scx, (pbw & BW_NEEDS) >> 8, sp.cdv[j]);
         c2c |= (uint64_t)(v < 255 ? v : 255) << (8 * j);
       }
+    uint64_t tc_val = 0;          // exact T bounds of this parent's children per T-bound class (tc_ok: known)
+    uint32_t tc_ok = 0;
     uint64_t w2m = 0, w3m = 0;
     for (int b0 = 0; b0 < G; b0 += GS) {
       const int g = b0 + le;
@@ -259,10 +261,14 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(WPE))) void
               // before the cut ranks and the T transform are computed.
               const bool tpar = (pbw & BW_TUNK) != BW_TUNK && h.tdel != 255;
               const int Tp = (int)(pbw & BW_TUNK);
-              const bool tknown = tpar && h.tdel == 0;
-              int T = tpar ? (Tp > h.tdel ? Tp - h.tdel : 0) : 0;   // exact if tknown, else a lower bound
+              // a sibling of the same T-bound class already has its exact bound: it is this child's too
+              const int tc = (h.cls >> 8) - 1;
+              const bool tcached = tc >= 0 && ((tc_ok >> tc) & 1);
+              const bool tknown = (tpar && h.tdel == 0) || tcached;
+              int T = tcached ? (int)((tc_val >> (8 * tc)) & 0xff)
+                              : tpar ? (Tp > h.tdel ? Tp - h.tdel : 0) : 0;   // exact if tknown, else a lower bound
               bool texact = tknown;
-              const int c2lo = cdcache ? (int)((c2c >> (8 * h.cdid)) & 0xff)
+              const int c2lo = cdcache ? (int)((c2c >> (8 * (h.cls & 0xff))) & 0xff)
                                : !sp.lite && sp.cx_mono && (pbw & BW_CUTS) ? cx_lower(sp, scx, (pbw & BW_NEEDS) >> 8, (uint32_t)h.cdel)
                                : 0;
               if (combine_bound(hh, (T + sp.tdiv - 1) / sp.tdiv, c2lo, supp) > rem) alive = false;
@@ -291,6 +297,10 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(WPE))) void
                     T = t_bound_any<NQ>(s, k, le);
                   }
                   texact = true;
+                  if (tc >= 0 && T < 255) {
+                    tc_val |= (uint64_t)T << (8 * tc);
+                    tc_ok |= 1u << tc;
+                  }
                   alive = combine_bound(hh, (T + sp.tdiv - 1) / sp.tdiv, c2, supp) <= rem;
                 }
               }
@@ -745,7 +755,7 @@ struct chz_ctx {
   long long m = 0;             // nodes in the current frontier (buffer cur)
   int cur = 0, gstep = -1, nslots = 0;
   bool shared_steps = false;      // the batch has held more than one search at once (captured steps are used)
-  std::vector<uint32_t> cdv;      // distinct cut-delta values of the spec's gates (DevGate::cdid indexes them)
+  std::vector<uint32_t> cdv;      // distinct cut-delta values of the spec's gates (bits 0-7 of DevGate::cls index them)
   // level control block, uploaded in one copy at the start of every level and read back in one copy:
   // [cnt: C_N counters | icnt: per-slot IC_N counters | d_rem: per-slot budgets | d_keepi: per-slot keep flags]
   chz::DBuf<unsigned long long> ctl;
@@ -1478,7 +1488,7 @@ static void build_level_tabs(LevelTabs &T, const std::vector<DevGate> &dg, const
         if (nz != 1) kind = 1;
       }
     }
-    T.shdr[g] = GateHdr{d.nq, d.L, d.mask, d.ik, d.q[0], d.q[1], d.q[2], kind, d.tdel, d.cutloc, d.cdel, d.cdid};
+    T.shdr[g] = GateHdr{d.nq, d.L, d.mask, d.ik, d.q[0], d.q[1], d.q[2], kind, d.tdel, d.cutloc, d.cdel, d.cls};
   }
   if (n == 3)
     for (int y = 0; y < 16 && y < k.nsym; y++)
@@ -1545,9 +1555,25 @@ static void upload_spec(chz_ctx *c) {
     size_t j = 0;
     while (j < cdv.size() && cdv[j] != (uint32_t)d.cdel) j++;
     if (j == cdv.size()) cdv.push_back((uint32_t)d.cdel);
-    d.cdid = (int)j;
+    d.cls = (int)j;
   }
   c->cdv = cdv;
+  // T-bound classes: non-Clifford candidates whose children always have equal T bounds (same_t_bound), so a
+  // parent's expansion computes the bound once per class
+  {
+    std::vector<int> cls(G, -1);
+    int ncls = 0;
+    for (int g = 0; g < G; g++) {
+      if (dg[g].tdel == 0 || cls[g] >= 0) continue;
+      std::vector<int> mem{g};
+      for (int h2 = g + 1; h2 < G; h2++)
+        if (dg[h2].tdel != 0 && cls[h2] < 0 && same_t_bound(s.gates[g], s.gates[h2])) mem.push_back(h2);
+      if (mem.size() < 2 || ncls >= TC_MAX) continue;
+      for (int m : mem) cls[m] = ncls;
+      ncls++;
+    }
+    for (int g = 0; g < G; g++) dg[g].cls |= (cls[g] + 1) << 8;
+  }
   std::vector<int4> goal(N * N);
   for (int i = 0; i < N * N; i++) goal[i] = make_int4((int)s.goal[i].a, (int)s.goal[i].b, (int)s.goal[i].c, (int)s.goal[i].d);
   std::vector<int> cx = s.cxtable.empty() ? std::vector<int>{0} : s.cxtable;
@@ -1800,7 +1826,7 @@ int chz_run(chz_ctx *c, chz_level_cb cb, void *user, chz_result *out) {
 // the schedule.  out (may be NULL) receives the chosen table as JSON text.
 int chz_tune(chz_ctx *c, int reps, char *out, size_t cap) {
   if (chz_batch_live(c)) { chz::set_err("the context's batch holds searches"); return 3; }
-  const int vars[3] = {0, 1, 2}, caps[3] = {4096, 8192, 16384}, tws[3] = {2048, 4096, 8192};
+  const int vars[3] = {0, 1, 2}, caps[3] = {4096, 8192, 16384}, tws[5] = {2048, 4096, 8192, 16384, 32768};
   std::vector<chz_ctx::Sched> cands;
   for (int v : vars) for (int cp : caps) for (int tw : tws) { chz_ctx::Sched s; s.var = v; s.cap = cp; s.tw = tw; cands.push_back(s); }
   std::vector<std::vector<double>> sum(cands.size(), std::vector<double>(32, 0.0));

@@ -319,6 +319,27 @@ int gate_ptm_sde(const HostGate &g) {
 
 bool gate_is_clifford(const HostGate &g) { return gate_ptm_sde(g) == 0; }
 
+bool same_t_bound(const HostGate &a, const HostGate &b) {
+  if (a.nq != b.nq) return false;
+  for (int j = 0; j < a.nq; j++) if (a.q[j] != b.q[j]) return false;
+  const int L = 1 << a.nq;
+  if ((int)a.fwd.size() != L * L || (int)b.inv.size() != L * L) return false;
+  HostGate c;
+  c.nq = a.nq;
+  c.gk = a.gk + b.ik;
+  c.fwd.assign(L * L, Zq{0, 0, 0, 0});
+  for (int r = 0; r < L; r++)
+    for (int col = 0; col < L; col++) {
+      Zq acc{0, 0, 0, 0};
+      for (int j = 0; j < L; j++) {
+        const Zq t = zq_mul(a.fwd[r * L + j], b.inv[j * L + col]);
+        acc = {acc.a + t.a, acc.b + t.b, acc.c + t.c, acc.d + t.d};
+      }
+      c.fwd[r * L + col] = acc;
+    }
+  return gate_ptm_sde(c) == 0;
+}
+
 // Operator-Schmidt rank over F_p (w -> 3^((p-1)/8)) of the gate's local matrix across the split of its qubits
 // into those in cut mask A and the rest: rank of the realigned matrix R[(oa, ia)][(ob, ib)] = M[(oa, ob)][(ia, ib)].
 int gate_cut_rank(const HostGate &g, int A) {

@@ -69,5 +69,9 @@ bool gate_is_clifford(const HostGate &g);
 int gate_ptm_sde(const HostGate &g);
 // operator-Schmidt rank over F_p of the gate across cut mask A (1 when all its qubits are on one side)
 int gate_cut_rank(const HostGate &g, int A);
+// true when candidates a and b act on the same qubits and a b^-1 is a Clifford: then for every residual S the
+// children S a^-1 and S b^-1 = (S a^-1)(a b^-1) have equal T bounds (a Clifford's transfer matrix is a signed
+// permutation).  T and T^dagger on one qubit are such a pair: T T = S.
+bool same_t_bound(const HostGate &a, const HostGate &b);
 
 }  // namespace chz

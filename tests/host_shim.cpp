@@ -31,6 +31,15 @@ extern "C" int shim_goal_preimage(const char *spec, long len, int g, long long *
   return X.k;
 }
 
+// whether candidates g1 and g2 of the spec give children of equal T bound from every residual (same_t_bound)
+extern "C" int shim_same_t_bound(const char *spec, long len, int g1, int g2) {
+  chz::HostSpec s;
+  std::string err;
+  if (chz::parse_spec(std::string(spec, (size_t)len), s, err) || g1 < 0 || g2 < 0 || g1 >= s.G() || g2 >= s.G())
+    return -1;
+  return chz::same_t_bound(s.gates[g1], s.gates[g2]) ? 1 : 0;
+}
+
 // whether every candidate has a goal pre-image (the condition of the found-level skip), after multiplying the
 // goal's entries by `scale` (a large scale makes pre-images leave int32, so some candidates have none)
 extern "C" int shim_goal_skip(const char *spec, long len, long long scale) {

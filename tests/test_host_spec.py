@@ -36,6 +36,8 @@ def shim(tmp_path_factory):
                                      ctypes.c_int]
     L.shim_goal_skip.restype = ctypes.c_int
     L.shim_goal_skip.argtypes = [ctypes.c_char_p, ctypes.c_long, ctypes.c_longlong]
+    L.shim_same_t_bound.restype = ctypes.c_int
+    L.shim_same_t_bound.argtypes = [ctypes.c_char_p, ctypes.c_long, ctypes.c_int, ctypes.c_int]
     L.shim_gate_info.restype = ctypes.c_int
     L.shim_gate_info.argtypes = [ctypes.c_char_p, ctypes.c_long, ctypes.c_int, ctypes.POINTER(ctypes.c_int),
                                  ctypes.c_int]
@@ -134,3 +136,42 @@ def test_found_level_skip_needs_every_preimage(shim, name):
     spec = read_spec(name).encode()
     assert shim.shim_goal_skip(spec, len(spec), 1) == 1
     assert shim.shim_goal_skip(spec, len(spec), 1 << 40) == 0
+
+
+def test_t_bound_classes_share_the_oracle_bound(shim, oracle_lib):
+    """candidates a, b with a b^-1 Clifford give children of equal T bound from any residual, so the expansion
+    computes one T transform per class and parent.  On the Toffoli candidate set the host pairs exactly T and
+    T-dagger on each qubit among its non-Clifford candidates (Clifford children inherit their parent's bound
+    anyway); the oracle's T bound (the definitional transfer-matrix exponent) of S a^-1 and S b^-1 agrees
+    on random residuals S for every pair, and differs for T on two different qubits (a pairing the rule rejects)"""
+    import random
+    spec = read_spec("toffoli-d12").encode()
+    names = [ln.split()[0] for ln in read_spec("toffoli-d12").split("gates ")[1].splitlines()[1:28]]
+    G = len(names)
+    info = (ctypes.c_int * 8)()
+    nonclifford = []
+    for g in range(G):
+        shim.shim_gate_info(spec, len(spec), g, info, 8)
+        if not info[0]:
+            nonclifford.append(g)
+    pairs = {(a, b) for a in nonclifford for b in nonclifford
+             if a < b and shim.shim_same_t_bound(spec, len(spec), a, b) == 1}
+    want = {(names.index(f"L_t_{q}"), names.index(f"L_tdg_{q}")) for q in range(3)}
+    assert pairs == want
+    rng = random.Random(5)
+    N, E = 8, 64
+
+    def tbound(seq):
+        buf = (ctypes.c_longlong * (4 * E))()
+        arr = (ctypes.c_int * max(1, len(seq)))(*seq)
+        k = oracle_lib.orc_residual(spec, len(spec), arr, len(seq), buf)
+        assert k >= 0
+        return oracle_lib.orc_bound(3, k, buf, 1, 0)
+
+    differ = 0
+    for _ in range(40):
+        prefix = [rng.randrange(G) for _ in range(rng.randrange(0, 6))]
+        for a, b in want:
+            assert tbound(prefix + [a]) == tbound(prefix + [b])
+        differ += tbound(prefix + [names.index("L_t_0")]) != tbound(prefix + [names.index("L_t_1")])
+    assert differ > 0

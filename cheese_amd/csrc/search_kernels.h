@@ -172,6 +172,7 @@ template <int NQ> __device__ __forceinline__ int cut_rank(uint32_t fp, int le, i
     const bool mine = !((used >> mrow) & 1);
     const uint64_t bb = gballot<GS>(mine && v != 0);
     if (!bb) break;
+    if (rank + 1 == stop) return stop;          // the rank reaches the cap: no need to eliminate this pivot
     const int p = __ffsll((long long)bb) - 1;
     const int prow = p >> lgC, pcol = p & cm;
     const uint32_t pv = __shfl(v, p, GS);

@@ -64,6 +64,7 @@ SIGNATURES = {
     "chz_profile": (ctypes.c_int, [ctypes.c_void_p, ctypes.POINTER(ctypes.c_char_p), ctypes.POINTER(ctypes.c_double), ctypes.c_int]),
     "chz_kernel_stats": (ctypes.c_int, [ctypes.c_void_p, ctypes.POINTER(KStat), ctypes.c_int]),
     "chz_expand_stages": (ctypes.c_int, [ctypes.c_void_p, ctypes.POINTER(ctypes.c_double)]),
+    "chz_time_kernels": (ctypes.c_uint, [ctypes.c_void_p, ctypes.c_uint]),
     "chz_verify": (ctypes.c_int, [ctypes.c_void_p, ctypes.POINTER(ctypes.c_int), ctypes.c_int]),
     "chz_last_error": (ctypes.c_char_p, []),
     "chz_verify_spec": (ctypes.c_int, [ctypes.c_char_p, ctypes.c_size_t, ctypes.POINTER(ctypes.c_int), ctypes.c_int]),
@@ -76,6 +77,17 @@ SIGNATURES = {
     "chz_batch_set_budget": (None, [ctypes.c_void_p, ctypes.c_double]),
     "chz_batch_waits": (ctypes.c_longlong, [ctypes.c_void_p]),
 }
+
+
+# kernel families of chz_kernel_stats, in order (bit f of chz_time_kernels)
+FAMILIES = ("expand", "compact", "sort", "dedupe", "gather", "level_graph")
+
+
+def family_mask(families) -> int:
+    """chz_time_kernels mask of family names; "all" for every family"""
+    if families == "all":
+        return (1 << len(FAMILIES)) - 1
+    return sum(1 << FAMILIES.index(f) for f in families)
 
 
 def lib():
@@ -118,7 +130,7 @@ def search(spec_text: str, depth: int | None = None, dedupe: bool = True, sym: b
     """Runs the gate-count search on a .spec text (the reference's `cheese SPEC [flags]`; table=K is `--table K`,
     mitm `--mitm`).  Returns dict(rc, levels=[...], circuit=[names] or None, found, seconds, split, kernel_stats)."""
     L = lib()
-    o = Opts(-1 if depth is None else depth, int(dedupe), int(sym), chunk, 0, device, table, int(mitm))
+    o = Opts(-1 if depth is None else depth, int(dedupe), int(sym), chunk, int(stats), device, table, int(mitm))
     ctx = ctypes.c_void_p()
     raw = spec_text.encode()
     rc = L.chz_open(raw, len(raw), ctypes.byref(o), ctypes.byref(ctx))
@@ -172,6 +184,11 @@ class Searcher:
         rc = L.chz_open(raw, len(raw), ctypes.byref(o), ctypes.byref(self._ctx))
         if rc:
             raise SearchError(rc, L.chz_last_error().decode())
+
+    def time_kernels(self, families="all") -> None:
+        """kernel families whose launches record HIP events for run(stats=True)'s milliseconds (chz_time_kernels);
+        none by default, since each timed launch costs host time"""
+        lib().chz_time_kernels(self._ctx, family_mask(families))
 
     def tune(self, reps: int = 2) -> dict:
         """chz_tune: the fastest expansion schedule per level-size bucket, kept by this context"""
@@ -299,8 +316,13 @@ class Batch:
         """per-level memory planning: searches that would not fit wait a step (chz_batch_set_budget)"""
         lib().chz_batch_set_budget(self._ctx, float(nbytes))
 
+    def time_kernels(self, families="all") -> None:
+        """kernel families whose launches record HIP events for kernel_stats' milliseconds (chz_time_kernels)"""
+        lib().chz_time_kernels(self._ctx, family_mask(families))
+
     def kernel_stats(self) -> dict:
-        """chz_kernel_stats of the batch's context: launches, milliseconds and work per kernel family"""
+        """chz_kernel_stats of the batch's context: launches, milliseconds (of timed families) and work per kernel
+        family"""
         ks = (KStat * 16)()
         nk = lib().chz_kernel_stats(self._ctx, ks, 16)
         return {ks[i].name.decode(): {"launches": ks[i].launches, "ms": ks[i].ms, "work": ks[i].work} for i in range(nk)}
@@ -326,6 +348,8 @@ def search_many(specs, device: int = 0, budget: float = 0, stats: dict | None = 
     per-level memory planning in bytes (0: none); stats (a dict) receives the number of search-levels that waited
     and the batch's kernel statistics"""
     b = Batch(specs[0], device)
+    if stats is not None:
+        b.time_kernels("all")
     if budget:
         b.set_budget(budget)
     outs = [{"rc": None, "levels": [], "circuit": None} for _ in specs]

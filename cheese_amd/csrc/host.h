@@ -208,8 +208,11 @@ struct Pinned {
 
 inline double now_s() { return std::chrono::duration<double>(std::chrono::steady_clock::now().time_since_epoch()).count(); }
 
-// HIP-event timing of kernel families on one stream; pairs are read after the stream synchronised
+// HIP-event timing of kernel families on one stream; pairs are read after the stream synchronised.  Only the
+// families in `mask` record events (two records and an elapsed-time query per launch cost host time on every
+// level); the others count launches only.
 struct EventLog {
+  unsigned mask = ~0u;
   std::vector<hipEvent_t> pool;
   std::vector<std::pair<int, std::pair<hipEvent_t, hipEvent_t>>> open;
   hipEvent_t take() {
@@ -217,24 +220,29 @@ struct EventLog {
     hipEvent_t e = pool.back(); pool.pop_back(); return e;
   }
   void begin(int fam, hipStream_t s) {
+    if (!((mask >> fam) & 1u)) { open.push_back({fam, {nullptr, nullptr}}); return; }
     hipEvent_t a = take(), b = take();
     HIPCHK(hipEventRecord(a, s));
     open.push_back({fam, {a, b}});
   }
-  void end(hipStream_t s) { HIPCHK(hipEventRecord(open.back().second.second, s)); }
+  void end(hipStream_t s) {
+    if (hipEvent_t b = open.back().second.second) HIPCHK(hipEventRecord(b, s));
+  }
   void flush(double *ms, long *launches) {
     for (auto &o : open) {
+      launches[o.first]++;
+      if (!o.second.first) continue;
       float x = 0;
       HIPCHK(hipEventElapsedTime(&x, o.second.first, o.second.second));
       ms[o.first] += x;
-      launches[o.first]++;
       pool.push_back(o.second.first);
       pool.push_back(o.second.second);
     }
     open.clear();
   }
   void destroy() {
-    for (auto &o : open) { pool.push_back(o.second.first); pool.push_back(o.second.second); }
+    for (auto &o : open)
+      if (o.second.first) { pool.push_back(o.second.first); pool.push_back(o.second.second); }
     open.clear();
     for (auto e : pool) (void)hipEventDestroy(e);
     pool.clear();

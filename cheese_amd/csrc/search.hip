@@ -1758,6 +1758,7 @@ int chz_open(const char *spec_text, size_t len, const chz_opts *opts, chz_ctx **
   if (opts) o = *opts;
   c->opts = o;
   c->depth = o.depth >= 0 ? o.depth : c->hs.depth;
+  c->ev.mask = o.profile ? ~0u : 0u;
   if (c->depth > CHZ_DMAX) { chz::set_err("depth > 48"); return 3; }
   if (c->hs.states) { chz::set_err("state targets are not supported by this build"); return 3; }
   if (c->hs.n < 1 || c->hs.n > 3) { chz::set_err("n = " + std::to_string(c->hs.n) + " not built (1..3)"); return 3; }
@@ -1950,6 +1951,12 @@ int chz_profile(const chz_ctx *c, const char **names, double *seconds, int n) {
     if (seconds) seconds[j] = c->k_ms[f] / 1000.0;
   }
   return j;
+}
+
+unsigned chz_time_kernels(chz_ctx *c, unsigned families) {
+  const unsigned prev = c->ev.mask;
+  c->ev.mask = families;
+  return prev;
 }
 
 int chz_expand_stages(const chz_ctx *c, double *out) {

@@ -86,6 +86,10 @@ long long device_used_bytes() {
   return (long long)(total_b - free_b);
 }
 
+// kernel families an engine times for its statistics: the expansion and the captured level steps, which hold
+// nearly all of a batch's kernel time, at one event pair per launch
+constexpr unsigned kServerTimed = (1u << CHZ_FAM_EXPAND) | (1u << CHZ_FAM_LEVEL_GRAPH);
+
 double ctx_kernel_ms(const chz_ctx *c) {
   chz_kstat ks[16];
   const int n = chz_kernel_stats(c, ks, 16);
@@ -182,6 +186,7 @@ void gate_worker(chz_server *s, int qi) {
         chz_ctx *fresh = nullptr;
         if (chz_open(s->base[qi].data(), s->base[qi].size(), &co, &fresh) == 0) {
           ctx = s->gctx[qi] = fresh;
+          chz_time_kernels(ctx, kServerTimed);
           chz_batch_set_budget(ctx, share);
           continue;
         }
@@ -386,6 +391,7 @@ int chz_server_open(const char *spec2, size_t len2, const char *spec3, size_t le
     if (!bs[qi]) continue;
     s->base[qi].assign(bs[qi], bl[qi]);
     if (const int rc = chz_open(bs[qi], bl[qi], &co, &s->gctx[qi])) { chz_server_close(s); return rc; }
+    chz_time_kernels(s->gctx[qi], kServerTimed);
   }
   if (tspec3) {
     if (const int rc = chz_tc_table(tspec3, lent, s->o.tcount_K, s->o.device, &s->tc)) { chz_server_close(s); return rc; }

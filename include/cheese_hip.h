@@ -83,7 +83,8 @@ void chz_close(chz_ctx *ctx);
 int chz_profile(const chz_ctx *ctx, const char **names, double *seconds, int n);
 
 /* Device-side kernel statistics of the last run (no reference counterpart; for measurement): per kernel
- * family, launches, summed HIP-event milliseconds and the algorithmic work it did. */
+ * family, launches, summed HIP-event milliseconds (0 for a family that chz_time_kernels leaves untimed) and the
+ * algorithmic work it did. */
 typedef struct chz_kstat {
   const char *name;
   long launches;
@@ -91,6 +92,16 @@ typedef struct chz_kstat {
   double work;    /* algorithmic units: integer ops for the expansion kernels, bytes for sort/dedupe */
 } chz_kstat;
 int chz_kernel_stats(const chz_ctx *ctx, chz_kstat *out, int n);
+
+/* kernel families, in chz_kernel_stats order */
+enum { CHZ_FAM_EXPAND = 0, CHZ_FAM_COMPACT, CHZ_FAM_SORT, CHZ_FAM_DEDUPE, CHZ_FAM_GATHER, CHZ_FAM_LEVEL_GRAPH };
+
+/* Which kernel families record HIP events around their launches (no reference counterpart; for measurement): bit f
+ * = family f.  chz_open sets every family when opts->profile is nonzero (`--profile` reads these times through
+ * chz_profile) and none otherwise: an event pair per launch costs host time on every level, 0.4 ms of a 9 ms
+ * Toffoli budget-12 proof with every family timed.  Launch counts are kept either way.  Returns the previous mask;
+ * takes effect at the next launch. */
+unsigned chz_time_kernels(chz_ctx *ctx, unsigned families);
 
 /* Stage counts of the last run's expansion (no reference counterpart; for measurement), the factors of its work
  * model: out[0] residuals formed, [1] children through the structural bounds, [2] cut ranks computed, [3] T-bound
@@ -291,8 +302,9 @@ long long chz_server_outstanding(const chz_server *s);            /* submitted a
 long long chz_server_peak_bytes(const chz_server *s);             /* largest frontier-memory estimate seen */
 long long chz_server_peak_device_bytes(const chz_server *s);      /* largest device memory in use, measured */
 /* per-engine work so far (call between bursts): out[0..1] level steps of the 2- and 3-qubit gate batches,
- * [2..3] their seconds, [4..5] their kernel milliseconds, [6..7] their most live searches, [8] T-count batches,
- * [9] their seconds, [10] Clifford batches, [11] their seconds; returns the number written */
+ * [2..3] their seconds, [4..5] their kernel milliseconds in expansion launches and captured level steps,
+ * [6..7] their most live searches, [8] T-count batches, [9] their seconds, [10] Clifford batches, [11] their
+ * seconds; returns the number written */
 int chz_server_stats(chz_server *s, double *out, int n);
 const char *chz_server_error(chz_server *s);                       /* last device error a worker met ("" none) */
 double chz_server_clock(const chz_server *s);                      /* seconds since the server opened */

@@ -168,7 +168,7 @@ def search(spec_text: str, depth: int | None = None, dedupe: bool = True, sym: b
 
 
 # expansion stage counts (chz_expand_stages)
-STAGES = ("residuals", "structural", "cut_ranks", "t_bounds", "keys")
+STAGES = ("residuals", "structural", "cut_ranks", "t_bounds", "keys", "t_passes")
 
 
 class Searcher:
@@ -222,7 +222,7 @@ class Searcher:
             nk = L.chz_kernel_stats(self._ctx, ks, 16)
             out["kernel_stats"] = {ks[i].name.decode(): {"launches": ks[i].launches, "ms": ks[i].ms, "work": ks[i].work}
                                    for i in range(nk)}
-            st = (ctypes.c_double * 5)()
+            st = (ctypes.c_double * len(STAGES))()
             L.chz_expand_stages(self._ctx, st)
             out["expand_stages"] = dict(zip(STAGES, [int(v) for v in st]))
         return out

@@ -26,11 +26,12 @@ namespace chz {
 // ------------------------------------------------------------------------------------------------------
 // counters of one level (device): indices into a u64 array
 // C_EVAL children whose residual is formed, C_CHEAP through the structural bounds, C_CUTS cut ranks computed,
-// C_NEEDT T-bound evaluations, C_KEYS class keys: the stage counts of the expansion's work model
+// C_NEEDT T-bound evaluations (their f = 0 parts), C_KEYS class keys, C_TPASS flip-pair passes of the T bound: the
+// stage counts of the expansion's work model
 // C_M, C_NCC: the level's parents and children, written by the host with the level's control upload (read by the
 // kernels of a captured level step, whose launches are sized for a capacity)
 enum { C_W2 = 0, C_W3, C_DEAD, C_FOUND, C_ALIVE, C_OVF, C_NEEDT, C_FIRST, C_RUNS, C_REPOVF, C_KEPT, C_EVAL, C_SURV,
-       C_CHEAP, C_CUTS, C_KEYS, C_M, C_NCC, C_N };
+       C_CHEAP, C_CUTS, C_KEYS, C_TPASS, C_M, C_NCC, C_N };
 // per-instance counters of one level (a batch carries several searches that share the spec's gate set)
 enum { IC_W2 = 0, IC_W3, IC_DEAD, IC_FOUND, IC_ALIVE, IC_OVF, IC_FIRST, IC_N };
 
@@ -109,7 +110,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(WPE))) void
   // tallies, updated outside lane-divergent code, so for n = 3 (one group per wave) they stay wave-uniform and
   // live in scalar registers.
   uint32_t n_w2 = 0, n_w3 = 0, n_dead = 0, n_found = 0, n_ovf = 0, n_alive = 0, n_eval = 0, n_t = 0;
-  uint32_t n_cheap = 0, n_cuts = 0, n_keys = 0;
+  uint32_t n_cheap = 0, n_cuts = 0, n_keys = 0, n_tp = 0;
   int cur = -1;
   auto flush = [&]() {
     if (le == 0 && cur >= 0) {
@@ -165,8 +166,10 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(WPE))) void
         const int v = cx_lower(sp, scx, (pbw & BW_NEEDS) >> 8, sp.cdv[j]);
         c2c |= (uint64_t)(v < 255 ? v : 255) << (8 * j);
       }
-    uint64_t tc_val = 0;          // exact T bounds of this parent's children per T-bound class (tc_ok: known)
-    uint32_t tc_ok = 0;
+    // T bounds of this parent's children per T-bound class: tc_ok bit set when one is known, tc_ex when it is the
+    // exact bound, else it is a lower bound (a transform stopped once the child was dead)
+    uint64_t tc_val = 0;
+    uint32_t tc_ok = 0, tc_ex = 0;
     uint64_t w2m = 0, w3m = 0;
     for (int b0 = 0; b0 < G; b0 += GS) {
       const int g = b0 + le;
@@ -261,12 +264,15 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(WPE))) void
               // before the cut ranks and the T transform are computed.
               const bool tpar = (pbw & BW_TUNK) != BW_TUNK && h.tdel != 255;
               const int Tp = (int)(pbw & BW_TUNK);
-              // a sibling of the same T-bound class already has its exact bound: it is this child's too
+              // a sibling of the same T-bound class already has its exact bound (it is this child's too) or a
+              // lower bound
               const int tc = (h.cls >> 8) - 1;
               const bool tcached = tc >= 0 && ((tc_ok >> tc) & 1);
-              const bool tknown = (tpar && h.tdel == 0) || tcached;
-              int T = tcached ? (int)((tc_val >> (8 * tc)) & 0xff)
-                              : tpar ? (Tp > h.tdel ? Tp - h.tdel : 0) : 0;   // exact if tknown, else a lower bound
+              const int tcv = tcached ? (int)((tc_val >> (8 * tc)) & 0xff) : 0;
+              const bool tcex = tcached && ((tc_ex >> tc) & 1);
+              const bool tknown = (tpar && h.tdel == 0) || tcex;
+              const int Tbr = tpar ? (Tp > h.tdel ? Tp - h.tdel : 0) : 0;
+              int T = tcex ? tcv : (Tbr > tcv ? Tbr : tcv);    // exact if tknown, else a lower bound
               bool texact = tknown;
               const int c2lo = cdcache ? (int)((c2c >> (8 * (h.cls & 0xff))) & 0xff)
                                : !sp.lite && sp.cx_mono && (pbw & BW_CUTS) ? cx_lower(sp, scx, (pbw & BW_NEEDS) >> 8, (uint32_t)h.cdel)
@@ -277,7 +283,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(WPE))) void
                   uint32_t needs = (pbw & BW_NEEDS) >> 8;
                   const uint32_t keepc = (pbw & BW_CUTS) ? (uint32_t)h.loc : 0u;
                   n_cuts += sp.ncuts - __popc(keepc & ((1u << sp.ncuts) - 1));
-                  c2 = cx_bound<NQ>(s, le, sp, scx, scsrc, keepc, needs);
+                  c2 = wuni<NQ>(cx_bound<NQ>(s, le, sp, scx, scsrc, keepc, needs));
                   bw = (needs << 8 & ~BW_CUTS) | (sp.ncuts <= BW_MAXCUTS ? BW_CUTS : 0u) | BW_TUNK;
                 }
                 if (combine_bound(hh, (T + sp.tdiv - 1) / sp.tdiv, c2, supp) > rem) alive = false;
@@ -288,18 +294,42 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(WPE))) void
                 } else if (!texact) {
                   n_t++;
                   if (NQ + k <= 16) {
-                    __builtin_amdgcn_wave_barrier();
-                    t_stage<NQ>(st, s, le);
-                    __builtin_amdgcn_wave_barrier();
-                    T = t_bound16<NQ>(st, s, k, le);
-                    __builtin_amdgcn_wave_barrier();
+                    // the f = 0 part, then the flip-pair passes (odd first) while the lower bound 2(n + k) - v
+                    // leaves the child alive and a remaining pass could still lower v
+                    const int top = 2 * (NQ + k), Tlo = T;
+                    uint32_t aG, aY;
+                    t16_f0<NQ>(s, le, aG, aY);
+                    aG = (uint32_t)wuni<NQ>((int)aG);     // group ORs: uniform over the group
+                    aY = (uint32_t)wuni<NQ>((int)aY);
+                    int v = 2 * ctz16(aG);
+                    const int vy = 1 + 2 * ctz16(aY);
+                    v = vy < v ? vy : v;
+                    T = top - v > Tlo ? top - v : Tlo;
+                    texact = v <= 1;
+                    if (!texact && combine_bound(hh, (T + sp.tdiv - 1) / sp.tdiv, c2, supp) <= rem) {
+                      __builtin_amdgcn_wave_barrier();
+                      t_stage<NQ>(st, s, le);
+                      __builtin_amdgcn_wave_barrier();
+#pragma unroll 1
+                      for (int pass = 1; pass >= 0 && !texact &&
+                                         combine_bound(hh, (T + sp.tdiv - 1) / sp.tdiv, c2, supp) <= rem; pass--) {
+                        const uint32_t acc = (uint32_t)wuni<NQ>((int)t16_pass<NQ>(st, le, pass));
+                        n_tp++;
+                        const int vp = (pass ? 1 : 2) + 2 * ctz16(acc);
+                        v = vp < v ? vp : v;
+                        T = top - v > Tlo ? top - v : Tlo;
+                        texact = pass == 0 || v <= 2;
+                      }
+                      __builtin_amdgcn_wave_barrier();
+                    }
                   } else {
                     T = t_bound_any<NQ>(s, k, le);
+                    texact = true;
                   }
-                  texact = true;
-                  if (tc >= 0 && T < 255) {
-                    tc_val |= (uint64_t)T << (8 * tc);
+                  if (tc >= 0 && T < 255 && (texact || !tcached || T > tcv)) {
+                    tc_val = (tc_val & ~(0xffull << (8 * tc))) | (uint64_t)T << (8 * tc);
                     tc_ok |= 1u << tc;
+                    if (texact) tc_ex |= 1u << tc;
                   }
                   alive = combine_bound(hh, (T + sp.tdiv - 1) / sp.tdiv, c2, supp) <= rem;
                 }
@@ -335,18 +365,20 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(WPE))) void
     __builtin_amdgcn_wave_barrier();
   }
   flush();
-  if (le != 0) n_eval = n_t = n_cheap = n_cuts = n_keys = 0;     // one copy per group
+  if (le != 0) n_eval = n_t = n_cheap = n_cuts = n_keys = n_tp = 0;     // one copy per group
   n_eval = gsum<64>(n_eval);
   n_t = gsum<64>(n_t);
   n_cheap = gsum<64>(n_cheap);
   n_cuts = gsum<64>(n_cuts);
   n_keys = gsum<64>(n_keys);
+  n_tp = gsum<64>(n_tp);
   if (lane == 0) {
     if (n_eval) atomicAdd(&a.cnt[C_EVAL], (unsigned long long)n_eval);
     if (n_t) atomicAdd(&a.cnt[C_NEEDT], (unsigned long long)n_t);
     if (n_cheap) atomicAdd(&a.cnt[C_CHEAP], (unsigned long long)n_cheap);
     if (n_cuts) atomicAdd(&a.cnt[C_CUTS], (unsigned long long)n_cuts);
     if (n_keys) atomicAdd(&a.cnt[C_KEYS], (unsigned long long)n_keys);
+    if (n_tp) atomicAdd(&a.cnt[C_TPASS], (unsigned long long)n_tp);
   }
 }
 
@@ -783,7 +815,8 @@ struct chz_ctx {
   chz::EventLog ev;
   long k_launch[chz::K_NFAM] = {};
   double k_ms[chz::K_NFAM] = {}, k_work[chz::K_NFAM] = {};
-  double k_stage[5] = {};       // expansion stage counts: residuals, structural bounds, cut ranks, T bounds, keys
+  double k_stage[6] = {};       // expansion stage counts: residuals, structural bounds, cut ranks, T bounds, keys,
+                                // T-bound passes
   double avg_terms = 1;         // nonzero terms per inverse column, averaged over the candidates
   bool goal_skip = false;       // every candidate has a goal pre-image: the found-level skip is exact
   // per-level memory planning (chz_batch_set_budget): searches in frontier order; with a budget, a level expands
@@ -831,12 +864,17 @@ static void tree_alloc(chz_ctx *c, long long n, uint32_t *&tp, uint8_t *&tg) {
   tg = (uint8_t *)(tp + std::max<long long>(n, 1));
 }
 
-// 16-bit integer operations of one T-bound evaluation as implemented: for each of the 4^n - 1 flip pairs,
-// E/2 Z[w] products (16 multiplies + 12 adds), a (2n-1)-stage transform of E/2 four-coefficient entries and
-// 6 operations per entry for the final combination; plus the f = 0 transform (E entries, 2 coefficients).
-static double t_ops16(int nq) {
+// Lane operations of the T bound as implemented.  The f = 0 part (32-bit): per entry 12 for |S|^2 = r + y sqrt2
+// and 2 per transform stage for each of r and y.  One flip-pair pass (16-bit, two to a packed lane operation): for
+// each of the 4^n - 1 flip pairs, half of E/2 Z[w] products (16 multiplies + 12 adds), of a (2n-1)-stage
+// transform of E/2 four-coefficient entries and of 6 operations per entry for the final combination.
+static double t_f0_ops(int nq) {
+  const double E = (double)(1 << (2 * nq));
+  return E * 12 + 2 * nq * E * 2;
+}
+static double t_pass_ops(int nq) {
   const double E = (double)(1 << (2 * nq)), H = E / 2;
-  return (E - 1) * (H * 28 + (2 * nq - 1) * H * 4 + H * 6) + E * 12 + 2 * nq * E * 2;
+  return (E - 1) * (H * 28 + (2 * nq - 1) * H * 4 + H * 6) / 2 / 2;
 }
 
 // Algorithmic integer operations of one level's expansion, by stage, in 32-bit lane operations (two 16-bit
@@ -846,7 +884,7 @@ static double t_ops16(int nq) {
 //   structural bounds: per entry 4 for the nonzero test, 5 per qubit for the support comparison;
 //   cut rank: per entry 30 for the F_p image (4 Montgomery products and reductions) and 14 per elimination
 //             step (2 Montgomery products, 1 modular subtraction), stop = 3 steps;
-//   T bound: t_ops16 / 2;
+//   T bound: t_f0_ops per evaluation, t_pass_ops per flip-pair pass it ran;
 //   class key: per entry 20 per entry hash (2), 22 per symmetry (mix and 64-bit add).
 static double expand_ops(const chz_ctx *c, int nq, const unsigned long long *hc) {
   const double E = (double)(1 << (2 * nq));
@@ -855,7 +893,7 @@ static double expand_ops(const chz_ctx *c, int nq, const unsigned long long *hc)
   // and two 32-bit products with their shifts and xors, 13 ops) and its two-word share of the 64-bit sums
   const double key = E * (2 * 16.0 + 15.0 * c->ks.nsym);
   return (double)hc[C_EVAL] * child + (double)hc[C_CHEAP] * cheap + (double)hc[C_CUTS] * cut +
-         (double)hc[C_NEEDT] * t_ops16(nq) / 2 + (double)hc[C_KEYS] * key;
+         (double)hc[C_NEEDT] * t_f0_ops(nq) + (double)hc[C_TPASS] * t_pass_ops(nq) + (double)hc[C_KEYS] * key;
 }
 
 
@@ -1199,8 +1237,8 @@ template <int NQ> static void batch_step(chz_ctx *c) {
     nsurv = (long long)hc[C_SURV];
     distinct = (long long)*ptot;
     c->k_work[K_STEP] += expand_ops(c, NQ, hc) + (double)(m * G) + (double)nsurv * (24.0 + 16.0 + 4 * 20.0 + 9.0);
-    static const int stage_ctr[5] = {C_EVAL, C_CHEAP, C_CUTS, C_NEEDT, C_KEYS};
-    for (int i = 0; i < 5; i++) c->k_stage[i] += (double)hc[stage_ctr[i]];
+    static const int stage_ctr[6] = {C_EVAL, C_CHEAP, C_CUTS, C_NEEDT, C_KEYS, C_TPASS};
+    for (int i = 0; i < 6; i++) c->k_stage[i] += (double)hc[stage_ctr[i]];
     for (const BInst &b : c->binst)
       if (b.live) {
         const unsigned long long *x = &ic[(size_t)b.slot * IC_N];
@@ -1268,8 +1306,8 @@ template <int NQ> static void batch_step(chz_ctx *c) {
   nsurv = (long long)hc[C_SURV];
   c->k_work[K_COMPACT] += (double)nsurv * 24.0;
   c->k_work[K_EXPAND] += expand_ops(c, NQ, hc);
-  static const int stage_ctr[5] = {C_EVAL, C_CHEAP, C_CUTS, C_NEEDT, C_KEYS};
-  for (int i = 0; i < 5; i++) c->k_stage[i] += (double)hc[stage_ctr[i]];
+  static const int stage_ctr[6] = {C_EVAL, C_CHEAP, C_CUTS, C_NEEDT, C_KEYS, C_TPASS};
+  for (int i = 0; i < 6; i++) c->k_stage[i] += (double)hc[stage_ctr[i]];
   // searches that end at this level with a circuit or an overflow; the rest continue if their budget allows
   for (const BInst &b : c->binst)
     if (b.live) {
@@ -1960,8 +1998,8 @@ unsigned chz_time_kernels(chz_ctx *c, unsigned families) {
 }
 
 int chz_expand_stages(const chz_ctx *c, double *out) {
-  for (int i = 0; i < 5; i++) out[i] = c->k_stage[i];
-  return 5;
+  for (int i = 0; i < 6; i++) out[i] = c->k_stage[i];
+  return 6;
 }
 
 int chz_kernel_stats(const chz_ctx *c, chz_kstat *out, int n) {

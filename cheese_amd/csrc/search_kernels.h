@@ -317,70 +317,82 @@ __device__ __forceinline__ int ctz16(uint32_t v) { return v & 0xffffu ? __ffs((i
 // f != 0, with j the top bit of f and R = {e : e_j = 0}, the 2-D transform is
 //   W_f[z] = A[z] +- conj(A[z]),   A[z] = sum_{e in R} (-1)^{e.z} Q_f[e]   (a transform over the other bits),
 // and A + conj A = 2a + (b - d) sqrt2, A - conj A = i (2c + (b + d) sqrt2) for A = (a, b, c, d).  An element
-// x + y sqrt2 has valuation min(2 v2(x), 2 v2(y) + 1), so the smallest valuation over all outputs is read
-// from three OR accumulators.  f = 0 (Q_0 = |S|^2, real) is transformed by the whole group, one lane per entry.
-// Lane f of the group handles flip pair f (lane 0 repeats f = 1; OR-accumulation is idempotent).
-template <int NQ> __device__ __forceinline__ int t_bound16(TStage st, int4 s, int k, int le) {
+// x + y sqrt2 has valuation min(2 v2(x), 2 v2(y) + 1), so the smallest valuation v over all outputs is the
+// minimum of three parts, each read from an OR accumulator:
+//   f = 0 (Q_0 = |S|^2 = r + y sqrt2, real; transformed by the whole group, one lane per entry): 2 v2(r),
+//       1 + 2 v2(y)                                                                      (t16_f0)
+//   pass 1, (b - d | b + d) of every A:  1 + 2 v2, at least 1                           (t16_pass)
+//   pass 0, (a | c) of every A:          2 + 2 v2, at least 2
+// and the bound is 2(n + k) - v.  Each part only lowers v, so 2(n + k) minus the minimum over the parts computed
+// so far is a lower bound on the T bound, exact once no remaining part can go below it.  A pass is the costly
+// part (E/2 Z[w] products and a transform per lane); the expansion runs them only for children the lower bound
+// leaves undecided, the odd pass first (it sets the minimum far more often).
+
+// one butterfly stage of a Walsh-Hadamard transform over the wave's 64 lanes, on two values
+template <int H> __device__ __forceinline__ void wht_stage(uint32_t &r, uint32_t &y, int le) {
+  const uint32_t ro = xor64<H>(r), yo = xor64<H>(y);
+  if (le & H) { r = ro - r; y = yo - y; } else { r = r + ro; y = y + yo; }
+}
+
+// f = 0: group ORs of the transform's outputs, r (low 16 bits) into accG and y into accY
+template <int NQ> __device__ __forceinline__ void t16_f0(int4 s, int le, uint32_t &accG, uint32_t &accY) {
+  constexpr int GS = 1 << (2 * NQ);
+  // |S[e]|^2 = r + y sqrt2 with r = a^2 + b^2 + c^2 + d^2, y = ab + bc + cd - da
+  uint32_t r = (uint32_t)(s.x * s.x + s.y * s.y + s.z * s.z + s.w * s.w);
+  uint32_t y = (uint32_t)(s.x * s.y + s.y * s.z + s.z * s.w - s.w * s.x);
+  if constexpr (GS == 64) {
+    wht_stage<1>(r, y, le);
+    wht_stage<2>(r, y, le);
+    wht_stage<4>(r, y, le);
+    wht_stage<8>(r, y, le);
+    wht_stage<16>(r, y, le);
+    wht_stage<32>(r, y, le);
+  } else {
+#pragma unroll
+    for (int h = 1; h < GS; h <<= 1) {
+      const uint32_t ro = __shfl_xor(r, h, GS), yo = __shfl_xor(y, h, GS);
+      if (le & h) { r = ro - r; y = yo - y; } else { r = r + ro; y = y + yo; }
+    }
+  }
+  accG = gor<GS>(r & 0xffffu);
+  accY = gor<GS>(y & 0xffffu);
+}
+
+// One pass over the flip pairs, from the operands t_stage left in LDS: pass 0 transforms (Qa | Qc), pass 1
+// (Qb - Qd | Qb + Qd).  Lane f of the group handles flip pair f (lane 0 repeats f = 1; OR-accumulation is
+// idempotent).  Returns the group OR of every output's 16-bit halves.
+template <int NQ> __device__ __forceinline__ uint32_t t16_pass(TStage st, int le, int pass) {
   constexpr int E = 1 << (2 * NQ), H = E / 2, GS = E;
   int f = le == 0 ? 1 : le;
   // keep the per-lane LDS address arithmetic inside the child loop: hoisted, its 64 addresses would
   // halve the kernel's occupancy
   asm volatile("" : "+v"(f));
   const int j = 31 - __clz(f);
-  const int lowm = (1 << j) - 1;
-  // pass 0 transforms (Qa | Qc), pass 1 (Qb - Qd | Qb + Qd); one pass at a time keeps H packed values live.
-  // The two results are kept in named registers: an array indexed by the pass would live in scratch memory.
-  uint32_t ev = 0, od = 0;
-#pragma unroll 1
-  for (int pass = 0; pass < 2; pass++) {
-    const uint4 *pm = st.p + pass * TS_STRIDE;
-    // the addresses below depend on a value the compiler cannot see through, so they are formed per pass
-    // instead of being hoisted out of the pass loop (64 more live registers)
-    int fp = f, lm = lowm;
-    asm volatile("" : "+v"(fp), "+v"(lm));
-    u16x2 q[H];
+  int lm = (1 << j) - 1;
+  asm volatile("" : "+v"(lm));
+  const uint4 *pm = st.p + pass * TS_STRIDE;
+  u16x2 q[H];
 #pragma unroll
-    for (int i = 0; i < H; i++) {
-      const int e = ((i & ~lm) << 1) | (i & lm);         // i with a 0 inserted at bit j
-      const uint2 xv = st.x[e ^ fp];
-      const u16x2 x01 = as16(xv.x), x23 = as16(xv.y);
-      const uint4 m = pm[e];
-      q[i] = x01.xx * as16(m.x) + x01.yy * as16(m.y) + x23.xx * as16(m.z) + x23.yy * as16(m.w);
-    }
-#pragma unroll
-    for (int h = 1; h < H; h <<= 1)
-#pragma unroll
-      for (int i = 0; i < H; i++)
-        if (!(i & h)) {
-          const u16x2 u = q[i], v = q[i | h];
-          q[i] = u + v;
-          q[i | h] = u - v;
-        }
-    uint32_t o = 0;
-#pragma unroll
-    for (int i = 0; i < H; i++) o |= bits(q[i]);
-    const uint32_t folded = (o & 0xffffu) | (o >> 16);
-    if (pass == 0) ev = folded;                // a, c (even part)
-    else od = folded;                          // b - d, b + d (odd part)
+  for (int i = 0; i < H; i++) {
+    const int e = ((i & ~lm) << 1) | (i & lm);         // i with a 0 inserted at bit j
+    const uint2 xv = st.x[e ^ f];
+    const u16x2 x01 = as16(xv.x), x23 = as16(xv.y);
+    const uint4 m = pm[e];
+    q[i] = x01.xx * as16(m.x) + x01.yy * as16(m.y) + x23.xx * as16(m.z) + x23.yy * as16(m.w);
   }
-  // f = 0: |S[e]|^2 = r + y sqrt2 with r = a^2 + b^2 + c^2 + d^2, y = ab + bc + cd - da
-  uint32_t r = (uint32_t)(s.x * s.x + s.y * s.y + s.z * s.z + s.w * s.w);
-  uint32_t y = (uint32_t)(s.x * s.y + s.y * s.z + s.z * s.w - s.w * s.x);
 #pragma unroll
-  for (int h = 1; h < GS; h <<= 1) {
-    const uint32_t ro = __shfl_xor(r, h, GS), yo = __shfl_xor(y, h, GS);
-    if (le & h) { r = ro - r; y = yo - y; } else { r = r + ro; y = y + yo; }
-  }
-  uint32_t accE = ev, accF = od | (y & 0xffffu), accG = r & 0xffffu;
-  accE = gor<GS>(accE);
-  accF = gor<GS>(accF);
-  accG = gor<GS>(accG);
-  int v = 2 + 2 * ctz16(accE);
-  const int vf = 1 + 2 * ctz16(accF), vg = 2 * ctz16(accG);
-  v = vf < v ? vf : v;
-  v = vg < v ? vg : v;
-  const int need = 2 * (NQ + k) - v;
-  return need > 0 ? need : 0;
+  for (int h = 1; h < H; h <<= 1)
+#pragma unroll
+    for (int i = 0; i < H; i++)
+      if (!(i & h)) {
+        const u16x2 u = q[i], v = q[i | h];
+        q[i] = u + v;
+        q[i | h] = u - v;
+      }
+  uint32_t o = 0;
+#pragma unroll
+  for (int i = 0; i < H; i++) o |= bits(q[i]);
+  return gor<GS>((o & 0xffffu) | (o >> 16));
 }
 
 template <int NQ> __device__ __forceinline__ int t_bound_any(int4 s, int k, int le) {
@@ -460,8 +472,6 @@ __device__ __forceinline__ uint64_t partner16(uint64_t x, int lane) {       // x
   const auto hi = __builtin_amdgcn_permlane16_swap((uint32_t)(x >> 32), (uint32_t)(x >> 32), false, false);
   return (lane & 16) ? u64_of(lo[0], hi[0]) : u64_of(lo[1], hi[1]);
 }
-// DPP controls: row mirror (lane ^ 15 within a 16-lane row), half-row mirror (lane ^ 7), quad permutations
-constexpr int DPP_MIRROR = 0x140, DPP_HALF_MIRROR = 0x141, DPP_QXOR1 = 0xB1, DPP_QXOR2 = 0x4E;
 template <int CTRL> __device__ __forceinline__ uint64_t dpp64(uint64_t v) {
   const uint32_t lo = (uint32_t)__builtin_amdgcn_update_dpp(0, (int)(uint32_t)v, CTRL, 0xf, 0xf, false);
   const uint32_t hi = (uint32_t)__builtin_amdgcn_update_dpp(0, (int)(uint32_t)(v >> 32), CTRL, 0xf, 0xf, false);

@@ -105,7 +105,8 @@ unsigned chz_time_kernels(chz_ctx *ctx, unsigned families);
 
 /* Stage counts of the last run's expansion (no reference counterpart; for measurement), the factors of its work
  * model: out[0] residuals formed, [1] children through the structural bounds, [2] cut ranks computed, [3] T-bound
- * evaluations, [4] class keys.  Returns 5. */
+ * evaluations, [4] class keys, [5] flip-pair passes of the T-bound evaluations (0 to 2 each: a pass runs only while
+ * the child is undecided).  out holds 6 values; returns 6. */
 int chz_expand_stages(const chz_ctx *ctx, double *out);
 
 /* Autotuner (no reference counterpart): times every candidate expansion schedule (occupancy variant x grid cap x

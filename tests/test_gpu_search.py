@@ -91,6 +91,8 @@ def test_expansion_stage_counts_add_up():
     assert st["structural"] <= st["residuals"]
     assert 0 < st["t_bounds"] < st["structural"]
     assert 0 < st["cut_ranks"] <= 3 * st["structural"]
+    # a T-bound evaluation runs at most both flip-pair passes, and the lower bounds settle most children early
+    assert 0 < st["t_passes"] < 2 * st["t_bounds"]
 
 
 def test_table_size_within_seven_gates():

@@ -394,7 +394,8 @@ def main():
                      "kernel": "k_level", "kernel_ms_per_step": round(tb["ms"] / a.steps, 3),
                      "launches_per_step": tb["launches"] / a.steps,
                      "work": "algorithmic 32-bit integer lane ops: stage counts x per-stage ops (residual, "
-                             "structural bounds, cut ranks, T transform at two 16-bit ops per lane op, class keys)",
+                             "structural bounds, cut ranks, T bound: its f = 0 part per evaluation and each "
+                             "flip-pair pass it ran at two 16-bit ops per lane op, class keys)",
                      "stages_per_step": {k: v // a.steps for k, v in stages.items()},
                      "valu_busy_profiled": valu_busy},
         "kernel_launches_per_step": {k: v["launches"] / a.steps for k, v in kstats.items() if v["launches"]},

@@ -76,6 +76,9 @@ SIGNATURES = {
     "chz_batch_nodes": (ctypes.c_longlong, [ctypes.c_void_p]),
     "chz_batch_set_budget": (None, [ctypes.c_void_p, ctypes.c_double]),
     "chz_batch_waits": (ctypes.c_longlong, [ctypes.c_void_p]),
+    "chz_batch_frontier": (ctypes.c_longlong, [ctypes.c_void_p, ctypes.c_longlong, ctypes.c_longlong,
+                                               ctypes.POINTER(ctypes.c_int), ctypes.POINTER(ctypes.c_int),
+                                               ctypes.POINTER(ctypes.c_uint32), ctypes.POINTER(ctypes.c_int)]),
 }
 
 
@@ -326,6 +329,18 @@ class Batch:
         ks = (KStat * 16)()
         nk = lib().chz_kernel_stats(self._ctx, ks, 16)
         return {ks[i].name.decode(): {"launches": ks[i].launches, "ms": ks[i].ms, "work": ks[i].work} for i in range(nk)}
+
+    def frontier(self, max_nodes: int, stride: int = 1):
+        """every stride-th node of the current frontier, at most max_nodes (chz_batch_frontier): a list of
+        (tag, candidate indices from the root, bound word)"""
+        paths = (ctypes.c_int * (max_nodes * DMAX))()
+        lens = (ctypes.c_int * max_nodes)()
+        bw = (ctypes.c_uint32 * max_nodes)()
+        tags = (ctypes.c_int * max_nodes)()
+        n = lib().chz_batch_frontier(self._ctx, max_nodes, stride, paths, lens, bw, tags)
+        if n < 0:
+            raise SearchError(5, lib().chz_last_error().decode())
+        return [(tags[i], [paths[i * DMAX + t] for t in range(lens[i])], bw[i]) for i in range(n)]
 
     @property
     def waits(self) -> int:

@@ -1957,6 +1957,46 @@ void chz_batch_set_budget(chz_ctx *c, double bytes) { c->mem_budget = bytes > 0 
 
 long long chz_batch_waits(const chz_ctx *c) { return c->carried_levels; }
 
+long long chz_batch_frontier(chz_ctx *c, long long max, long long stride, int *paths, int *lens, uint32_t *bw,
+                             int *tags) {
+  if (stride < 1) stride = 1;
+  long long n = 0;
+  try {
+    HIPCHK(hipSetDevice(c->device));
+    chz::StreamScope scope(c->stream);
+    HIPCHK(hipStreamSynchronize(c->stream));
+    std::vector<int> seq;
+    for (long long j = 0; j < c->m && n < max; j += stride, n++) {
+      uint32_t w = 0, slot = 0;
+      chz::copy_sync(&w, c->pb[c->cur].p + j, 4, hipMemcpyDeviceToHost, c->stream);
+      chz::copy_sync(&slot, c->finst[c->cur].p + j, 4, hipMemcpyDeviceToHost, c->stream);
+      // the node's gates from its root: the tree walked up from the current level, as walk_circuit does
+      seq.clear();
+      uint32_t i = (uint32_t)j;
+      for (int s = c->gstep; s >= 0 && c->tree_par[s]; s--) {
+        uint32_t par;
+        uint8_t g;
+        chz::copy_sync(&par, c->tree_par[s] + i, 4, hipMemcpyDeviceToHost, c->stream);
+        if (par == chz::NO_PARENT) break;
+        chz::copy_sync(&g, c->tree_gate[s] + i, 1, hipMemcpyDeviceToHost, c->stream);
+        if (g != chz::CARRY_GATE) seq.push_back(g);
+        i = par;
+      }
+      std::reverse(seq.begin(), seq.end());
+      const int len = (int)std::min<size_t>(seq.size(), CHZ_DMAX);
+      for (int t = 0; t < len; t++) paths[(size_t)n * CHZ_DMAX + t] = seq[t];
+      lens[n] = len;
+      bw[n] = w;
+      tags[n] = -1;
+      for (const chz::BInst &b : c->binst)
+        if (b.live && (uint32_t)b.slot == slot) tags[n] = b.tag;
+    }
+  } catch (chz::DeviceError &) {
+    return -1;
+  }
+  return n;
+}
+
 int chz_batch_events(chz_ctx *c, chz_event *out, int max) {
   int k = 0;
   while (k < max && !c->events.empty()) { out[k++] = c->events.front(); c->events.pop_front(); }

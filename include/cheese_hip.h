@@ -149,6 +149,13 @@ long long chz_batch_nodes(const chz_ctx *ctx);  /* frontier nodes held on the de
  * carried to the next level unexpanded (their level lines and results are unchanged, only later).  0: no limit. */
 void chz_batch_set_budget(chz_ctx *ctx, double bytes);
 long long chz_batch_waits(const chz_ctx *ctx);  /* search-levels that waited so far */
+/* Test access to the frontier between steps (no reference counterpart; for tests): up to max nodes of the
+ * batch's current frontier, every stride-th from the first.  Per node: its gates from its search's root
+ * (paths[i * CHZ_DMAX + t], lens[i] of them), its bound word (bits 0-7 the exact T bound or 255 when unknown,
+ * bits 8 + 3c the need of cut c, bit 31 set when the needs are known) and its search's tag.  Returns the number
+ * of nodes written, -1 on a device error. */
+long long chz_batch_frontier(chz_ctx *ctx, long long max, long long stride, int *paths, int *lens, uint32_t *bw,
+                             int *tags);
 
 /* ---------------- Clifford-group table (src/clifford.cu) ----------------
  * The whole n-qubit Clifford group (n <= 3, up to global phase) tabulated by breadth-first search over the gate

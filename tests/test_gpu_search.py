@@ -132,3 +132,59 @@ def test_table_and_join_find_circuits_of_the_plain_length(seed, mode):
     assert plain["rc"] == other["rc"] == 0
     assert other["verified"]
     assert len(other["circuit"]) == len(plain["circuit"])
+
+
+def _cuts(spec):
+    """(qubit mask, radix) of each cut of a .spec: its `cuts` section lists the size and the qubits of one side"""
+    lines = spec.splitlines()
+    i = next(j for j, l in enumerate(lines) if l.startswith("cuts "))
+    out = []
+    for l in lines[i + 1:i + 1 + int(lines[i].split()[1])]:
+        f = [int(x) for x in l.split()]
+        out.append((sum(1 << q for q in f[1:1 + f[0]]), 2 * f[0] + 1))
+    return out
+
+
+@pytest.mark.parametrize("name", ["toffoli-d12", "random3-s5"])
+def test_frontier_bound_words_equal_the_oracle(oracle_lib, name):
+    """the bound word every frontier node carries holds the oracle's exact values for the node's own residual (the
+    residual of its gate path from the target): its T bound (src/cheese.cu:397-449) wherever the word says it is
+    known, whether inherited from the parent, shared within a T-bound class or settled by the staged transform, and
+    each cut's need, the clipped ceil log2 of the F_p operator-Schmidt rank (:270-303, :585-597), wherever the
+    needs are known; sampled after every level"""
+    import ctypes
+    L = oracle_lib
+    spec = read_spec(name)
+    raw = spec.encode()
+    cuts = _cuts(spec)
+    b = cheese_amd.Batch(spec)
+    checked_t = checked_c = 0
+    try:
+        assert b.add(spec, 0) == 0
+        for level in range(1, 16):
+            b.events()
+            if not b.live:
+                break
+            assert b.step() == 0
+            if not b.live or b.nodes == 0:
+                break
+            for tag, path, w in b.frontier(120, max(1, b.nodes // 120)):
+                assert tag == 0
+                e = (ctypes.c_longlong * (4 * 64))()
+                arr = (ctypes.c_int * max(1, len(path)))(*path)
+                k = L.orc_residual(raw, len(raw), arr, len(path), e)
+                assert k >= 0, path
+                if w & 0xff != 0xff:
+                    assert w & 0xff == L.orc_bound(3, k, e, 0, 0), (level, path, w)
+                    checked_t += 1
+                if w >> 31:
+                    for c, (mask, rad) in enumerate(cuts):
+                        rank, need = L.orc_bound(3, k, e, 4, mask), 0
+                        while (1 << need) < rank:
+                            need += 1
+                        assert (w >> (8 + 3 * c)) & 7 == min(need, rad - 1), (level, path, c, w)
+                    checked_c += 1
+    finally:
+        b.close()
+    print(f"case {name}: {checked_t} T bounds and {checked_c} cut-need sets equal to the oracle's", flush=True)
+    assert checked_t >= 200 and checked_c >= 200

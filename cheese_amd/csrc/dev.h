@@ -75,36 +75,11 @@ template <int GS> __device__ __forceinline__ bool gany(bool p) { return gballot<
 // ---- whole-wave exchanges without the LDS pipeline (gfx950): DPP row controls and permlane16/32 swaps.  They
 // need no per-lane address registers (ds_bpermute does), which keeps the expansion kernel's register pressure
 // down.  Every lane of the wave must be active.
-// DPP controls: quad permutations (lane ^ 1, ^ 2, ^ 3 within a quad), half-row mirror (lane ^ 7 within 8 lanes),
-// row mirror (lane ^ 15 within a 16-lane row), row rotation by 8 (lane ^ 8 within a row)
-constexpr int DPP_QXOR1 = 0xB1, DPP_QXOR2 = 0x4E, DPP_QXOR3 = 0x1B, DPP_HALF_MIRROR = 0x141, DPP_MIRROR = 0x140,
-              DPP_ROR8 = 0x128;
+// DPP controls: quad permutations (lane ^ 1, ^ 2 within a quad), half-row mirror (lane ^ 7 within 8 lanes), row
+// mirror (lane ^ 15 within a 16-lane row)
+constexpr int DPP_QXOR1 = 0xB1, DPP_QXOR2 = 0x4E, DPP_HALF_MIRROR = 0x141, DPP_MIRROR = 0x140;
 template <int CTRL> __device__ __forceinline__ uint32_t dpp32(uint32_t v) {
   return (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, CTRL, 0xf, 0xf, false);
-}
-// v_permlane16_swap(x, x) returns rows (0, 0, 2, 2) and (1, 1, 3, 3) of x; v_permlane32_swap(x, x) halves (0, 0)
-// and (1, 1)
-__device__ __forceinline__ uint32_t xor16_64(uint32_t v) {          // v[lane ^ 16]
-  const auto p = __builtin_amdgcn_permlane16_swap(v, v, false, false);
-  return (__lane_id() & 16) ? p[0] : p[1];
-}
-__device__ __forceinline__ uint32_t xor32_64(uint32_t v) {          // v[lane ^ 32]
-  const auto p = __builtin_amdgcn_permlane32_swap(v, v, false, false);
-  return (__lane_id() & 32) ? p[0] : p[1];
-}
-// v[lane ^ M] for a constant M: lane ^ 4 = (lane ^ 7) ^ 3, so bits 0-2 take at most two DPP moves
-template <int M> __device__ __forceinline__ uint32_t xor64(uint32_t v) {
-  static_assert(M > 0 && M < 64, "lane mask");
-  constexpr int lo = M & 7;
-  if constexpr (lo >= 4 && lo != 7) v = dpp32<DPP_HALF_MIRROR>(v);
-  if constexpr (lo == 1 || lo == 6) v = dpp32<DPP_QXOR1>(v);
-  else if constexpr (lo == 2 || lo == 5) v = dpp32<DPP_QXOR2>(v);
-  else if constexpr (lo == 3 || lo == 4) v = dpp32<DPP_QXOR3>(v);
-  else if constexpr (lo == 7) v = dpp32<DPP_HALF_MIRROR>(v);
-  if constexpr ((M & 8) != 0) v = dpp32<DPP_ROR8>(v);
-  if constexpr ((M & 16) != 0) v = xor16_64(v);
-  if constexpr ((M & 32) != 0) v = xor32_64(v);
-  return v;
 }
 // all-reduce over the wave of a commutative operation: each step pairs every lane with one of another group whose
 // lanes already agree (quads, then 8, 16 and 32 lanes), so the pairing need not be a lane xor
@@ -113,6 +88,8 @@ template <typename Op> __device__ __forceinline__ uint32_t wave_reduce32(uint32_
   v = op(v, dpp32<DPP_QXOR2>(v));
   v = op(v, dpp32<DPP_HALF_MIRROR>(v));
   v = op(v, dpp32<DPP_MIRROR>(v));
+  // v_permlane16_swap(v, v) returns rows (0, 0, 2, 2) and (1, 1, 3, 3) of v; v_permlane32_swap(v, v) the halves
+  // (0, 0) and (1, 1)
   const auto p = __builtin_amdgcn_permlane16_swap(v, v, false, false);
   v = op(p[0], p[1]);
   const auto q = __builtin_amdgcn_permlane32_swap(v, v, false, false);
@@ -185,6 +162,14 @@ __device__ __forceinline__ int combine_bound(int h, int t, int c2, int supp) {  
     if (c + one < best) best = c + one;
   }
   return best;
+}
+
+// The largest t with combine_bound(h, t, c2, supp) <= rem (negative: none), for rem >= 0.  The bound is at most
+// rem exactly when some c in [c2, max(c2, supp)] has c + h + t <= rem and supp - c <= rem; the smallest such c is
+// max(c2, supp - rem), which never exceeds max(c2, supp).
+__device__ __forceinline__ int t_budget(int h, int c2, int supp, int rem) {
+  const int c = c2 > supp - rem ? c2 : supp - rem;
+  return rem - h - c;
 }
 
 __device__ __forceinline__ uint64_t mix64(uint64_t x) {

@@ -294,34 +294,11 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(WPE))) void
                 } else if (!texact) {
                   n_t++;
                   if (NQ + k <= 16) {
-                    // the f = 0 part, then the flip-pair passes (odd first) while the lower bound 2(n + k) - v
-                    // leaves the child alive and a remaining pass could still lower v
-                    const int top = 2 * (NQ + k), Tlo = T;
-                    uint32_t aG, aY;
-                    t16_f0<NQ>(s, le, aG, aY);
-                    aG = (uint32_t)wuni<NQ>((int)aG);     // group ORs: uniform over the group
-                    aY = (uint32_t)wuni<NQ>((int)aY);
-                    int v = 2 * ctz16(aG);
-                    const int vy = 1 + 2 * ctz16(aY);
-                    v = vy < v ? vy : v;
-                    T = top - v > Tlo ? top - v : Tlo;
-                    texact = v <= 1;
-                    if (!texact && combine_bound(hh, (T + sp.tdiv - 1) / sp.tdiv, c2, supp) <= rem) {
-                      __builtin_amdgcn_wave_barrier();
-                      t_stage<NQ>(st, s, le);
-                      __builtin_amdgcn_wave_barrier();
-#pragma unroll 1
-                      for (int pass = 1; pass >= 0 && !texact &&
-                                         combine_bound(hh, (T + sp.tdiv - 1) / sp.tdiv, c2, supp) <= rem; pass--) {
-                        const uint32_t acc = (uint32_t)wuni<NQ>((int)t16_pass<NQ>(st, le, pass));
-                        n_tp++;
-                        const int vp = (pass ? 1 : 2) + 2 * ctz16(acc);
-                        v = vp < v ? vp : v;
-                        T = top - v > Tlo ? top - v : Tlo;
-                        texact = pass == 0 || v <= 2;
-                      }
-                      __builtin_amdgcn_wave_barrier();
-                    }
+                    // the f = 0 part, then the flip-pair passes (odd first) while the lower bound leaves the child
+                    // alive: at T bound T it is alive exactly when T <= tmax
+                    uint32_t np = 0;
+                    T = t_bound16s<NQ>(st, s, k, le, T, t_budget(hh, c2, supp, rem) * sp.tdiv, np, texact);
+                    n_tp += np;
                   } else {
                     T = t_bound_any<NQ>(s, k, le);
                     texact = true;

@@ -320,79 +320,89 @@ __device__ __forceinline__ int ctz16(uint32_t v) { return v & 0xffffu ? __ffs((i
 // x + y sqrt2 has valuation min(2 v2(x), 2 v2(y) + 1), so the smallest valuation v over all outputs is the
 // minimum of three parts, each read from an OR accumulator:
 //   f = 0 (Q_0 = |S|^2 = r + y sqrt2, real; transformed by the whole group, one lane per entry): 2 v2(r),
-//       1 + 2 v2(y)                                                                      (t16_f0)
-//   pass 1, (b - d | b + d) of every A:  1 + 2 v2, at least 1                           (t16_pass)
+//       1 + 2 v2(y)
+//   pass 1, (b - d | b + d) of every A:  1 + 2 v2, at least 1
 //   pass 0, (a | c) of every A:          2 + 2 v2, at least 2
 // and the bound is 2(n + k) - v.  Each part only lowers v, so 2(n + k) minus the minimum over the parts computed
 // so far is a lower bound on the T bound, exact once no remaining part can go below it.  A pass is the costly
 // part (E/2 Z[w] products and a transform per lane); the expansion runs them only for children the lower bound
 // leaves undecided, the odd pass first (it sets the minimum far more often).
 
-// one butterfly stage of a Walsh-Hadamard transform over the wave's 64 lanes, on two values
-template <int H> __device__ __forceinline__ void wht_stage(uint32_t &r, uint32_t &y, int le) {
-  const uint32_t ro = xor64<H>(r), yo = xor64<H>(y);
-  if (le & H) { r = ro - r; y = yo - y; } else { r = r + ro; y = y + yo; }
-}
-
-// f = 0: group ORs of the transform's outputs, r (low 16 bits) into accG and y into accY
-template <int NQ> __device__ __forceinline__ void t16_f0(int4 s, int le, uint32_t &accG, uint32_t &accY) {
-  constexpr int GS = 1 << (2 * NQ);
-  // |S[e]|^2 = r + y sqrt2 with r = a^2 + b^2 + c^2 + d^2, y = ab + bc + cd - da
+// The staged T bound: the f = 0 part, then pass 1, then pass 0, each only while the lower bound
+// max(Tlo, 2(n + k) - v) stays at most tmax (the largest T bound at which the child is alive) and a remaining part
+// could still lower v.  Tlo is a lower bound the caller already has (the parent's bracket, a sibling's value).
+// Returns the bound reached; exact tells whether it is the T bound (always so when it is at most tmax).  npass
+// counts the passes run.  One function with the pass loop inside keeps the pass's LDS reads interleaved with its
+// products; split into separate helpers the compiler issued all 64 reads of a pass at once and spilled.
+template <int NQ> __device__ __forceinline__ int t_bound16s(TStage st, int4 s, int k, int le, int Tlo, int tmax,
+                                                             uint32_t &npass, bool &exact) {
+  constexpr int E = 1 << (2 * NQ), H = E / 2, GS = E;
+  const int top = 2 * (NQ + k);
+  // f = 0: |S[e]|^2 = r + y sqrt2 with r = a^2 + b^2 + c^2 + d^2, y = ab + bc + cd - da
   uint32_t r = (uint32_t)(s.x * s.x + s.y * s.y + s.z * s.z + s.w * s.w);
   uint32_t y = (uint32_t)(s.x * s.y + s.y * s.z + s.z * s.w - s.w * s.x);
-  if constexpr (GS == 64) {
-    wht_stage<1>(r, y, le);
-    wht_stage<2>(r, y, le);
-    wht_stage<4>(r, y, le);
-    wht_stage<8>(r, y, le);
-    wht_stage<16>(r, y, le);
-    wht_stage<32>(r, y, le);
-  } else {
 #pragma unroll
-    for (int h = 1; h < GS; h <<= 1) {
-      const uint32_t ro = __shfl_xor(r, h, GS), yo = __shfl_xor(y, h, GS);
-      if (le & h) { r = ro - r; y = yo - y; } else { r = r + ro; y = y + yo; }
-    }
+  for (int h = 1; h < GS; h <<= 1) {
+    const uint32_t ro = __shfl_xor(r, h, GS), yo = __shfl_xor(y, h, GS);
+    if (le & h) { r = ro - r; y = yo - y; } else { r = r + ro; y = y + yo; }
   }
-  accG = gor<GS>(r & 0xffffu);
-  accY = gor<GS>(y & 0xffffu);
-}
-
-// One pass over the flip pairs, from the operands t_stage left in LDS: pass 0 transforms (Qa | Qc), pass 1
-// (Qb - Qd | Qb + Qd).  Lane f of the group handles flip pair f (lane 0 repeats f = 1; OR-accumulation is
-// idempotent).  Returns the group OR of every output's 16-bit halves.
-template <int NQ> __device__ __forceinline__ uint32_t t16_pass(TStage st, int le, int pass) {
-  constexpr int E = 1 << (2 * NQ), H = E / 2, GS = E;
+  // group ORs are uniform over the group: read through readfirstlane (n = 3) they stay in scalar registers
+  const uint32_t accG = (uint32_t)wuni<NQ>((int)gor<GS>(r & 0xffffu));
+  const uint32_t accY = (uint32_t)wuni<NQ>((int)gor<GS>(y & 0xffffu));
+  int v = 2 * ctz16(accG);
+  const int vy = 1 + 2 * ctz16(accY);
+  v = vy < v ? vy : v;
+  int T = top - v > Tlo ? top - v : Tlo;
+  exact = v <= 1;
+  if (exact || T > tmax) return T;
+  __builtin_amdgcn_wave_barrier();
+  t_stage<NQ>(st, s, le);
+  __builtin_amdgcn_wave_barrier();
+  // lane f of the group handles flip pair f (lane 0 repeats f = 1; OR-accumulation is idempotent)
   int f = le == 0 ? 1 : le;
   // keep the per-lane LDS address arithmetic inside the child loop: hoisted, its 64 addresses would
   // halve the kernel's occupancy
   asm volatile("" : "+v"(f));
   const int j = 31 - __clz(f);
-  int lm = (1 << j) - 1;
-  asm volatile("" : "+v"(lm));
-  const uint4 *pm = st.p + pass * TS_STRIDE;
-  u16x2 q[H];
+  const int lowm = (1 << j) - 1;
+#pragma unroll 1
+  for (int pass = 1; pass >= 0; pass--) {
+    const uint4 *pm = st.p + pass * TS_STRIDE;
+    // the addresses below depend on a value the compiler cannot see through, so they are formed per pass
+    // instead of being hoisted out of the pass loop (64 more live registers)
+    int fp = f, lm = lowm;
+    asm volatile("" : "+v"(fp), "+v"(lm));
+    u16x2 q[H];
 #pragma unroll
-  for (int i = 0; i < H; i++) {
-    const int e = ((i & ~lm) << 1) | (i & lm);         // i with a 0 inserted at bit j
-    const uint2 xv = st.x[e ^ f];
-    const u16x2 x01 = as16(xv.x), x23 = as16(xv.y);
-    const uint4 m = pm[e];
-    q[i] = x01.xx * as16(m.x) + x01.yy * as16(m.y) + x23.xx * as16(m.z) + x23.yy * as16(m.w);
+    for (int i = 0; i < H; i++) {
+      const int e = ((i & ~lm) << 1) | (i & lm);         // i with a 0 inserted at bit j
+      const uint2 xv = st.x[e ^ fp];
+      const u16x2 x01 = as16(xv.x), x23 = as16(xv.y);
+      const uint4 m = pm[e];
+      q[i] = x01.xx * as16(m.x) + x01.yy * as16(m.y) + x23.xx * as16(m.z) + x23.yy * as16(m.w);
+    }
+#pragma unroll
+    for (int h = 1; h < H; h <<= 1)
+#pragma unroll
+      for (int i = 0; i < H; i++)
+        if (!(i & h)) {
+          const u16x2 u = q[i], w = q[i | h];
+          q[i] = u + w;
+          q[i | h] = u - w;
+        }
+    uint32_t o = 0;
+#pragma unroll
+    for (int i = 0; i < H; i++) o |= bits(q[i]);
+    const uint32_t acc = (uint32_t)wuni<NQ>((int)gor<GS>((o & 0xffffu) | (o >> 16)));
+    npass++;
+    const int vp = (pass ? 1 : 2) + 2 * ctz16(acc);    // pass 1: 1 + 2 v2 of (b - d | b + d); pass 0: 2 + 2 v2 of (a | c)
+    v = vp < v ? vp : v;
+    T = top - v > Tlo ? top - v : Tlo;
+    exact = pass == 0 || v <= 2;
+    if (exact || T > tmax) break;
   }
-#pragma unroll
-  for (int h = 1; h < H; h <<= 1)
-#pragma unroll
-    for (int i = 0; i < H; i++)
-      if (!(i & h)) {
-        const u16x2 u = q[i], v = q[i | h];
-        q[i] = u + v;
-        q[i | h] = u - v;
-      }
-  uint32_t o = 0;
-#pragma unroll
-  for (int i = 0; i < H; i++) o |= bits(q[i]);
-  return gor<GS>((o & 0xffffu) | (o >> 16));
+  __builtin_amdgcn_wave_barrier();
+  return T;
 }
 
 template <int NQ> __device__ __forceinline__ int t_bound_any(int4 s, int k, int le) {

@@ -256,3 +256,19 @@ def test_invariances_the_expansion_carries(oracle_lib):
                 if (mask & A) == 0 or (mask & ~A) == 0:
                     assert L.orc_bound(n1, k1, b1, 4, A) == r, (seq, names[g], A)
     assert changed > 0
+
+
+def test_alive_budget_closed_form():
+    """the expansion decides "alive at T bound t" by t <= rem - h - max(c2, supp - rem) instead of evaluating the
+    bound combination (src/cheese.cu:599-607: the minimum over c in [c2, max(c2, supp)] of c + max(h + t,
+    supp - 2c)) against rem; the two agree for every h, t, c2, supp and rem >= 0 in range"""
+    def combine(h, t, c2, supp):
+        return min(c + max(h + t, supp - 2 * c) for c in range(c2, max(c2, supp) + 1))
+
+    for h in range(12):
+        for c2 in range(8):
+            for supp in range(8):
+                for rem in range(16):
+                    budget = rem - h - max(c2, supp - rem)
+                    for t in range(24):
+                        assert (combine(h, t, c2, supp) <= rem) == (t <= budget), (h, t, c2, supp, rem)

@@ -1787,11 +1787,13 @@ int chz_open(const char *spec_text, size_t len, const chz_opts *opts, chz_ctx **
     for (auto &sc : c->sched) sc.var = 0;
   else
     // n = 3 defaults per level-size bucket (log2 of the parents in a launch), from the autotuner's winners on the
-    // Toffoli budget-12 and budget-13 proofs on MI355X: 8K-32K parents run one parent's candidates in two lane
-    // groups at 6 waves per SIMD, mid-size levels want fewer blocks, the largest more
+    // Toffoli budget-12 and budget-13 proofs on MI355X: up to 32K parents run at 4 waves per SIMD (the variant with
+    // the most registers), below 8K parents with a parent's candidates split over lane groups down to about 2,048
+    // groups per launch, 8K-32K in two lane groups per parent; mid-size levels want fewer blocks, the largest more
     for (int b = 0; b < (int)(sizeof c->sched / sizeof c->sched[0]); b++) {
       chz_ctx::Sched &sc = c->sched[b];
-      if (b == 13 || b == 14) { sc.var = 2; sc.cap = 4096; sc.tw = 16384; }
+      if (b == 13 || b == 14) { sc.var = 0; sc.cap = 4096; sc.tw = 16384; }
+      else if (b <= 12) { sc.var = 0; sc.tw = 2048; }
       else if (b >= 15 && b <= 17) sc.cap = 4096;
       else if (b >= 19) sc.cap = 16384;
     }

@@ -213,15 +213,12 @@ __global__ __launch_bounds__(RS_THREADS) void k_radix_hist(const uint32_t *keys,
   hist[(long long)threadIdx.x * nblocks + blockIdx.x] = h[threadIdx.x];
 }
 
-__global__ __launch_bounds__(RS_THREADS) void k_radix_scatter(const uint32_t *kin, const uint32_t *vin, long long n,
-                                                              int shift, const unsigned long long *hist,
-                                                              long long nblocks, uint32_t *kout, uint32_t *vout,
-                                                              const unsigned long long *nd = nullptr) {
-  n = dev_n(n, nd);
-  __shared__ unsigned long long base[256];
+// this block's tile of (key, value) pairs placed from base[d], the first output position of digit d for the tile
+// (shared; advanced as the rounds go)
+__device__ __forceinline__ void radix_scatter_tile(const uint32_t *kin, const uint32_t *vin, long long n, int shift,
+                                                   unsigned long long *base, uint32_t *kout, uint32_t *vout) {
   __shared__ unsigned int wc[RS_THREADS / 64][256];
   const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
-  base[tid] = hist[(long long)tid * nblocks + blockIdx.x];
   for (int k = 0; k < RS_THREADS / 64; k++) wc[k][tid] = 0;
   __syncthreads();
   const long long t0 = (long long)blockIdx.x * RS_TILE;
@@ -253,6 +250,77 @@ __global__ __launch_bounds__(RS_THREADS) void k_radix_scatter(const uint32_t *ki
     base[tid] += tot;
     __syncthreads();
   }
+}
+
+__global__ __launch_bounds__(RS_THREADS) void k_radix_scatter(const uint32_t *kin, const uint32_t *vin, long long n,
+                                                              int shift, const unsigned long long *hist,
+                                                              long long nblocks, uint32_t *kout, uint32_t *vout,
+                                                              const unsigned long long *nd = nullptr) {
+  n = dev_n(n, nd);
+  __shared__ unsigned long long base[256];
+  base[threadIdx.x] = hist[(long long)threadIdx.x * nblocks + blockIdx.x];
+  radix_scatter_tile(kin, vin, n, shift, base, kout, vout);
+}
+
+// Sorts of up to RS_DIRECT_MAX tiles skip the scan of the histogram: k_radix_count writes each tile's digit counts
+// (cnt[tile * 256 + d], 32-bit) and every scatter block forms its own 256 start positions from them, the counts of
+// all smaller digits plus those of its digit in earlier tiles.  Two launches per pass instead of six (count, three
+// scan kernels, the scan's total, scatter); each block reads the whole count table, at most 512 KB from L2.
+constexpr long long RS_DIRECT_MAX = 512;
+
+__global__ __launch_bounds__(RS_THREADS) void k_radix_count(const uint32_t *keys, long long n, int shift,
+                                                            uint32_t *cnt, const unsigned long long *nd = nullptr) {
+  n = dev_n(n, nd);
+  __shared__ unsigned int h[256];
+  h[threadIdx.x] = 0;
+  __syncthreads();
+  const long long t0 = (long long)blockIdx.x * RS_TILE;
+  for (int r = 0; r < RS_ROUNDS; r++) {
+    const long long x = t0 + (long long)r * RS_THREADS + threadIdx.x;
+    if (x < n) atomicAdd(&h[(keys[x] >> shift) & 255], 1u);
+  }
+  __syncthreads();
+  cnt[(long long)blockIdx.x * 256 + threadIdx.x] = h[threadIdx.x];
+}
+
+__global__ __launch_bounds__(RS_THREADS) void k_radix_scatter_direct(const uint32_t *kin, const uint32_t *vin,
+                                                                     long long n, int shift, const uint32_t *cnt,
+                                                                     long long nblocks, uint32_t *kout,
+                                                                     uint32_t *vout,
+                                                                     const unsigned long long *nd = nullptr) {
+  n = dev_n(n, nd);
+  __shared__ unsigned long long base[256];
+  __shared__ unsigned long long wsum[RS_THREADS / 64];
+  const int d = threadIdx.x, lane = d & 63, w = d >> 6;
+  // digit d: its count over all tiles and over the tiles before this one (coalesced rows of 256 counts)
+  unsigned long long tot = 0, below = 0;
+  const long long me = blockIdx.x;
+  long long t = 0;
+  for (; t + 4 <= nblocks; t += 4) {
+    const uint32_t c0 = cnt[t * 256 + d], c1 = cnt[(t + 1) * 256 + d], c2 = cnt[(t + 2) * 256 + d],
+                   c3 = cnt[(t + 3) * 256 + d];
+    tot += (unsigned long long)c0 + c1 + c2 + c3;
+    below += (t < me ? c0 : 0u) + (t + 1 < me ? c1 : 0u) + (t + 2 < me ? c2 : 0u) + (t + 3 < me ? c3 : 0u);
+  }
+  for (; t < nblocks; t++) {
+    const uint32_t c = cnt[t * 256 + d];
+    tot += c;
+    if (t < me) below += c;
+  }
+  // exclusive scan of the digit totals over the 256 digits
+  unsigned long long inc = tot;
+#pragma unroll
+  for (int k = 1; k < 64; k <<= 1) {
+    const unsigned long long y = __shfl_up(inc, k, 64);
+    if (lane >= k) inc += y;
+  }
+  if (lane == 63) wsum[w] = inc;
+  __syncthreads();
+  unsigned long long off = 0;
+  for (int k = 0; k < w; k++) off += wsum[k];
+  base[d] = off + inc - tot + below;
+  __syncthreads();
+  radix_scatter_tile(kin, vin, n, shift, base, kout, vout);
 }
 
 
@@ -321,6 +389,16 @@ static inline hipError_t radix_sort_pairs(uint32_t *k0, uint32_t *v0, uint32_t *
   if (n <= 1) return hipSuccess;
   const long long nb = (n + RS_TILE - 1) / RS_TILE;
   uint32_t *ks[2] = {k0, k1}, *vs[2] = {v0, v1};
+  if (nb <= RS_DIRECT_MAX) {
+    uint32_t *cnt = reinterpret_cast<uint32_t *>(hist);     // nb * 256 counts fit the histogram's space
+    for (int pass = 0; pass < 4; pass++) {
+      const int a = pass & 1, b = a ^ 1;
+      hipLaunchKernelGGL(k_radix_count, dim3((unsigned)nb), dim3(RS_THREADS), 0, s, ks[a], n, pass * 8, cnt, nd);
+      hipLaunchKernelGGL(k_radix_scatter_direct, dim3((unsigned)nb), dim3(RS_THREADS), 0, s, ks[a], vs[a], n,
+                         pass * 8, cnt, nb, ks[b], vs[b], nd);
+    }
+    return hipGetLastError();
+  }
   for (int pass = 0; pass < 4; pass++) {
     const int a = pass & 1, b = a ^ 1;
     hipLaunchKernelGGL(k_radix_hist, dim3((unsigned)nb), dim3(RS_THREADS), 0, s, ks[a], n, pass * 8, hist, nb, nd);

@@ -46,7 +46,7 @@ struct ExpandArgs {
   uint64_t *keys;           // [ncc] class keys of kept children
   uint32_t *info;           // [ncc] bound words of kept children
   const uint32_t *pb;       // parents' bound words
-  unsigned long long *cnt;  // global counters (stage counts C_EVAL, C_CHEAP, C_CUTS, C_NEEDT, C_KEYS)
+  unsigned long long *cnt;  // global counters (stage counts C_EVAL, C_CHEAP, C_CUTS, C_NEEDT, C_KEYS, C_TPASS)
   const uint32_t *pinst;    // instance slot of every parent of the level (null: one instance, slot 0)
   const int *rem;           // per slot: gate budget left after this level's gate (depth - t)
   const uint8_t *keepi;     // per slot: this level's survivors become parents (t < depth)
@@ -378,10 +378,6 @@ struct FlagKept {       // kept survivors of instances still searching
   __device__ bool operator()(long long i) const {
     return kept[i] != 0 && active[pinst ? pinst[surv_o[i] / (uint64_t)G] : 0] != 0;
   }
-};
-struct FlagNeed {       // children that need the T bound, in child order
-  const uint8_t *status;
-  __device__ bool operator()(long long i) const { return status[i] == ST_NEEDT; }
 };
 
 // survivors of a chunk -> level survivor list (child index in the level, class key)

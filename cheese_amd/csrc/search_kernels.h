@@ -13,7 +13,7 @@ template <int NQ> __device__ __forceinline__ int wuni(int v) {
   else return v;
 }
 
-enum : uint8_t { ST_DEAD = 0, ST_ALIVE = 1, ST_FOUND = 2, ST_W2 = 3, ST_W3 = 4, ST_NEEDT = 5, ST_OVF = 6 };
+enum : uint8_t { ST_DEAD = 0, ST_ALIVE = 1, ST_FOUND = 2, ST_W2 = 3, ST_W3 = 4, ST_OVF = 6 };
 constexpr uint32_t INFO_NOBOUND = 1u << 31;    // spec without bounds: every child is kept
 
 // x * w^p for a coefficient vector of int32 values (entries below 2^28 in magnitude: no overflow)

@@ -60,6 +60,13 @@ struct KSpec {                                 // device view of a spec (kernel 
   uint32_t maglim;
 };
 
+// Index of the calling wave in the grid (blocks of a multiple of 64 threads), read through readfirstlane: the
+// compiler then treats it, and the loop bounds and indices derived from it, as wave-uniform scalars instead of
+// per-lane vector values.
+__device__ __forceinline__ long long wave_index() {
+  return (long long)blockIdx.x * (blockDim.x >> 6) + __builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6));
+}
+
 // ---- lane groups: GS lanes (a power of two dividing 64) cooperate on one matrix ----
 template <int GS> __device__ __forceinline__ uint64_t gballot(bool p) {
   const uint64_t b = __ballot(p);

@@ -102,27 +102,35 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(WPE))) void
   const int r = le >> NQ, c = le & (N - 1);
   const TStage st{stage_x + (threadIdx.x & ~(GS - 1)), stage_p + (threadIdx.x & ~(GS - 1))};
   int4 *P = spar + (threadIdx.x & ~(GS - 1));
-  const long long wave = ((long long)blockIdx.x * blockDim.x + threadIdx.x) >> 6;
+  const long long wave = wave_index();
   const long long nwaves = ((long long)gridDim.x * blockDim.x) >> 6;
   const int G = sp.G;
-  // per-instance tallies live in each group's leader lane and are flushed when the group's instance changes
-  // (32-bit: one wave's share of a launch is far below 2^32 children).  Every lane of a group holds its group's
-  // tallies, updated outside lane-divergent code, so for n = 3 (one group per wave) they stay wave-uniform and
-  // live in scalar registers.
-  uint32_t n_w2 = 0, n_w3 = 0, n_dead = 0, n_found = 0, n_ovf = 0, n_alive = 0, n_eval = 0, n_t = 0;
-  uint32_t n_cheap = 0, n_cuts = 0, n_keys = 0, n_tp = 0;
+  // per-instance status tallies: one LDS counter per status value and group, bumped by the group's leader lane
+  // and flushed when the group's instance changes (32-bit: one wave's share of a launch is far below 2^32
+  // children).  Held in registers they cost two vector or scalar instructions per tally and child, and the
+  // scalar registers they take push others into spill lanes.
+  __shared__ uint32_t stally[256 / GS][8];
+  uint32_t *const tl = stally[threadIdx.x / GS];
+  for (int i = le; i < 8; i += GS) tl[i] = 0;
+  __builtin_amdgcn_wave_barrier();
+  uint32_t n_eval = 0, n_t = 0, n_cheap = 0, n_cuts = 0, n_keys = 0, n_tp = 0;
   int cur = -1;
   auto flush = [&]() {
+    __builtin_amdgcn_wave_barrier();
     if (le == 0 && cur >= 0) {
       unsigned long long *ic = a.icnt + (size_t)cur * IC_N;
-      if (n_w2) atomicAdd(&ic[IC_W2], (unsigned long long)n_w2);
-      if (n_w3) atomicAdd(&ic[IC_W3], (unsigned long long)n_w3);
-      if (n_dead) atomicAdd(&ic[IC_DEAD], (unsigned long long)n_dead);
-      if (n_found) atomicAdd(&ic[IC_FOUND], (unsigned long long)n_found);
-      if (n_ovf) atomicAdd(&ic[IC_OVF], (unsigned long long)n_ovf);
-      if (n_alive) atomicAdd(&ic[IC_ALIVE], (unsigned long long)n_alive);
+      const uint32_t w2 = tl[ST_W2], w3 = tl[ST_W3], dead = tl[ST_DEAD], found = tl[ST_FOUND], ovf = tl[ST_OVF],
+                     alive = tl[ST_ALIVE];
+      if (w2) atomicAdd(&ic[IC_W2], (unsigned long long)w2);
+      if (w3) atomicAdd(&ic[IC_W3], (unsigned long long)w3);
+      if (dead) atomicAdd(&ic[IC_DEAD], (unsigned long long)dead);
+      if (found) atomicAdd(&ic[IC_FOUND], (unsigned long long)found);
+      if (ovf) atomicAdd(&ic[IC_OVF], (unsigned long long)ovf);
+      if (alive) atomicAdd(&ic[IC_ALIVE], (unsigned long long)alive);
     }
-    n_w2 = n_w3 = n_dead = n_found = n_ovf = n_alive = 0;
+    __builtin_amdgcn_wave_barrier();
+    for (int i = le; i < 8; i += GS) tl[i] = 0;
+    __builtin_amdgcn_wave_barrier();
   };
   int gsplit = a.gsplit;
   if (a.npd) {
@@ -135,8 +143,10 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(WPE))) void
   for (long long base = wave * CPW; base < nitems; base += nwaves * CPW) {
     const long long item = base + slot;
     if (item >= nitems) continue;
-    const long long pl = item / gsplit;
-    const int gbeg = (int)(item % gsplit) * cs, gend = gbeg + cs < G ? gbeg + cs : G;
+    // the 64-bit division runs on the vector unit; readfirstlane keeps the parent and the candidate range (the
+    // child loop's bounds) scalar for n = 3
+    const long long pl = wuni64<NQ>(item / gsplit);
+    const int gbeg = wuni<NQ>((int)(item - pl * gsplit) * cs), gend = gbeg + cs < G ? gbeg + cs : G;
     const long long pi = a.off + pl;
     const int inst = wuni<NQ>(a.pinst ? (int)a.pinst[pi] : 0);
     if (inst != cur) { flush(); cur = inst; }
@@ -326,13 +336,8 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(WPE))) void
           }
         }
       }
-      n_w2 += stt == ST_W2;
-      n_w3 += stt == ST_W3;
-      n_dead += stt == ST_DEAD;
-      n_ovf += stt == ST_OVF;
-      n_alive += stt == ST_ALIVE;
-      n_found += stt == ST_FOUND;
       if (le == 0) {
+        atomicAdd(&tl[stt], 1u);
         a.status[cc] = stt;
         if (stt == ST_ALIVE && keep) { a.keys[cc] = key; a.info[cc] = bw; }
         if (stt == ST_FOUND)
@@ -501,7 +506,7 @@ void k_dedupe_runs(KSpec sp, DedupeArgs a) {
   __syncthreads();
   const int lane = threadIdx.x & 63, le = lane & (GS - 1), slot = lane / GS;
   const int grp = (threadIdx.x >> 6) * CPW + slot;
-  const long long wave = ((long long)blockIdx.x * blockDim.x + threadIdx.x) >> 6;
+  const long long wave = wave_index();
   const long long nwaves = ((long long)gridDim.x * blockDim.x) >> 6;
   const long long nruns = (long long)a.cnt[C_RUNS];
   const long long n = dev_n(a.n, a.nd);
@@ -615,7 +620,7 @@ template <int NQ>
 __global__ __launch_bounds__(256) void k_gather(KSpec sp, GatherArgs a) {
   constexpr int N = 1 << NQ, E = N * N, GS = E, CPW = 64 / GS;
   const int lane = threadIdx.x & 63, le = lane & (GS - 1), slot = lane / GS;
-  const long long wave = ((long long)blockIdx.x * blockDim.x + threadIdx.x) >> 6;
+  const long long wave = wave_index();
   const long long nwaves = ((long long)gridDim.x * blockDim.x) >> 6;
   const long long nk = dev_n(a.nk, a.nkd);
   const int tn = a.tn;

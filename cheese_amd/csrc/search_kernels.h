@@ -12,6 +12,12 @@ template <int NQ> __device__ __forceinline__ int wuni(int v) {
   if constexpr (NQ == 3) return __builtin_amdgcn_readfirstlane(v);
   else return v;
 }
+template <int NQ> __device__ __forceinline__ long long wuni64(long long v) {
+  if constexpr (NQ == 3)
+    return (long long)(((uint64_t)(uint32_t)__builtin_amdgcn_readfirstlane((int)(v >> 32)) << 32) |
+                       (uint32_t)__builtin_amdgcn_readfirstlane((int)v));
+  else return v;
+}
 
 enum : uint8_t { ST_DEAD = 0, ST_ALIVE = 1, ST_FOUND = 2, ST_W2 = 3, ST_W3 = 4, ST_OVF = 6 };
 constexpr uint32_t INFO_NOBOUND = 1u << 31;    // spec without bounds: every child is kept

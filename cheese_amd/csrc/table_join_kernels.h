@@ -37,7 +37,7 @@ template <int NQ> __global__ __launch_bounds__(256) void k_node_keys(KSpec sp, c
                                                                     long long m, uint64_t *keys) {
   constexpr int E = 1 << (2 * NQ), GS = E, CPW = 64 / GS;
   const int lane = threadIdx.x & 63, le = lane & (GS - 1);
-  const long long wave = ((long long)blockIdx.x * blockDim.x + threadIdx.x) >> 6;
+  const long long wave = wave_index();
   const long long nwaves = ((long long)gridDim.x * blockDim.x) >> 6;
   for (long long b = wave * CPW; b < m; b += nwaves * CPW) {       // group-uniform trip count
     const long long j = b + lane / GS;
@@ -70,7 +70,7 @@ template <int NQ> __global__ __launch_bounds__(256) void k_gt_filter(KSpec sp, G
                                                                     const int4 *res, const int *pk, uint8_t *kept) {
   constexpr int E = 1 << (2 * NQ), GS = E, CPW = 64 / GS;
   const int lane = threadIdx.x & 63, le = lane & (GS - 1);
-  const long long wave = ((long long)blockIdx.x * blockDim.x + threadIdx.x) >> 6;
+  const long long wave = wave_index();
   const long long nwaves = ((long long)gridDim.x * blockDim.x) >> 6;
   const uint64_t G = (uint64_t)sp.G;
   for (long long b = wave * CPW; b < n; b += nwaves * CPW) {
@@ -105,7 +105,7 @@ template <int NQ> __global__ __launch_bounds__(256) void k_gt_probe(KSpec sp, GT
                                                                    unsigned long long *best) {
   constexpr int E = 1 << (2 * NQ), GS = E, CPW = 64 / GS;
   const int lane = threadIdx.x & 63, le = lane & (GS - 1);
-  const long long wave = ((long long)blockIdx.x * blockDim.x + threadIdx.x) >> 6;
+  const long long wave = wave_index();
   const long long nwaves = ((long long)gridDim.x * blockDim.x) >> 6;
   for (long long b = wave * CPW; b < m; b += nwaves * CPW) {
     const long long j = b + lane / GS;
@@ -163,7 +163,7 @@ template <int NQ> __global__ __launch_bounds__(256) void k_mitm_join(KSpec sp, G
                                                                     unsigned long long *ovf) {
   constexpr int E = 1 << (2 * NQ), GS = E, CPW = 64 / GS;
   const int lane = threadIdx.x & 63, le = lane & (GS - 1);
-  const long long wave = ((long long)blockIdx.x * blockDim.x + threadIdx.x) >> 6;
+  const long long wave = wave_index();
   const long long nwaves = ((long long)gridDim.x * blockDim.x) >> 6;
   for (long long b = wave * CPW; b < m; b += nwaves * CPW) {
     const long long j = b + lane / GS;

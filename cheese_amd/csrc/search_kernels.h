@@ -497,14 +497,16 @@ template <int CTRL> __device__ __forceinline__ uint64_t dpp64(uint64_t v) {
 // class_key for n = 3 (one matrix per wave64, whole wave active), at most 16 symmetries: the 16 per-lane
 // hashes are reduce-scattered over the wave (lane ^ 32, ^ 16 by permlane swaps; ^ 15, ^ 7 by DPP mirrors; then
 // sums over each quad), so lane L ends with the wave sum of slot (L >> 2) & 15; one mix per lane and a min over
-// lane bits 2-5 give the same key as class_key.  spos[y * 64 + le]: position of entry le under symmetry y;
-// cjm bit y: symmetry y conjugates.
+// lane bits 2-5 give the same key as class_key.  spos[y * 64 + le]: position of entry le under symmetry y (any
+// entry index for y >= nsym: those slots are hashed too, and their lanes are left out of the final min);
+// cjm bit y: symmetry y conjugates.  The conjugation bits are tested on a vector copy of cjm: tested on the
+// scalar they become 16 loop-invariant lane masks that the compiler hoists out of the child loop, where they
+// spill.
 __device__ __forceinline__ uint64_t class_key3(int4 s, int k, int lane, const uint8_t *spos, uint32_t cjm, int nsym) {
   const uint32_t hv = entry_hash32(s), hc = entry_hash32(zconj(s));
-  auto slot = [&](int y) -> uint64_t {
-    if (y >= nsym) return 0;
-    return slot64(((cjm >> y) & 1) ? hc : hv, (uint32_t)spos[y * 64 + lane] + 1);
-  };
+  uint32_t cjv = cjm;
+  asm volatile("" : "+v"(cjv));
+  auto slot = [&](int y) -> uint64_t { return slot64(((cjv >> y) & 1) ? hc : hv, (uint32_t)spos[y * 64 + lane] + 1); };
   uint64_t h[8];      // slots y and y + 8 are combined as soon as they are hashed: 8 live values, not 16
 #pragma unroll
   for (int j = 0; j < 8; j++) h[j] = rs32(slot(j), slot(j + 8));

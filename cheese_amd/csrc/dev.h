@@ -135,9 +135,12 @@ template <int GS, typename T> __device__ __forceinline__ T gor(T v) {
 __device__ __forceinline__ bool zzero(int4 v) { return !(v.x | v.y | v.z | v.w); }
 __device__ __forceinline__ bool zeq(int4 u, int4 v) { return u.x == v.x && u.y == v.y && u.z == v.z && u.w == v.w; }
 __device__ __forceinline__ bool zhalf(int4 v) { return !((v.x ^ v.z) & 1) && !((v.y ^ v.w) & 1); }   // sqrt2 | v
-__device__ __forceinline__ int4 zdown(int4 v) {          // v / sqrt2 (exact when zhalf)
-  return make_int4((int)(((long long)v.y - v.w) >> 1), (int)(((long long)v.x + v.z) >> 1),
-                   (int)(((long long)v.y + v.w) >> 1), (int)(((long long)v.z - v.x) >> 1));
+// v / sqrt2 = ((b - d) / 2, (a + c) / 2, (b + d) / 2, (c - a) / 2), exact when zhalf (a = c, b = d mod 2).  In 32
+// bits without overflow: with floor halves, (a + c) / 2 = (a >> 1) + (c >> 1) + (a & 1) and (c - a) / 2 =
+// (c >> 1) - (a >> 1) when a and c have equal parity (the same as the int64 sums shifted, for every int32 input).
+__device__ __forceinline__ int4 zdown(int4 v) {
+  const int a = v.x >> 1, b = v.y >> 1, c = v.z >> 1, d = v.w >> 1;
+  return make_int4(b - d, a + c + (v.x & 1), b + d + (v.y & 1), c - a);
 }
 __device__ __forceinline__ int4 zconj(int4 v) { return make_int4(v.x, -v.w, -v.z, -v.y); }
 

@@ -188,3 +188,36 @@ def test_frontier_bound_words_equal_the_oracle(oracle_lib, name):
         b.close()
     print(f"case {name}: {checked_t} T bounds and {checked_c} cut-need sets equal to the oracle's", flush=True)
     assert checked_t >= 200 and checked_c >= 200
+
+
+def _one_qubit_case(name):
+    """(circuit, spec at budget = length): seeded random 1-qubit Clifford+T words, or (H T)^k, whose shortest
+    circuit is the word itself"""
+    from cheese_amd import specgen
+    kind, v = name.split("-")
+    if kind == "ht":
+        circ = [("h", (0,)), ("t", (0,))] * int(v)
+    else:
+        circ = workloads.random_circuit(int(v), 1, 8, 14, gates=("x", "z", "h", "s", "sdg", "t", "tdg"))
+    return circ, specgen.spec_text(1, circ, len(circ))
+
+
+@pytest.mark.parametrize("name", ["rand-2", "rand-3", "rand-5", "ht-5", "ht-8"])
+def test_one_qubit_search_matches_the_oracle(oracle_lib, name):
+    """the n = 1 instantiation of the level kernels (lane groups of 4 entries, 16 groups per wave, per-group status
+    tallies): level lines and result line equal the oracle's (src/cheese.cu:1249-1377) at budget = length, and at
+    one gate below the circuit found, where the search proves the budget impossible"""
+    from conftest import oracle_run
+    circ, spec = _one_qubit_case(name)
+    print(f"case n1 {name} length {len(circ)}", flush=True)
+    out = cheese_amd.search(spec)
+    rc, ref = oracle_run(oracle_lib, spec)
+    assert out["rc"] == rc == 0 and out["verified"]
+    assert strip_seconds(_lines(out)) == strip_seconds(ref)
+    d = len(out["circuit"]) - 1
+    assert d >= 1
+    spec_d = spec.replace(f"\ndepth {len(circ)}\n", f"\ndepth {d}\n")
+    out = cheese_amd.search(spec_d)
+    rc, ref = oracle_run(oracle_lib, spec_d)
+    assert out["rc"] == rc == 1
+    assert strip_seconds(_lines(out)) == strip_seconds(ref)

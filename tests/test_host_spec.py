@@ -175,3 +175,23 @@ def test_t_bound_classes_share_the_oracle_bound(shim, oracle_lib):
             assert tbound(prefix + [a]) == tbound(prefix + [b])
         differ += tbound(prefix + [names.index("L_t_0")]) != tbound(prefix + [names.index("L_t_1")])
     assert differ > 0
+
+
+def test_zdown_halves_in_32_bits():
+    """the device's division by sqrt2 (dev.h zdown) forms (a + c) / 2 and (c - a) / 2 of int32 coefficients of equal
+    parity as (a >> 1) + (c >> 1) + (a & 1) and (c >> 1) - (a >> 1), with no 64-bit sum: equal to the exact halves on
+    every pair of extreme values and on random ones, and within int32"""
+    import random
+    lo, hi = -(1 << 31), (1 << 31) - 1
+    edge = [lo, lo + 1, lo + 2, -3, -2, -1, 0, 1, 2, 3, hi - 2, hi - 1, hi]
+    rnd = random.Random(11)
+    vals = edge + [rnd.randint(lo, hi) >> rnd.randint(0, 31) for _ in range(4000)]
+    n = 0
+    for a in vals[:200] if len(vals) > 200 else vals:
+        for c in vals:
+            if (a ^ c) & 1:
+                continue
+            s, d = (a >> 1) + (c >> 1) + (a & 1), (c >> 1) - (a >> 1)
+            assert s == (a + c) // 2 and d == (c - a) // 2 and lo <= s <= hi and lo <= d <= hi, (a, c)
+            n += 1
+    assert n > 100000
